@@ -1,0 +1,107 @@
+// bia_runtime.hip -- context lifetime, error strings, the default context of the legacy ABI.
+#include "bia_runtime.hpp"
+
+namespace bia {
+
+static bia_ctx* g_default = nullptr;
+static std::mutex g_default_mu;
+
+bia_ctx* default_ctx() {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (!g_default) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    bia_ctx* c = nullptr;
+    int s = bia_create(&c, dev);
+    if (s != BIA_OK) legacy_fail("bia default context", s, c);
+    g_default = c;
+  }
+  return g_default;
+}
+
+void legacy_fail(const char* where, int code, bia_ctx* ctx) {
+  fprintf(stderr, "libbia_hip: %s failed: %s%s%s\n", where, bia_error_string(code),
+          (ctx && ctx->last_error[0]) ? " -- " : "", (ctx && ctx->last_error[0]) ? ctx->last_error : "");
+  fflush(stderr);
+  abort();
+}
+
+int ensure_tiles(bia_ctx* ctx, int nt) {
+  if (ctx->tiles_nt == nt) return BIA_OK;
+  const size_t ntiles = (size_t)nt * (nt + 1) / 2;
+  BIA_TRY(ctx, ctx->tiles.ensure(ntiles * sizeof(int2)));
+  int2* h = (int2*)malloc(ntiles * sizeof(int2));
+  if (!h) return BIA_ERR_OOM;
+  size_t k = 0;
+  for (int I = 0; I < nt; I++)
+    for (int J = I; J < nt; J++) h[k++] = make_int2(I, J);
+  hipError_t e = hipMemcpy(ctx->tiles.p, h, ntiles * sizeof(int2), hipMemcpyHostToDevice);
+  free(h);
+  BIA_HIP_TRY(ctx, e);
+  ctx->tiles_nt = nt;
+  return BIA_OK;
+}
+
+}  // namespace bia
+
+extern "C" {
+
+int bia_version(void) { return 10000; }
+
+const char* bia_error_string(int status) {
+  switch (status) {
+    case BIA_OK: return "ok";
+    case BIA_ERR_ARG: return "invalid argument";
+    case BIA_ERR_HIP: return "HIP runtime error";
+    case BIA_ERR_OOM: return "device out of memory";
+    case BIA_ERR_STATE: return "invalid call sequence";
+    case BIA_ERR_RKF: return "RKF45 integration failed";
+    default: return "unknown status";
+  }
+}
+
+const char* bia_last_error(bia_ctx* ctx) { return ctx ? ctx->last_error : ""; }
+
+int bia_create(bia_ctx** out, int device) {
+  if (!out) return BIA_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BIA_ERR_HIP;
+  if (device < 0 || device >= ndev) return BIA_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return BIA_ERR_HIP;
+  bia_ctx* c = new bia_ctx();
+  c->device = device;
+  c->stream = nullptr;
+  *out = c;
+  return BIA_OK;
+}
+
+int bia_destroy(bia_ctx* c) {
+  if (!c) return BIA_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  bia::DevBuf* bufs[] = {&c->part, &c->tiles, &c->state, &c->delta, &c->stats, &c->leg_in, &c->leg_out,
+                         &c->inv_state, &c->inv_part, &c->inv_out, &c->dim_work, &c->dim_ctl,
+                         &c->mlp_w, &c->mlp_ws};
+  for (auto* b : bufs) b->release();
+  delete c;
+  return BIA_OK;
+}
+
+int bia_set_stream(bia_ctx* c, void* stream) {
+  if (!c) return BIA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->stream = static_cast<hipStream_t>(stream);
+  return BIA_OK;
+}
+
+int bia_synchronize(bia_ctx* c) {
+  if (!c) return BIA_ERR_ARG;
+  BIA_HIP_TRY(c, hipSetDevice(c->device));
+  BIA_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return BIA_OK;
+}
+
+void sincos_(double* x, double* s, double* c) { ::sincos(*x, s, c); }
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// bia_runtime.hpp -- library context, error handling and cached device buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <mutex>
+
+#include "../../include/bia_hip.h"
+
+namespace bia {
+
+// grow-only device buffer; reused across calls so steady-state steps allocate nothing
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need) {
+    if (need <= bytes) return BIA_OK;
+    if (p) {
+      if (hipFree(p) != hipSuccess) return BIA_ERR_HIP;
+      p = nullptr;
+      bytes = 0;
+    }
+    if (need == 0) return BIA_OK;
+    if (hipMalloc(&p, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return BIA_ERR_OOM;
+    }
+    bytes = need;
+    return BIA_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+}  // namespace bia
+
+struct bia_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;  // default: the null stream (torch's default stream too)
+  std::mutex mu;                 // one call at a time per context
+  // step workspace
+  bia::DevBuf part, tiles, state, delta, stats;
+  int tiles_nt = -1;
+  // legacy per-row buffers
+  bia::DevBuf leg_in, leg_out;
+  // invariance workspace
+  bia::DevBuf inv_state, inv_part, inv_out;
+  // DIM workspace
+  bia::DevBuf dim_work, dim_ctl;
+  // learned initialiser weights (device copy) and batch workspace
+  bia::DevBuf mlp_w, mlp_ws;
+  int mlp_ready = 0;
+  char last_error[256] = {0};
+};
+
+#define BIA_HIP_TRY(ctx, expr)                                                                \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) {                                                                   \
+      snprintf((ctx)->last_error, sizeof((ctx)->last_error), "%s:%d: %s: %s", __FILE__,       \
+               __LINE__, #expr, hipGetErrorString(e_));                                       \
+      return BIA_ERR_HIP;                                                                     \
+    }                                                                                         \
+  } while (0)
+
+#define BIA_TRY(ctx, expr)                                                                   \
+  do {                                                                                       \
+    int s_ = (expr);                                                                         \
+    if (s_ != BIA_OK) {                                                                      \
+      if (!(ctx)->last_error[0])                                                             \
+        snprintf((ctx)->last_error, sizeof((ctx)->last_error), "%s:%d: %s -> %d", __FILE__, \
+                 __LINE__, #expr, s_);                                                       \
+      return s_;                                                                             \
+    }                                                                                        \
+  } while (0)
+
+namespace bia {
+// default context used by the legacy Fortran-callable symbols (the reference ABI has no handle)
+bia_ctx* default_ctx();
+// abort with a message: the legacy symbols have no error return, so failures are loud
+[[noreturn]] void legacy_fail(const char* where, int code, bia_ctx* ctx);
+// upper-triangle list of 64-wide tile pairs (I <= J) for nt tiles, cached on the context
+int ensure_tiles(bia_ctx* ctx, int nt);
+// learned-initialiser step (bia_learn.hip); caller holds ctx->mu
+int step_learn_locked(bia_ctx* ctx, int n, double eps, double dt, int itr, const double* d_state, int ld,
+                      double* d_delta, unsigned long long* d_stats);
+}  // namespace bia

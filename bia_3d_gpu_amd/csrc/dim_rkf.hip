@@ -1,0 +1,368 @@
+// dim_rkf.hip -- DIM: direct-sum N-body RKF45, the accuracy oracle of the BIA method
+// (reference dim/rkf_kernel.cu:29-53 force, :75-145 Fehlberg stages, :194-495 stepper;
+// driven by dim/dim_3D.f90:30).
+//
+// Device layout: SoA rows x,y,z,u,v,w of stride ld.  One Fehlberg stage is one kernel:
+// thread i evaluates the direct-sum acceleration at the stage input w_{s-1} (j staged through
+// LDS 256 at a time, summed in ascending j like the reference), stores k_s[i] and in the
+// same pass forms particle i's next stage input w_s[i] = y0[i] + h sum_l b_sl k_l[i].  The
+// last stage forms y_n and, for the error-controlled step, the per-block maximum of
+// |err_i| / (relerr (|y0|+|yn|)/2 + abserr), so only nblocks doubles cross PCIe per trial step.
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "bia_pair.hpp"
+#include "bia_runtime.hpp"
+
+namespace bia {
+
+constexpr int kDimBlock = 256;
+
+struct DimBufs {
+  double* y[4];   // yl0, yln, yh0, yhn (swapped by pointer)
+  double* k;      // 6 stages x 6 rows x ld
+  double* w[2];   // ping-pong stage inputs
+  double* er;     // 6 x ld
+  double* bmax;   // per-block error maxima (+1 flag slot per block)
+};
+
+__device__ __forceinline__ void dim_accel(int np, int ld, double eps, const double* __restrict__ w, int i,
+                                          double& ax, double& ay, double& az) {
+  __shared__ double sx[kDimBlock], sy[kDimBlock], sz[kDimBlock];
+  const bool ok = i < np;
+  const double xi = ok ? w[i] : 0, yi = ok ? w[ld + i] : 0, zi = ok ? w[2 * (size_t)ld + i] : 0;
+  ax = ay = az = 0;
+  for (int j0 = 0; j0 < np; j0 += kDimBlock) {
+    const int jl = j0 + threadIdx.x;
+    __syncthreads();
+    if (jl < np) {
+      sx[threadIdx.x] = w[jl];
+      sy[threadIdx.x] = w[ld + jl];
+      sz[threadIdx.x] = w[2 * (size_t)ld + jl];
+    }
+    __syncthreads();
+    const int jn = min(kDimBlock, np - j0);
+    if (ok) {
+      for (int jj = 0; jj < jn; ++jj) {
+        double xij = xi - sx[jj], yij = yi - sy[jj], zij = zi - sz[jj];
+        double rsq = xij * xij + yij * yij + zij * zij + 1e-99;  // rkf_kernel.cu:44
+        double e = eps / (rsq * sqrt(rsq));
+        ax += xij * e;
+        ay += yij * e;
+        az += zij * e;
+      }
+    }
+  }
+}
+
+// stage S (1..6) of one Fehlberg step of size h from y0:
+//   k_S = f(win);  S < 6: wout = y0 + h sum_{l<=S} b_{S+1,l} k_l
+//                  S = 6: yn = y0 + h sum ga_l k_l; ERR: er, block max of |er|/et
+template <int S, bool ERR>
+__global__ __launch_bounds__(kDimBlock) void k_dim_stage(int np, int ld, double eps, double h,
+                                                         const double* __restrict__ y0,
+                                                         const double* __restrict__ win, double* __restrict__ k,
+                                                         double* __restrict__ wout, double* __restrict__ yn,
+                                                         double* __restrict__ er, double* __restrict__ bmax,
+                                                         double relerr, double abserr) {
+  using namespace fehl;
+  const int i = blockIdx.x * kDimBlock + threadIdx.x;
+  double ax, ay, az;
+  dim_accel(np, ld, eps, win, i, ax, ay, az);
+  double emax = 0, bad = 0;
+  if (i < np) {
+    const size_t sl = (size_t)6 * ld;  // stride between stages
+    double* ks = k + (size_t)(S - 1) * sl;
+    ks[i] = win[3 * (size_t)ld + i];
+    ks[ld + i] = win[4 * (size_t)ld + i];
+    ks[2 * (size_t)ld + i] = win[5 * (size_t)ld + i];
+    ks[3 * (size_t)ld + i] = ax;
+    ks[4 * (size_t)ld + i] = ay;
+    ks[5 * (size_t)ld + i] = az;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      const size_t o = (size_t)c * ld + i;
+      const double k1 = k[o], k2 = (S >= 2) ? k[sl + o] : 0, k3 = (S >= 3) ? k[2 * sl + o] : 0,
+                   k4 = (S >= 4) ? k[3 * sl + o] : 0, k5 = (S >= 5) ? k[4 * sl + o] : 0,
+                   k6 = (S >= 6) ? k[5 * sl + o] : 0;
+      const double y = y0[o];
+      if (S == 1) wout[o] = y + h * B21 * k1;  // rkf_kernel.cu:82
+      if (S == 2) wout[o] = y + h * (B31 * k1 + B32 * k2);
+      if (S == 3) wout[o] = y + h * (B41 * k1 + B42 * k2 + B43 * k3);
+      if (S == 4) wout[o] = y + h * (B51 * k1 + B52 * k2 + B53 * k3 + B54 * k4);
+      if (S == 5) wout[o] = y + h * (B61 * k1 + B62 * k2 + B63 * k3 + B64 * k4 + B65 * k5);
+      if (S == 6) {
+        const double ynv = y + h * (GA1 * k1 + GA3 * k3 + GA4 * k4 + GA5 * k5 + GA6 * k6);
+        yn[o] = ynv;
+        if (ERR) {
+          const double ev = h * (DA1 * k1 + DA3 * k3 + DA4 * k4 + DA5 * k5 + DA6 * k6);
+          er[o] = ev;
+          const double et = relerr * (fabs(y) + fabs(ynv)) / 2 + abserr;
+          if (et == 0) bad = 1;
+          else emax = fmax(emax, fabs(ev) / et);
+        }
+      }
+    }
+  }
+  if (S == 6 && ERR) {
+    __shared__ double red[kDimBlock / 64][2];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      emax = fmax(emax, __shfl_xor(emax, off, 64));
+      bad = fmax(bad, __shfl_xor(bad, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      red[threadIdx.x / 64][0] = emax;
+      red[threadIdx.x / 64][1] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = red[0][0], b = red[0][1];
+      for (int q = 1; q < kDimBlock / 64; q++) {
+        m = fmax(m, red[q][0]);
+        b = fmax(b, red[q][1]);
+      }
+      bmax[2 * blockIdx.x] = m;
+      bmax[2 * blockIdx.x + 1] = b;
+    }
+  }
+}
+
+// derivative only (initial step-size estimate): f = (v, a) at y
+__global__ __launch_bounds__(kDimBlock) void k_dim_func(int np, int ld, double eps, const double* __restrict__ y,
+                                                        double* __restrict__ f) {
+  const int i = blockIdx.x * kDimBlock + threadIdx.x;
+  double ax, ay, az;
+  dim_accel(np, ld, eps, y, i, ax, ay, az);
+  if (i < np) {
+    f[i] = y[3 * (size_t)ld + i];
+    f[ld + i] = y[4 * (size_t)ld + i];
+    f[2 * (size_t)ld + i] = y[5 * (size_t)ld + i];
+    f[3 * (size_t)ld + i] = ax;
+    f[4 * (size_t)ld + i] = ay;
+    f[5 * (size_t)ld + i] = az;
+  }
+}
+
+struct DimRun {
+  bia_ctx* ctx;
+  int np, ld, nb;
+  double eps, relerr, abserr;
+  DimBufs b;
+  int evals = 0;
+
+  int launch_fehl(const double* y0, double* yn, double h, bool err) {
+    dim3 g(nb), t(kDimBlock);
+    hipStream_t s = ctx->stream;
+    double *w0 = b.w[0], *w1 = b.w[1];
+    hipLaunchKernelGGL((k_dim_stage<1, false>), g, t, 0, s, np, ld, eps, h, y0, y0, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
+    hipLaunchKernelGGL((k_dim_stage<2, false>), g, t, 0, s, np, ld, eps, h, y0, w0, b.k, w1, nullptr, nullptr, nullptr, relerr, abserr);
+    hipLaunchKernelGGL((k_dim_stage<3, false>), g, t, 0, s, np, ld, eps, h, y0, w1, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
+    hipLaunchKernelGGL((k_dim_stage<4, false>), g, t, 0, s, np, ld, eps, h, y0, w0, b.k, w1, nullptr, nullptr, nullptr, relerr, abserr);
+    hipLaunchKernelGGL((k_dim_stage<5, false>), g, t, 0, s, np, ld, eps, h, y0, w1, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
+    if (err)
+      hipLaunchKernelGGL((k_dim_stage<6, true>), g, t, 0, s, np, ld, eps, h, y0, w0, b.k, nullptr, yn, b.er, b.bmax, relerr, abserr);
+    else
+      hipLaunchKernelGGL((k_dim_stage<6, false>), g, t, 0, s, np, ld, eps, h, y0, w0, b.k, nullptr, yn, nullptr, nullptr, relerr, abserr);
+    evals += 6;
+    BIA_HIP_TRY(ctx, hipGetLastError());
+    return BIA_OK;
+  }
+};
+
+static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double xe, double* y, int* init,
+                            double relerr, double abserr, double* hio, double* ese, int* nde, int* naccept,
+                            int* nreject, int* ier, FILE* trace) {
+  const double EPSMIN = 1e-15;  // rkf_kernel.cu:201-202
+  const int ITEMAX = 10000;
+  if (np < 1 || !x0 || !y || !hio || !ier) return BIA_ERR_ARG;
+  const int ld = ((np + kDimBlock - 1) / kDimBlock) * kDimBlock;
+  const int nb = ld / kDimBlock;
+  const size_t row = (size_t)6 * ld;
+  // layout: yl0 yln yh0 yhn | k (6 stages) | w0 w1 | er | bmax
+  const size_t ndbl = 4 * row + 6 * row + 2 * row + row + 2 * (size_t)nb;
+  BIA_TRY(ctx, ctx->dim_work.ensure(ndbl * sizeof(double)));
+  double* base = ctx->dim_work.as<double>();
+  DimRun R{ctx, np, ld, nb, eps, relerr, abserr, {}, 0};
+  for (int q = 0; q < 4; q++) R.b.y[q] = base + q * row;
+  R.b.k = base + 4 * row;
+  R.b.w[0] = base + 10 * row;
+  R.b.w[1] = base + 11 * row;
+  R.b.er = base + 12 * row;
+  R.b.bmax = base + 13 * row;
+  double *yl0 = R.b.y[0], *yln = R.b.y[1], *yh0 = R.b.y[2], *yhn = R.b.y[3];
+  hipStream_t s = ctx->stream;
+  // AoS host -> SoA device (padding rows zeroed once)
+  std::vector<double> soa(row, 0.0);
+  for (int i = 0; i < np; i++)
+    for (int c = 0; c < 6; c++) soa[(size_t)c * ld + i] = y[(size_t)6 * i + c];
+  BIA_HIP_TRY(ctx, hipMemcpyAsync(yh0, soa.data(), row * sizeof(double), hipMemcpyHostToDevice, s));
+  BIA_HIP_TRY(ctx, hipMemcpyAsync(yl0, yh0, row * sizeof(double), hipMemcpyDeviceToDevice, s));
+  double H = *hio;
+  if (init && *init == 1) {  // initial step from the derivative, :237-265
+    hipLaunchKernelGGL(k_dim_func, dim3(nb), dim3(kDimBlock), 0, s, np, ld, eps, yl0, R.b.k);
+    BIA_HIP_TRY(ctx, hipGetLastError());
+    std::vector<double> f(row);
+    BIA_HIP_TRY(ctx, hipMemcpyAsync(f.data(), R.b.k, row * sizeof(double), hipMemcpyDeviceToHost, s));
+    BIA_HIP_TRY(ctx, hipStreamSynchronize(s));
+    double toldy = 1.7976931348623157e308;
+    bool changed = false;
+    for (int i = 0; i < np; i++)
+      for (int c = 0; c < 6; c++) {
+        const double fv = f[(size_t)c * ld + i];
+        if (fabs(fv) > 0.0) {
+          const double tol = relerr * fabs(y[(size_t)6 * i + c]) + abserr;
+          toldy = fmin(toldy, fabs(tol / fv));
+          changed = true;
+        }
+      }
+    H = changed ? fmin((xe - *x0) / 100e0, pow(toldy, 0.2)) : EPSMIN * (xe - *x0);
+    *init = 2;
+    R.evals = 1;
+  }
+  std::vector<double> bm(2 * (size_t)nb);
+  std::vector<double> p0(6);
+  auto trace_row = [&](double t, const double* dy) -> int {
+    if (!trace) return BIA_OK;
+    for (int c = 0; c < 6; c++)
+      BIA_HIP_TRY(ctx, hipMemcpyAsync(&p0[c], dy + (size_t)c * ld, sizeof(double), hipMemcpyDeviceToHost, s));
+    BIA_HIP_TRY(ctx, hipStreamSynchronize(s));
+    fprintf(trace, "%24.16e%25.16e%25.16e%25.16e%25.16e%25.16e%25.16e\n", t, p0[0], p0[1], p0[2], p0[3], p0[4],
+            p0[5]);
+    return BIA_OK;
+  };
+  if (trace) {
+    fprintf(trace, "%24s%25s%25s%25s%25s%25s%25s\n", "time", "x", "y", "z", "u", "v", "w");
+    BIA_TRY(ctx, trace_row(*x0, yh0));
+  }
+  int acc = 0, rej = 0;
+  *ier = 10000;
+  double X = *x0;
+  for (int iter = 0; iter < ITEMAX; iter++) {
+    const double h = H;
+    BIA_TRY(ctx, R.launch_fehl(yl0, yln, h, true));
+    BIA_HIP_TRY(ctx, hipMemcpyAsync(bm.data(), R.b.bmax, bm.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    BIA_HIP_TRY(ctx, hipStreamSynchronize(s));
+    double erret = 0;
+    bool bad = false;
+    for (int q = 0; q < nb; q++) {
+      erret = fmax(erret, bm[2 * q]);
+      bad = bad || bm[2 * q + 1] != 0;
+    }
+    if (bad) {
+      *ier = 20000;
+      break;
+    }
+    if (erret > 1) {
+      rej++;
+      H *= (erret >= 59049) ? 0.1 : 0.9 / pow(erret, 0.2);
+      if (H <= EPSMIN) {
+        *ier = 30000;
+        break;
+      }
+    } else if (X + H < xe) {
+      // high solution: two half steps, :318-363
+      BIA_TRY(ctx, R.launch_fehl(yh0, yhn, H / 2, false));
+      std::swap(yh0, yhn);
+      BIA_TRY(ctx, R.launch_fehl(yh0, yhn, H / 2, false));
+      X += H;
+      *x0 = X;
+      std::swap(yl0, yln);
+      std::swap(yh0, yhn);
+      acc++;
+      BIA_TRY(ctx, trace_row(X, yh0));
+      H *= (erret <= 1.889568e-4) ? 5.0 : 0.9 / pow(erret, 0.2);
+    } else {
+      H = xe - X;  // last phase, :387-483
+      BIA_TRY(ctx, R.launch_fehl(yl0, yln, H, true));
+      BIA_TRY(ctx, R.launch_fehl(yh0, yhn, H / 2, false));
+      std::swap(yh0, yhn);
+      BIA_TRY(ctx, R.launch_fehl(yh0, yhn, H / 2, false));
+      acc++;
+      *x0 = xe;
+      // results back to AoS host arrays
+      std::vector<double> outh(row), outl(row);
+      BIA_HIP_TRY(ctx, hipMemcpyAsync(outh.data(), yhn, row * sizeof(double), hipMemcpyDeviceToHost, s));
+      BIA_HIP_TRY(ctx, hipMemcpyAsync(outl.data(), yln, row * sizeof(double), hipMemcpyDeviceToHost, s));
+      BIA_HIP_TRY(ctx, hipStreamSynchronize(s));
+      for (int i = 0; i < np; i++)
+        for (int c = 0; c < 6; c++) {
+          const double vh = outh[(size_t)c * ld + i], vl = outl[(size_t)c * ld + i];
+          y[(size_t)6 * i + c] = vh;
+          if (ese) ese[(size_t)6 * i + c] = (vl - vh) / 31;
+        }
+      BIA_TRY(ctx, trace_row(xe, yhn));
+      *ier = 0;
+      break;
+    }
+  }
+  *hio = H;
+  if (nde) *nde = R.evals;
+  if (naccept) *naccept = acc;
+  if (nreject) *nreject = rej;
+  return *ier == 0 ? BIA_OK : BIA_ERR_RKF;
+}
+
+}  // namespace bia
+
+using namespace bia;
+
+extern "C" {
+
+int bia_dim_rkf45_host(bia_ctx* ctx, int np, double eps, double* x0, double xe, double* y, int* init, double relerr,
+                       double abserr, double* h, double* ese, int* nde, int* naccept, int* nreject, int* ier) {
+  if (!ctx) return BIA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->last_error[0] = 0;
+  BIA_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return dim_rkf45_locked(ctx, np, eps, x0, xe, y, init, relerr, abserr, h, ese, nde, naccept, nreject, ier,
+                          nullptr);
+}
+
+// Reference-compatible rkf_c_.  When BIA_DIM_TRACE names a file, the accepted-step trajectory of
+// particle 1 is written there in the reference's text format (its DEBUG output,
+// rkf_kernel.cu:209-216,376-377,476-477).
+void rkf_c_(int neq, double eps_th, double* X0, double* XE, double* YH0, int* INIT, const double* RELERR,
+            const double* ABSERR, double* YHN, double* ESE, int* NDE, int* IER, double* H, double* YL0,
+            double* YLN, double* ERR, double* WRK) {
+  (void)ERR;
+  (void)WRK;
+  bia_ctx* ctx = default_ctx();
+  if (neq < 6 || neq % 6 != 0) {
+    snprintf(ctx->last_error, sizeof(ctx->last_error), "neq=%d", neq);
+    legacy_fail("rkf_c_ arguments", BIA_ERR_ARG, ctx);
+  }
+  const int np = neq / 6;
+  std::vector<double> y(YH0, YH0 + neq);
+  FILE* tr = nullptr;
+  if (const char* p = getenv("BIA_DIM_TRACE")) tr = fopen(p, "w");
+  int ier = 0, nde = 0;
+  int s;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->last_error[0] = 0;
+    if (hipSetDevice(ctx->device) != hipSuccess) legacy_fail("rkf_c_", BIA_ERR_HIP, ctx);
+    s = dim_rkf45_locked(ctx, np, eps_th, X0, *XE, y.data(), INIT, *RELERR, *ABSERR, H, ESE, &nde, nullptr,
+                         nullptr, &ier, tr);
+  }
+  if (tr) fclose(tr);
+  *NDE = nde;
+  *IER = ier;
+  if (s == BIA_ERR_RKF) {
+    // the reference prints the reason and exits with status 0 (rkf_kernel.cu:298-315,491-494)
+    if (ier == 20000) printf("(SUBR -RKF) TROUBLE(INAPPROPRIATE ERROR TOLERANCE)\n");
+    else if (ier == 30000) printf("(SUBR -RKF) TROUBLE(TOO SMALL STEP SIZE) H %25.16e\n", *H);
+    else printf("(SUBR -RKF) TROUBLE(TOO MANY ITERATIONS/ ITER =%d)\n", 10000);
+    fflush(stdout);
+    exit(0);
+  }
+  if (s != BIA_OK) legacy_fail("rkf_c_", s, ctx);
+  (void)YL0;
+  (void)YLN;
+  memcpy(YHN, y.data(), sizeof(double) * neq);
+  memcpy(YH0, y.data(), sizeof(double) * neq);
+}
+
+}  // extern "C"

@@ -1,0 +1,140 @@
+/*
+ * bia_hip.h -- C ABI of libbia_hip.so, the MI355X (gfx950) library for the pairwise hot path
+ * of the Binary Interaction Approximation (BIA) N-body method.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Reference-compatible symbols.  Same names, argument order, by-value/by-reference
+ *     conventions and in-place semantics as the reference's CUDA objects, so the reference
+ *     Fortran host programs (bia/bia.f90, dim/dim_3D.f90) link against this library
+ *     unmodified.  They have no error return (as in the reference): on a HIP failure they
+ *     print the cause and abort.
+ *
+ *  2. New entry points (prefix bia_).  They return an int status (BIA_OK = 0), never exit,
+ *     and run on the context's stream.  Device-pointer variants (_dev) take state already
+ *     resident in HBM in structure-of-arrays layout: six rows x[ld], y[ld], z[ld], u[ld],
+ *     v[ld], w[ld] with row stride ld >= n.  Host variants (_host) copy in and out.
+ *     A context serialises its own calls; use one context per host thread.
+ *
+ * Units and conventions follow the reference: eps is the two-body coupling of the relative
+ * equation of motion r'' = eps r/|r|^3 (eps = -2Gm for equal-mass gravity, bia/bia.f90:37),
+ * dt the step, itr the Newton iteration cap (18 in bia/bia.f90:52).
+ */
+#ifndef BIA_HIP_H
+#define BIA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define BIA_OK 0
+#define BIA_ERR_ARG 1     /* invalid argument */
+#define BIA_ERR_HIP 2     /* HIP runtime error (text via bia_last_error) */
+#define BIA_ERR_OOM 3     /* device allocation failed */
+#define BIA_ERR_STATE 4   /* call sequence error (e.g. learned mode without trained weights) */
+#define BIA_ERR_RKF 5     /* DIM RKF45 failure; the reference's IER code is returned separately */
+
+/* method codes (reference METHOD values, bia/bia.f90:135-139) */
+#define BIA_METHOD_RKF 0  /* per-pair adaptive RKF45 */
+#define BIA_METHOD_ALG 1  /* analytic ellipse / parabola / hyperbola */
+
+/* initialiser modes for the analytic hyperbola branch */
+#define BIA_MODE_PLAIN 0  /* the reference's starting guess and iteration control */
+#define BIA_MODE_LEARN 1  /* learned starting guess + safeguarded Newton, reference guess as fallback */
+
+/* per-step statistics written by the step entry points (uint64 counters, BIA_STATS_LEN long) */
+#define BIA_STATS_LEN 16
+/* [0..3]  pairs per branch (hyperbola, parabola, ellipse, rkf)
+ * [4..7]  summed solver iterations per branch (Newton evaluations / accepted RKF steps)
+ * [8..15] pairs with status bit b set: 1 best-iterate never updated, 2 exact-repeat exit,
+ *         4 iteration cap, 8 acos clamp, 16 RKF failure, 32 non-finite, 64 learned guess used,
+ *         128 learned path fell back to the reference guess */
+
+typedef struct bia_ctx bia_ctx;
+
+/* ---------------------------------------------------------------------------------------- */
+/* 1. reference-compatible symbols                                                          */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Replaces find_dxyuv_kernel_ (bia/find_dXYUV_kernel.cu:526-561), called from
+ * bia/bia.f90:135-139 as
+ *   call find_dxyuv_kernel(%VAL(METHOD), %VAL(N), %VAL(I), %VAL(eps), %VAL(dt), %VAL(ITR),
+ *                          dX, dY, dZ, dU, dV, dW, RXYZUVW, dXYZUVW)
+ * I is 1-based.  RXYZUVW[6j..6j+5] = state_I - state_j (AoS).  Writes dXYZUVW[6j..] for j > I-1
+ * and accumulates them into dX..dW: + at I-1, - at j.  The caller adds the drift. */
+void find_dxyuv_kernel_(int METHOD, int N, int I, double EPS, double DT, int ITR, double *dX, double *dY,
+                        double *dZ, double *dU, double *dV, double *dW, double *RXYZUVW, double *dXYZUVW);
+
+/* Replaces invariance_gpu_ (bia/invariance.cu:50-110), called from bia/bia.f90:67-68.
+ * KE = sum |v|^2, PE = eps * sum_{i<j} 1/r_ij, L = sum r x v, and the plain sums of x..w.
+ * PE is accumulated in double-double (compensated) arithmetic. */
+void invariance_gpu_(int N, double eps_th, double *X, double *Y, double *Z, double *U, double *V, double *W,
+                     double *KE, double *PE, double *Lx, double *Ly, double *Lz, double *Ua, double *Va,
+                     double *Wa, double *Xa, double *Ya, double *Za);
+
+/* Replaces rkf_c_ (dim/rkf_kernel.cu:194-495), called from dim/dim_3D.f90:30.  Direct-sum
+ * adaptive Fehlberg 4(5) with step doubling from *X0 to *XE on the AoS state YH0[neq]
+ * (neq = 6 N).  On return YHN holds the state at *XE, *X0 = *XE, ESE = (YLN - YHN)/31.
+ * H is in/out (initial step set when *INIT == 1).  YL0, YLN, ERR (neq) and WRK (11 neq)
+ * are the caller's work arrays in the reference; the work lives on the device here and they
+ * are left untouched.  On failure *IER = 10000 (too many
+ * steps), 20000 (zero tolerance) or 30000 (step underflow) and, as in the reference, the
+ * process exits with status 0 after printing the reason. */
+void rkf_c_(int neq, double eps_th, double *X0, double *XE, double *YH0, int *INIT, const double *RELERR,
+            const double *ABSERR, double *YHN, double *ESE, int *NDE, int *IER, double *H, double *YL0,
+            double *YLN, double *ERR, double *WRK);
+
+/* Replaces sincos_ (bia/sincos.c:3-6), the helper the reference CPU routines call. */
+void sincos_(double *x, double *s, double *c);
+
+/* ---------------------------------------------------------------------------------------- */
+/* 2. new entry points                                                                      */
+/* ---------------------------------------------------------------------------------------- */
+
+int bia_version(void); /* major*10000 + minor*100 + patch */
+const char *bia_error_string(int status);
+const char *bia_last_error(bia_ctx *ctx);
+
+int bia_create(bia_ctx **ctx, int device);
+int bia_destroy(bia_ctx *ctx);
+/* stream is a hipStream_t (0 = null stream); subsequent calls are ordered on it */
+int bia_set_stream(bia_ctx *ctx, void *stream);
+int bia_synchronize(bia_ctx *ctx);
+
+/* One whole BIA step of every unordered pair of an n-particle system.  No counterpart in the
+ * reference ABI: it replaces the per-particle loop of bia/bia.f90:128-202 (N calls of
+ * find_dxyuv_kernel_ with host scatter) by one device-resident pass.
+ *   d_state  SoA [6][ld] positions and velocities (device)
+ *   d_delta  SoA [6][ld] output: dX..dW including the drift v*dt (bia/bia.f90:194-196), i.e.
+ *            the same values the reference accumulates into dX..dW (device)
+ *   d_stats  BIA_STATS_LEN uint64 counters, accumulated into (device, may be NULL) */
+int bia_step_dev(bia_ctx *ctx, int method, int mode, int n, double eps, double dt, int itr,
+                 const double *d_state, int ld, double *d_delta, unsigned long long *d_stats);
+
+/* Host-array variant of bia_step_dev for Fortran callers (bind(C) interface in
+ * bia_3d_gpu_amd/fortran/bia_hip.f90).  dX..dW are overwritten; stats (may be NULL) is
+ * BIA_STATS_LEN counters for this call. */
+int bia_step_host(bia_ctx *ctx, int method, int mode, int n, double eps, double dt, int itr, const double *X,
+                  const double *Y, const double *Z, const double *U, const double *V, const double *W,
+                  double *dX, double *dY, double *dZ, double *dU, double *dV, double *dW,
+                  unsigned long long *stats);
+
+/* Invariants of an SoA state; out[11] = KE, PE, Lx, Ly, Lz, Ua, Va, Wa, Xa, Ya, Za
+ * (the order of invariance_gpu_'s outputs).  d_out is device memory. */
+int bia_invariants_dev(bia_ctx *ctx, int n, double eps, const double *d_state, int ld, double *d_out);
+int bia_invariants_host(bia_ctx *ctx, int n, double eps, const double *X, const double *Y, const double *Z,
+                        const double *U, const double *V, const double *W, double *out);
+
+/* DIM direct-sum RKF45 over [*x0, xe] on an AoS state y[6 np] (host, in/out), with the
+ * reference's step controller (dim/rkf_kernel.cu:237-491).  *h is the step (in/out; set
+ * when *init == 1).  ese (may be NULL) receives (YLN - YHN)/31.  Counters may be NULL.
+ * Returns BIA_OK, or BIA_ERR_RKF with *ier = 10000/20000/30000. */
+int bia_dim_rkf45_host(bia_ctx *ctx, int np, double eps, double *x0, double xe, double *y, int *init,
+                       double relerr, double abserr, double *h, double *ese, int *nde, int *naccept,
+                       int *nreject, int *ier);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BIA_HIP_H */

@@ -1,0 +1,761 @@
+/*
+ * bia_oracle.c -- CPU restatement of the reference's pairwise BIA hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg, never by the product library.
+ *
+ * Semantics follow the reference GPU path (bia/find_dXYUV_kernel.cu, bia/bia-rkf.cu,
+ * bia/invariance.cu, dim/rkf_kernel.cu) step for step, including its iterate-selection
+ * rules, so that the HIP kernels can be checked against it.  Each function cites the
+ * reference lines it restates.  Transcendentals come from glibc libm; the reference GPU
+ * build used CUDA libdevice, so agreement with the reference's own outputs is
+ * tolerance-based (pinned by the OUT/ fixtures and the amdflang-built CPU routines).
+ *
+ * Parallelism: OpenMP over rows only where the result does not depend on it (pair
+ * results are computed in parallel, then accumulated in the reference's serial order).
+ */
+#include "bia_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <quadmath.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PI 3.14159265358979323846264338327950288419716939937510582097494
+#define ORC_CONV_H 2.22e-16 /* find_dXYUV_kernel.cu:18 */
+#define ORC_CONV_E 2e-16    /* find_dXYUV_kernel.cu:284 */
+
+/* ------------------------------------------------------------------------------------ */
+/* orbital-plane frame: find_dXYUV_kernel.cu:467-488 (CPU twin bia.f90:284-314)           */
+/* ------------------------------------------------------------------------------------ */
+void orc_frame(const double R[3], const double G[3], double e_xi[3], double e_eta[3], double YZ[4]) {
+  double gm = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+  e_eta[0] = G[0] / gm;
+  e_eta[1] = G[1] / gm;
+  e_eta[2] = G[2] / gm;
+  double L0 = R[1] * G[2] - R[2] * G[1];
+  double L1 = R[2] * G[0] - R[0] * G[2];
+  double L2 = R[0] * G[1] - R[1] * G[0];
+  double Lm = sqrt(L0 * L0 + L1 * L1 + L2 * L2);
+  double z0 = L0 / Lm, z1 = L1 / Lm, z2 = L2 / Lm;
+  e_xi[0] = e_eta[1] * z2 - e_eta[2] * z1;
+  e_xi[1] = e_eta[2] * z0 - e_eta[0] * z2;
+  e_xi[2] = e_eta[0] * z1 - e_eta[1] * z0;
+  YZ[0] = R[0] * e_xi[0] + R[1] * e_xi[1] + R[2] * e_xi[2];
+  YZ[1] = R[0] * e_eta[0] + R[1] * e_eta[1] + R[2] * e_eta[2];
+  YZ[2] = 0;
+  YZ[3] = gm;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* hyperbola: find_dXYUV_kernel.cu:94-238 with find_theta_end_cu :57-66                  */
+/* P = {theta_0, sn_0, cs_0, tn_0}                                                        */
+/* ------------------------------------------------------------------------------------ */
+static double hyp_newton(double theta, double tend, const double P[4], double *err) {
+  double snp = sin((P[0] + theta) / 2); /* solution_sn_p_cu :46-50 */
+  double snm = sin((P[0] - theta) / 2); /* solution_sn_m_cu :51-55 */
+  double cs = cos(theta), sn = sin(theta);
+  double Dinv = 1 / (cs - P[2]);
+  double dfdth = P[3] * (1 - P[2] * cs) * Dinv * Dinv + P[1] / (2 * snp * snm);
+  *err = P[3] * sn * Dinv - tend + log(snp / snm);
+  return theta - *err / dfdth;
+}
+
+void orc_hyperbola(double eps, double dt, int itr, const double YZ[4], double YN[4], int *iters,
+                   int *flags) {
+  int fl = 0;
+  double X = YZ[0], Y = YZ[1], U = YZ[2], V = YZ[3];
+  double W_ini = U * U + V * V + 2 * eps / sqrt(X * X + Y * Y); /* solution_energy_cu :81-86 */
+  double r_g = X * U + Y * V;
+  double L = X * V - Y * U;
+  double r = sqrt(X * X + Y * Y);
+  double g0 = sqrt(W_ini);
+  double b0 = eps / W_ini;
+  double b = fabs(L) / g0;
+  double P[4];
+  double th0 = P[0] = atan2(fabs(L), b0 * g0);
+  double sn0 = P[1] = fabs(L) / sqrt(b0 * b0 * g0 * g0 + L * L);
+  double cs0 = P[2] = b0 * g0 / sqrt(b0 * b0 * g0 * g0 + L * L);
+  double tn0 = P[3] = sn0 / cs0;
+  double th_ini, cs, sn;
+  if (fabs(cs0 + b * sn0 / r) > 1) { /* :119-126 */
+    th_ini = 1e-30;
+    cs = cos(th_ini);
+    fl |= ORC_F_ACOS_CLAMP;
+  } else {
+    th_ini = acos(cs0 + b * sn0 / r);
+    cs = cs0 + b * sn0 / r;
+  }
+  if (L * r_g < 0) th_ini = -th_ini;
+  sn = sin(th_ini);
+  double x = r * cs, y = r * sn;
+  double g0_sn0 = (L > 0) ? g0 / sn0 : -g0 / sn0; /* :136-142 */
+  double u = g0_sn0 * (cs0 * sn);
+  double v = g0_sn0 * (1 - cs0 * cs);
+  double cs_a = (x * X + y * Y) / sqrt((X * X + Y * Y) * (x * x + y * y)); /* :150-151 */
+  double sn_a = (x * Y - y * X) / sqrt((X * X + Y * Y) * (x * x + y * y));
+  double time_ini = (tn0 * sn / (cs - cs0) + log(sin((th0 + th_ini) / 2) / sin((th0 - th_ini) / 2))) *
+                    b0 / g0; /* :153-155 */
+  double time_end = (L > 0) ? time_ini + dt : time_ini - dt;     /* :161-166 */
+  double TE = time_end * g0 / b0;
+  double th_end, err;
+  {
+    double e_ini, e_gss;
+    double th_gss = 2 / ORC_PI * th0 * atan(TE / tn0); /* :172 */
+    double th_ini_end = hyp_newton(th_ini, TE, P, &e_ini);
+    double th_gss_end = hyp_newton(th_gss, TE, P, &e_gss);
+    if (fabs(e_gss) > 10 * fabs(e_ini)) {
+      th_end = th_ini_end;
+      err = e_ini;
+    } else {
+      th_end = th_gss_end;
+      err = e_gss;
+    }
+  }
+  /* best-iterate Newton loop with exact-repeat detection: :187-217 */
+  int ir = 0;
+  double err_stock = err, th_stock = 0, hist[20];
+  int updated = 0, cycled = 0;
+  if (itr > 20) itr = 20;
+  while (ir < itr) {
+    double th_now = th_end;
+    ir++;
+    th_end = hyp_newton(th_now, TE, P, &err);
+    for (int num = ir - 1; num >= 1; num--) {
+      if (hist[num - 1] == err) {
+        cycled = 1;
+        break;
+      }
+    }
+    if (cycled) break;
+    hist[ir - 1] = err;
+    if (fabs(err) < fabs(err_stock)) {
+      err_stock = err;
+      th_stock = th_end;
+      updated = 1;
+    }
+    if (fabs(err_stock) < ORC_CONV_H) break;
+  }
+  if (cycled) fl |= ORC_F_CYCLE;
+  else if (!(fabs(err_stock) < ORC_CONV_H)) fl |= ORC_F_ITR_CAP;
+  if (!updated) fl |= ORC_F_STOCK_DEFAULT;
+  th_end = th_stock;
+  /* state after dt and rotation back to the (xi, eta) frame: :220-237 */
+  double cs_end = cos(th_end), sn_end = sin(th_end);
+  double u_end = g0_sn0 * cs0 * sn_end;
+  double v_end = g0_sn0 * (1 - cs0 * cs_end);
+  double du = u_end - u, dv = v_end - v;
+  double r_end = b * sn0 / (cs_end - cs0);
+  double x_end = r_end * cs_end, y_end = r_end * sn_end;
+  YN[0] = x_end * cs_a - y_end * sn_a;
+  YN[1] = x_end * sn_a + y_end * cs_a;
+  YN[2] = YZ[2] + (du * cs_a - dv * sn_a);
+  YN[3] = YZ[3] + (du * sn_a + dv * cs_a);
+  if (iters) *iters = ir;
+  if (flags) *flags = fl;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* ellipse: find_dXYUV_kernel.cu:292-364 with newton_ellipse :242-288                     */
+/* ------------------------------------------------------------------------------------ */
+static double ell_newton(int itr, double dt, double xi_ini, double ecc, double omega_a, double tau_end,
+                         int *iters, int *fl) {
+  double xi;
+  int ir = 0;
+  double error_ini = tau_end - xi_ini + ecc * sin(xi_ini);
+  double delta_ini = error_ini * (1 - ecc * cos(xi_ini)); /* the reference multiplies here */
+  if (error_ini == 0) {
+    *iters = 0;
+    return xi_ini;
+  }
+  double xi_2 = xi_ini + omega_a * dt;
+  if (cos(tau_end) < 0.9) {
+    xi_2 = tau_end;
+  } else {
+    xi_2 = (sin(tau_end) >= 0) ? ORC_PI / 2 : -ORC_PI / 2;
+  }
+  double error_tm2 = tau_end - xi_2 + ecc * sin(xi_2);
+  double delta_xi2 = error_tm2 / (1 - ecc * cos(xi_2));
+  if (fabs(error_ini) < fabs(error_tm2)) xi = xi_2 + delta_ini;
+  else xi = xi_2 + delta_xi2;
+  double err_stock = 100, th_stock = 0;
+  int conv = 0;
+  while (ir <= itr) { /* :274-285, at most itr + 1 corrections */
+    ir++;
+    double xi_1 = xi;
+    double e = tau_end - xi_1 + ecc * sin(xi_1);
+    double d = e / (1 - ecc * cos(xi_1));
+    xi = xi_1 + d;
+    if (fabs(e) < fabs(err_stock)) {
+      err_stock = e;
+      th_stock = xi;
+    }
+    if (fabs(err_stock) < ORC_CONV_E) {
+      conv = 1;
+      break;
+    }
+  }
+  if (!conv) *fl |= ORC_F_ITR_CAP;
+  *iters = ir;
+  return th_stock;
+}
+
+void orc_ellipse(double eps, double dt, int itr, const double YZ[4], double YN[4], int *iters,
+                 int *flags) {
+  int fl = 0, it = 0;
+  double X = YZ[0], Y = YZ[1], U = YZ[2], V = YZ[3];
+  double g2 = U * U + V * V;
+  double r_ini = sqrt(X * X + Y * Y);
+  double W = g2 + 2 * eps / r_ini;
+  double L = X * V - Y * U;
+  double r_g = X * U + Y * V;
+  double g_a = (L >= 0) ? sqrt(-W) : -sqrt(-W);
+  double a = eps / W;
+  double b = L / g_a;
+  double omega_a = g_a / a;
+  double xi_ecc = asin(L / (a * g_a));
+  double ecc = cos(xi_ecc); /* :311 (the CPU twin uses a square root instead) */
+  double xi_ini = 0;
+  if (r_g == 0) { /* apsides: :315-330 */
+    if (r_ini < a) {
+      xi_ini = 0;
+    } else if (r_ini == a) {
+      xi_ini = atan2(Y, X);
+      ecc = 0;
+      xi_ecc = ORC_PI / 2;
+    } else {
+      xi_ini = (L >= 0) ? ORC_PI : -ORC_PI;
+    }
+  } else {
+    xi_ini = atan2(r_g / g_a, a - r_ini);
+  }
+  double alpha = atan2(Y, X) - atan2(b * sin(xi_ini), a * (cos(xi_ini) - ecc)); /* :334 */
+  double tau_ini = xi_ini - ecc * sin(xi_ini);
+  double time_ini = tau_ini / omega_a;
+  double time_end = time_ini + dt;
+  double tau_end = omega_a * time_end;
+  if (dt == 0) {
+    /* the reference assigns the pointer (:340-344) and leaves YN unwritten; the intended
+       result, the unchanged state, is returned here */
+    YN[0] = YZ[0]; YN[1] = YZ[1]; YN[2] = YZ[2]; YN[3] = YZ[3];
+    if (iters) *iters = 0;
+    if (flags) *flags = 0;
+    return;
+  }
+  double xi_end = ell_newton(itr, dt, xi_ini, ecc, omega_a, tau_end, &it, &fl);
+  double cs_end = cos(xi_end), sn_end = sin(xi_end);
+  double xx = a * (cs_end - ecc);
+  double yy = b * sn_end;
+  double rr = a * (1 - ecc * cs_end);
+  double uu = -a * g_a * sn_end / rr;
+  double vv = b * g_a * cs_end / rr;
+  double cs_alp = cos(alpha), sn_alp = sin(alpha);
+  YN[0] = xx * cs_alp - yy * sn_alp;
+  YN[1] = xx * sn_alp + yy * cs_alp;
+  YN[2] = uu * cs_alp - vv * sn_alp;
+  YN[3] = uu * sn_alp + vv * cs_alp;
+  if (iters) *iters = it;
+  if (flags) *flags = fl;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* parabola (W == 0 exactly): find_dXYUV_kernel.cu:367-465 (Barker + Cardano guess)       */
+/* ------------------------------------------------------------------------------------ */
+static double par_time(double th, double rmg) { /* time_parabola :367-371 */
+  double sn = sin(th), cs = cos(th);
+  return (2 + cs) * sn / (1 + cs) / (1 + cs) * 4 * rmg / 3;
+}
+static double par_dtdth(double th, double rmg) { /* dt_dtheta :373-379 */
+  double t = 1 / cos(th / 2);
+  t = t * t;
+  t = t * t;
+  return rmg * t;
+}
+
+void orc_parabola(double eps, double dt, const double YZ[4], double YN[4], int *iters, int *flags) {
+  double r_ini = sqrt(YZ[0] * YZ[0] + YZ[1] * YZ[1]);
+  double r_dot_g = YZ[0] * YZ[2] + YZ[1] * YZ[3];
+  double r_crs_g = YZ[0] * YZ[3] - YZ[1] * YZ[2];
+  double g_max = -2 * eps / r_crs_g;
+  double r_min = r_crs_g / g_max;
+  double rmg = r_min / g_max;
+  double x_ini = 2 * r_min - r_ini;
+  double tmp = 2 * r_min * sqrt(r_ini / r_min - 1);
+  double y_ini = (r_dot_g > 0) ? tmp : -tmp;
+  double th_ini = atan2(y_ini, x_ini);
+  double time_ini = par_time(th_ini, rmg);
+  double time_end = time_ini + dt;
+  double te = 3 * g_max / (4 * r_min) * time_end; /* x^3 + 3x + 2 te = 0 form, :402 */
+  double Fp = cbrt(sqrt(te * te + 1) + te);
+  double Fm = cbrt(sqrt(te * te + 1) - te);
+  double th_gss = 2 * atan(Fp - Fm);
+  enum { NIT = 20 };
+  double th[NIT], et[NIT];
+  th[0] = th_gss;
+  et[0] = par_time(th[0], rmg) - time_end;
+  int i_min = 0, i;
+  double et_min = fabs(et[0]);
+  for (i = 1; i < NIT; i++) { /* :419-440 */
+    if (et[i - 1] == 0) {
+      i_min = i - 1;
+      break;
+    }
+    th[i] = th[i - 1] - et[i - 1] / par_dtdth(th[i - 1], rmg);
+    et[i] = par_time(th[i], rmg) - time_end;
+    if (th[i] == th[i - 1]) {
+      i_min = i;
+      break;
+    } else {
+      if (et_min > fabs(et[i])) {
+        i_min = i;
+        et_min = fabs(et[i]);
+      }
+      for (int j = 0; j < i; j++) {
+        if (et[i] == et[j]) {
+          et_min = fabs(et[i_min]);
+          i_min = j;
+          break;
+        }
+      }
+    }
+    if (i != i_min && et[i] == et[i_min]) break;
+  }
+  double th_end = th[i_min];
+  double be = th_end / 2;
+  double sb = sin(be), cb = cos(be);
+  double r_end = r_min / (cb * cb);
+  double g_end = g_max * cb;
+  double st = sin(th_end), ct = cos(th_end);
+  double x_end = r_end * ct, y_end = r_end * st;
+  double u_end = -g_end * sb, v_end = g_end * cb;
+  double gamma = atan2(x_ini * YZ[1] - y_ini * YZ[0], x_ini * YZ[0] + y_ini * YZ[1]);
+  double sg = sin(gamma), cg = cos(gamma);
+  YN[0] = x_end * cg - y_end * sg;
+  YN[1] = x_end * sg + y_end * cg;
+  YN[2] = u_end * cg - v_end * sg;
+  YN[3] = u_end * sg + v_end * cg;
+  if (iters) *iters = (i < NIT) ? i : NIT - 1;
+  if (flags) *flags = 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* per-pair RKF45 (Fehlberg 4(5) + step doubling): bia-rkf.cu:1-103                       */
+/* ------------------------------------------------------------------------------------ */
+static const double
+    AL2 = 1e0 / 4, AL3 = 3e0 / 8, AL4 = 12e0 / 13, AL5 = 1e0, AL6 = 1e0 / 2,
+    B21 = 1e0 / 4, B31 = 3e0 / 32, B32 = 9e0 / 32, B41 = 1932e0 / 2197, B42 = -7200e0 / 2197,
+    B43 = 7296e0 / 2197, B51 = 439e0 / 216, B52 = -8e0, B53 = 3680e0 / 513, B54 = -845e0 / 4104,
+    B61 = -8e0 / 27, B62 = 2e0, B63 = -3544e0 / 2565, B64 = 1859e0 / 4104, B65 = -11e0 / 40,
+    GA1 = 16e0 / 135, GA3 = 6656e0 / 12825, GA4 = 28561e0 / 56430, GA5 = -9e0 / 50, GA6 = 2e0 / 55,
+    DA1 = 1e0 / 360, DA3 = -128e0 / 4275, DA4 = -2197e0 / 75240, DA5 = 1e0 / 50, DA6 = 2e0 / 55;
+
+static void kep_f(double eps, const double *y, double *f) { /* FUNC :14-20 */
+  double rsq = y[0] * y[0] + y[1] * y[1];
+  double aaa = eps / (rsq * sqrt(rsq));
+  f[0] = y[2];
+  f[1] = y[3];
+  f[2] = y[0] * aaa;
+  f[3] = y[1] * aaa;
+}
+
+static void kep_fehl(int lindex, double eps, double h, const double *y0, double *yn, double *er) {
+  /* FEHL :22-30 (time argument unused: the field is autonomous) */
+  double k1[4], k2[4], k3[4], k4[4], k5[4], k6[4], w[4];
+  int i;
+  kep_f(eps, y0, k1);
+  for (i = 0; i < 4; i++) w[i] = y0[i] + h * (B21 * k1[i]);
+  kep_f(eps, w, k2);
+  for (i = 0; i < 4; i++) w[i] = y0[i] + h * (B31 * k1[i] + B32 * k2[i]);
+  kep_f(eps, w, k3);
+  for (i = 0; i < 4; i++) w[i] = y0[i] + h * (B41 * k1[i] + B42 * k2[i] + B43 * k3[i]);
+  kep_f(eps, w, k4);
+  for (i = 0; i < 4; i++) w[i] = y0[i] + h * (B51 * k1[i] + B52 * k2[i] + B53 * k3[i] + B54 * k4[i]);
+  kep_f(eps, w, k5);
+  for (i = 0; i < 4; i++)
+    w[i] = y0[i] + h * (B61 * k1[i] + B62 * k2[i] + B63 * k3[i] + B64 * k4[i] + B65 * k5[i]);
+  kep_f(eps, w, k6);
+  for (i = 0; i < 4; i++) yn[i] = y0[i] + h * (GA1 * k1[i] + GA3 * k3[i] + GA4 * k4[i] + GA5 * k5[i] + GA6 * k6[i]);
+  if (lindex == 1)
+    for (i = 0; i < 4; i++) er[i] = h * (DA1 * k1[i] + DA3 * k3[i] + DA4 * k4[i] + DA5 * k5[i] + DA6 * k6[i]);
+}
+
+static void kep_fhstep(double eps, double h, double *y0, double *yn) { /* FHSTEP :38-43 */
+  double h1 = h / 2;
+  kep_fehl(0, eps, h1, y0, yn, NULL);
+  for (int i = 0; i < 4; i++) y0[i] = yn[i];
+  kep_fehl(0, eps, h1, y0, yn, NULL);
+}
+
+void orc_rkf_pair(double eps, double dt, const double YZ[4], double YN[4], int *nsteps, int *flags) {
+  /* rkf_device :47-103; RELERR = 0, ABSERR = 1e-16, ITEMAX = 100000 (:3-5) */
+  const double RELERR = 0, ABSERR = 1e-16, MINLIMIT = 1e-300, EPSMIN = 1e-15;
+  const int ITEMAX = 100000;
+  double yh0[4] = {YZ[0], YZ[1], YZ[2], YZ[3]}, yl0[4], yln[4], er[4], f[4], yhn[4];
+  int i, steps = 0, done = 0;
+  for (i = 0; i < 4; i++) yl0[i] = yh0[i];
+  double x = 0;
+  kep_f(eps, yl0, f);
+  double toldy = 1e+75;
+  int changed = 0;
+  for (i = 0; i < 4; i++) {
+    if (fabs(f[i]) > MINLIMIT) {
+      double tol = RELERR * fabs(yl0[i]) + ABSERR;
+      toldy = fmin(toldy, fabs(tol / f[i]));
+      changed = 1;
+    }
+  }
+  double hmin = EPSMIN * dt;
+  double h = changed ? fmin(dt / 100, pow(toldy, 0.2)) : hmin;
+  for (i = 0; i < 4; i++) yhn[i] = yh0[i];
+  for (int iter = 0; iter < ITEMAX; iter++) {
+    kep_fehl(1, eps, h, yl0, yln, er);
+    double erret = 0;
+    for (i = 0; i < 4; i++) {
+      double et = RELERR * (fabs(yl0[i]) + fabs(yln[i])) / 2 + ABSERR;
+      if (et < MINLIMIT) iter = ITEMAX;
+      else erret = fmax(erret, fabs(er[i]) / et);
+    }
+    if (erret > 1) {
+      h *= (erret >= 59049) ? 0.1 : 0.9 / pow(erret, 0.2);
+      if (h <= EPSMIN) break;
+    } else if (x + h < dt) {
+      kep_fhstep(eps, h, yh0, yhn);
+      x += h;
+      steps++;
+      for (i = 0; i < 4; i++) {
+        yl0[i] = yln[i];
+        yh0[i] = yhn[i];
+      }
+      h *= (erret <= 1.889568e-4) ? 5.0 : 0.9 / pow(erret, 0.2);
+    } else {
+      h = dt - x;
+      kep_fehl(1, eps, h, yl0, yln, er);
+      kep_fhstep(eps, h, yh0, yhn);
+      steps++;
+      done = 1;
+      break;
+    }
+  }
+  for (i = 0; i < 4; i++) YN[i] = yhn[i];
+  if (nsteps) *nsteps = steps;
+  if (flags) *flags = done ? 0 : ORC_F_RKF_FAIL;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* one pair: BIA_Device body, find_dXYUV_kernel.cu:492-518                                */
+/* ------------------------------------------------------------------------------------ */
+static void solve2d(int method, double eps, double dt, int itr, const double YZ[4], double YN[4],
+                    int *branch, int *iters, int *flags) {
+  if (method == 0) {
+    *branch = ORC_RKF;
+    orc_rkf_pair(eps, dt, YZ, YN, iters, flags);
+    return;
+  }
+  double W = sqrt(YZ[0] * YZ[0] + YZ[1] * YZ[1]) * (YZ[2] * YZ[2] + YZ[3] * YZ[3]) + 2 * eps; /* :503 */
+  if (W > 0) {
+    *branch = ORC_HYPERBOLA;
+    orc_hyperbola(eps, dt, itr, YZ, YN, iters, flags);
+  } else if (W == 0) {
+    *branch = ORC_PARABOLA;
+    orc_parabola(eps, dt, YZ, YN, iters, flags);
+  } else {
+    *branch = ORC_ELLIPSE;
+    orc_ellipse(eps, dt, itr, YZ, YN, iters, flags);
+  }
+}
+
+void orc_pair(int method, double eps, double dt, int itr, const double rel[6], double out[6], int *branch,
+              int *iters, int *flags) {
+  double exi[3], eeta[3], YZ[4], YN[4];
+  int br = 0, it = 0, fl = 0;
+  orc_frame(rel, rel + 3, exi, eeta, YZ);
+  solve2d(method, eps, dt, itr, YZ, YN, &br, &it, &fl);
+  double drx = YN[0] - YZ[0], dre = YN[1] - YZ[1];
+  double dgx = YN[2] - YZ[2], dge = YN[3] - YZ[3];
+  for (int c = 0; c < 3; c++) {
+    out[c] = (drx * exi[c] + dre * eeta[c] - rel[3 + c] * dt) / 2; /* :510-512 */
+    out[3 + c] = (dgx * exi[c] + dge * eeta[c]) / 2;              /* :513-515 */
+  }
+  for (int c = 0; c < 6; c++)
+    if (!isfinite(out[c])) fl |= ORC_F_NONFINITE;
+  if (branch) *branch = br;
+  if (iters) *iters = it;
+  if (flags) *flags = fl;
+}
+
+void orc_pairs2d(int method, double eps, double dt, int itr, int64_t n, const double *yz, double *yn,
+                 int *branch, int *iters, int *flags) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t k = 0; k < n; k++) {
+    int br, it, fl;
+    solve2d(method, eps, dt, itr, yz + 4 * k, yn + 4 * k, &br, &it, &fl);
+    if (branch) branch[k] = br;
+    if (iters) iters[k] = it;
+    if (flags) flags[k] = fl;
+  }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* whole step: bia.f90:126-204 with the host scatter of find_dXYUV_kernel.cu:541-546      */
+/* ------------------------------------------------------------------------------------ */
+void orc_bia_rows(int method, int n, double eps, double dt, int itr, const double *st, int i0, int i1,
+                  double *delta, int64_t *hist) {
+  const double *X = st, *Y = st + n, *Z = st + 2 * (size_t)n, *U = st + 3 * (size_t)n,
+               *V = st + 4 * (size_t)n, *Wv = st + 5 * (size_t)n;
+  double *dX = delta, *dY = delta + n, *dZ = delta + 2 * (size_t)n, *dU = delta + 3 * (size_t)n,
+         *dV = delta + 4 * (size_t)n, *dW = delta + 5 * (size_t)n;
+  double *row = (double *)malloc(sizeof(double) * 6 * (size_t)n);
+  int *rbr = (int *)malloc(sizeof(int) * (size_t)n);
+  int *rit = (int *)malloc(sizeof(int) * (size_t)n);
+  for (int i = i0; i < i1; i++) {
+    /* pair results for row i computed in parallel (order-independent) ... */
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int j = i + 1; j < n; j++) {
+      double rel[6] = {X[i] - X[j], Y[i] - Y[j], Z[i] - Z[j], U[i] - U[j], V[i] - V[j], Wv[i] - Wv[j]};
+      int fl;
+      orc_pair(method, eps, dt, itr, rel, row + 6 * (size_t)j, rbr + j, rit + j, &fl);
+    }
+    /* ... then accumulated serially in the reference's order */
+    for (int j = i + 1; j < n; j++) {
+      const double *d = row + 6 * (size_t)j;
+      dX[i] += d[0]; dY[i] += d[1]; dZ[i] += d[2];
+      dX[j] -= d[0]; dY[j] -= d[1]; dZ[j] -= d[2];
+      dU[i] += d[3]; dV[i] += d[4]; dW[i] += d[5];
+      dU[j] -= d[3]; dV[j] -= d[4]; dW[j] -= d[5];
+      if (hist) {
+        int it = rit[j] < 0 ? 0 : (rit[j] > 63 ? 63 : rit[j]);
+        hist[rbr[j] * 64 + it]++;
+      }
+    }
+    dX[i] += U[i] * dt; /* drift, bia.f90:194-196 */
+    dY[i] += V[i] * dt;
+    dZ[i] += Wv[i] * dt;
+  }
+  free(row);
+  free(rbr);
+  free(rit);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* invariance                                                                            */
+/* ------------------------------------------------------------------------------------ */
+/* GPU order: row sums p_i over j > i (inv_device, invariance.cu:15-26), then the serial sums
+   of sum_device (:28-48) with p summed from the end (:45) */
+void orc_invariance_gpu(int n, double eps, const double *st, double out[11]) {
+  const double *x = st, *y = st + n, *z = st + 2 * (size_t)n, *u = st + 3 * (size_t)n,
+               *v = st + 4 * (size_t)n, *w = st + 5 * (size_t)n;
+  double *p = (double *)malloc(sizeof(double) * (size_t)(n > 1 ? n - 1 : 1));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n - 1; i++) {
+    double s = 0;
+    for (int j = i + 1; j < n; j++) {
+      double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+      s += 1 / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    p[i] = s;
+  }
+  double c[11] = {0};
+  for (int j = 0; j < n; j++) {
+    c[0] += x[j]; c[1] += y[j]; c[2] += z[j];
+    c[3] += u[j]; c[4] += v[j]; c[5] += w[j];
+    c[6] += y[j] * w[j] - z[j] * v[j];
+    c[7] += z[j] * u[j] - x[j] * w[j];
+    c[8] += x[j] * v[j] - y[j] * u[j];
+    c[9] += u[j] * u[j] + v[j] * v[j] + w[j] * w[j];
+  }
+  for (int j = n - 2; j >= 0; j--) c[10] += p[j];
+  free(p);
+  /* invariance.cu:88-91 */
+  out[0] = c[9]; out[1] = eps * c[10];
+  out[2] = c[6]; out[3] = c[7]; out[4] = c[8];
+  out[5] = c[3]; out[6] = c[4]; out[7] = c[5];
+  out[8] = c[0]; out[9] = c[1]; out[10] = c[2];
+}
+
+/* Invariance_CPU.f90:1-30, double precision, ascending order */
+void orc_invariance_cpu(int n, double eps, const double *st, double out[11]) {
+  const double *x = st, *y = st + n, *z = st + 2 * (size_t)n, *u = st + 3 * (size_t)n,
+               *v = st + 4 * (size_t)n, *w = st + 5 * (size_t)n;
+  double Xa = 0, Ya = 0, Za = 0, Ua = 0, Va = 0, Wa = 0, Lx = 0, Ly = 0, Lz = 0, KE = 0, PE = 0;
+  for (int j = 0; j < n; j++) {
+    Xa += x[j]; Ya += y[j]; Za += z[j]; Ua += u[j]; Va += v[j]; Wa += w[j];
+    Lx += y[j] * w[j] - z[j] * v[j];
+    Ly += z[j] * u[j] - x[j] * w[j];
+    Lz += x[j] * v[j] - y[j] * u[j];
+    KE += u[j] * u[j] + v[j] * v[j] + w[j] * w[j];
+  }
+  for (int i = 0; i < n - 1; i++)
+    for (int j = i + 1; j < n; j++) {
+      double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+      PE += 1 / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  out[0] = KE; out[1] = eps * PE;
+  out[2] = Lx; out[3] = Ly; out[4] = Lz;
+  out[5] = Ua; out[6] = Va; out[7] = Wa;
+  out[8] = Xa; out[9] = Ya; out[10] = Za;
+}
+
+/* Invariance_QPU.f90:1-21: inputs widened to quad precision, every sum in quad */
+void orc_invariance_quad(int n, double eps, const double *st, double out[11]) {
+  const double *x = st, *y = st + n, *z = st + 2 * (size_t)n, *u = st + 3 * (size_t)n,
+               *v = st + 4 * (size_t)n, *w = st + 5 * (size_t)n;
+  __float128 Xa = 0, Ya = 0, Za = 0, Ua = 0, Va = 0, Wa = 0, Lx = 0, Ly = 0, Lz = 0, KE = 0;
+  for (int j = 0; j < n; j++) {
+    __float128 X = x[j], Y = y[j], Z = z[j], U = u[j], V = v[j], W = w[j];
+    Xa += X; Ya += Y; Za += Z; Ua += U; Va += V; Wa += W;
+    Lx += Y * W - Z * V;
+    Ly += Z * U - X * W;
+    Lz += X * V - Y * U;
+    KE += U * U + V * V + W * W;
+  }
+  /* row partial sums in quad, combined in row order: the same value as one running sum up
+     to quad rounding (~1e-34 relative), far below the double result's resolution */
+  __float128 *prow = (__float128 *)malloc(sizeof(__float128) * (size_t)(n > 0 ? n : 1));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n - 1; i++) {
+    __float128 s = 0, xi = x[i], yi = y[i], zi = z[i];
+    for (int j = i + 1; j < n; j++) {
+      __float128 dx = xi - (__float128)x[j], dy = yi - (__float128)y[j], dz = zi - (__float128)z[j];
+      s += 1 / sqrtq(dx * dx + dy * dy + dz * dz);
+    }
+    prow[i] = s;
+  }
+  __float128 PE = 0;
+  for (int i = 0; i < n - 1; i++) PE += prow[i];
+  free(prow);
+  out[0] = (double)KE; out[1] = (double)((__float128)eps * PE);
+  out[2] = (double)Lx; out[3] = (double)Ly; out[4] = (double)Lz;
+  out[5] = (double)Ua; out[6] = (double)Va; out[7] = (double)Wa;
+  out[8] = (double)Xa; out[9] = (double)Ya; out[10] = (double)Za;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* DIM direct-sum RKF45: dim/rkf_kernel.cu:194-495 (host stepper) with func :29-53        */
+/* ------------------------------------------------------------------------------------ */
+void orc_dim_func(int np, double eps, const double *y, double *f) {
+#pragma omp parallel for schedule(static)
+  for (int ip = 0; ip < np; ip++) {
+    const double *yi = y + 6 * (size_t)ip;
+    double ax = 0, ay = 0, az = 0;
+    for (int jp = 0; jp < np; jp++) {
+      const double *yj = y + 6 * (size_t)jp;
+      double xij = yi[0] - yj[0], yij = yi[1] - yj[1], zij = yi[2] - yj[2];
+      double rsq = xij * xij + yij * yij + zij * zij + 1e-99;
+      double e = eps / (rsq * sqrt(rsq));
+      ax += xij * e;
+      ay += yij * e;
+      az += zij * e;
+    }
+    double *fi = f + 6 * (size_t)ip;
+    fi[0] = yi[3]; fi[1] = yi[4]; fi[2] = yi[5];
+    fi[3] = ax; fi[4] = ay; fi[5] = az;
+  }
+}
+
+static void dim_fehl(int lindex, int np, double eps, double h, const double *y0, double *yn, double *er,
+                     double *k /* 6*neq */, double *w /* neq */) {
+  size_t neq = 6 * (size_t)np, i;
+  double *k1 = k, *k2 = k + neq, *k3 = k + 2 * neq, *k4 = k + 3 * neq, *k5 = k + 4 * neq, *k6 = k + 5 * neq;
+  orc_dim_func(np, eps, y0, k1);
+  for (i = 0; i < neq; i++) w[i] = y0[i] + h * B21 * k1[i]; /* fehl_1 :76-84 */
+  orc_dim_func(np, eps, w, k2);
+  for (i = 0; i < neq; i++) w[i] = y0[i] + h * (B31 * k1[i] + B32 * k2[i]);
+  orc_dim_func(np, eps, w, k3);
+  for (i = 0; i < neq; i++) w[i] = y0[i] + h * (B41 * k1[i] + B42 * k2[i] + B43 * k3[i]);
+  orc_dim_func(np, eps, w, k4);
+  for (i = 0; i < neq; i++) w[i] = y0[i] + h * (B51 * k1[i] + B52 * k2[i] + B53 * k3[i] + B54 * k4[i]);
+  orc_dim_func(np, eps, w, k5);
+  for (i = 0; i < neq; i++)
+    w[i] = y0[i] + h * (B61 * k1[i] + B62 * k2[i] + B63 * k3[i] + B64 * k4[i] + B65 * k5[i]);
+  orc_dim_func(np, eps, w, k6);
+  for (i = 0; i < neq; i++) yn[i] = y0[i] + h * (GA1 * k1[i] + GA3 * k3[i] + GA4 * k4[i] + GA5 * k5[i] + GA6 * k6[i]);
+  if (lindex == 1)
+    for (i = 0; i < neq; i++) er[i] = h * (DA1 * k1[i] + DA3 * k3[i] + DA4 * k4[i] + DA5 * k5[i] + DA6 * k6[i]);
+}
+
+int orc_dim_rkf45(int np, double eps, double *x0, double xe, double *y, double relerr, double abserr,
+                  double *ese, int *nde, int *naccept, int *nreject, double *step_t, double *step_y1,
+                  int cap) {
+  const double EPSMIN = 1e-15; /* rkf_kernel.cu:201-202 */
+  const int ITEMAX = 10000;
+  size_t neq = 6 * (size_t)np, i;
+  double *yl0 = (double *)malloc(sizeof(double) * neq), *yln = (double *)malloc(sizeof(double) * neq);
+  double *yh0 = (double *)malloc(sizeof(double) * neq), *yhn = (double *)malloc(sizeof(double) * neq);
+  double *er = (double *)malloc(sizeof(double) * neq), *w = (double *)malloc(sizeof(double) * neq);
+  double *k = (double *)malloc(sizeof(double) * 6 * neq);
+  int ier = 10000, acc = 0, rej = 0, evals = 0;
+  memcpy(yh0, y, sizeof(double) * neq);
+  memcpy(yl0, y, sizeof(double) * neq);
+  /* initial step, :237-265 */
+  orc_dim_func(np, eps, yl0, k);
+  evals = 1;
+  double toldy = DBL_MAX;
+  int changed = 0;
+  for (i = 0; i < neq; i++)
+    if (fabs(k[i]) > 0.0) {
+      double tol = relerr * fabs(yl0[i]) + abserr;
+      toldy = fmin(toldy, fabs(tol / k[i]));
+      changed = 1;
+    }
+  double H = changed ? fmin((xe - *x0) / 100e0, pow(toldy, 0.2)) : EPSMIN * (xe - *x0);
+  double X = *x0;
+  for (int iter = 0; iter < ITEMAX; iter++) {
+    double h = H;
+    dim_fehl(1, np, eps, h, yl0, yln, er, k, w);
+    evals += 6;
+    double erret = 0;
+    int bad = 0;
+    for (i = 0; i < neq; i++) {
+      double et = relerr * (fabs(yl0[i]) + fabs(yln[i])) / 2 + abserr;
+      if (et == 0) { bad = 1; break; }
+      erret = fmax(erret, fabs(er[i]) / et);
+    }
+    if (bad) { ier = 20000; goto out; }
+    if (erret > 1) {
+      rej++;
+      H *= (erret >= 59049) ? 0.1 : 0.9 / pow(erret, 0.2);
+      if (H <= EPSMIN) { ier = 30000; goto out; }
+    } else if (X + H < xe) {
+      /* fhstep: two half steps of the high solution, :318-363 */
+      dim_fehl(0, np, eps, H / 2, yh0, yhn, NULL, k, w);
+      memcpy(yh0, yhn, sizeof(double) * neq);
+      dim_fehl(0, np, eps, H / 2, yh0, yhn, NULL, k, w);
+      evals += 12;
+      X += H;
+      *x0 = X;
+      memcpy(yl0, yln, sizeof(double) * neq);
+      memcpy(yh0, yhn, sizeof(double) * neq);
+      if (step_t && acc < cap) {
+        step_t[acc] = X;
+        if (step_y1) memcpy(step_y1 + 6 * (size_t)acc, yhn, sizeof(double) * 6);
+      }
+      acc++;
+      H *= (erret <= 1.889568e-4) ? 5.0 : 0.9 / pow(erret, 0.2);
+    } else {
+      h = H = xe - X; /* last phase, :387-483 */
+      dim_fehl(1, np, eps, h, yl0, yln, er, k, w);
+      dim_fehl(0, np, eps, h / 2, yh0, yhn, NULL, k, w);
+      memcpy(yh0, yhn, sizeof(double) * neq);
+      dim_fehl(0, np, eps, h / 2, yh0, yhn, NULL, k, w);
+      evals += 18;
+      if (ese)
+        for (i = 0; i < neq; i++) ese[i] = (yln[i] - yhn[i]) / 31;
+      *x0 = xe;
+      if (step_t && acc < cap) {
+        step_t[acc] = xe;
+        if (step_y1) memcpy(step_y1 + 6 * (size_t)acc, yhn, sizeof(double) * 6);
+      }
+      acc++;
+      memcpy(y, yhn, sizeof(double) * neq);
+      ier = 0;
+      goto out;
+    }
+  }
+out:
+  if (nde) *nde = evals;
+  if (naccept) *naccept = acc;
+  if (nreject) *nreject = rej;
+  free(yl0); free(yln); free(yh0); free(yhn); free(er); free(w); free(k);
+  return ier;
+}
