@@ -1,0 +1,169 @@
+"""Benchmark: FP64 pair-interactions/s of the BIA-ALG step on one MI355X.
+
+Workload (default): N = 65536 equal-mass gravitating particles, Plummer sphere (a = 1, M = 1,
+G = 1, m = 1/N) with 20 % of the particles in hard binaries (a in [1e-3, 1e-2], thermal e),
+dt = 1e-3, eps = -2Gm, seed 65536.  One "step" = one BIA step of every unordered pair
+(N(N-1)/2 pair interactions) followed by the state update, state resident in HBM.
+
+Printed: one JSON line (rank 0).  ``value`` is the pair-interactions/s of the headline mode
+(learned initialiser when trained weights are available, else plain).  ``roofline`` prices the
+dominant kernel (the pair kernel) with the algorithmic FP64 flop count of the reference
+algorithm (stats counters of an untimed step on the same input) over its average duration,
+measured with HIP events recorded on the kernel's own stream.  ``cpu_baseline`` times the
+reference's own Fortran CPU pair loop (oracle/_ref, single thread) on a bounded row sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 pair-interactions/s, BIA-ALG (learned init) vs plain, 1 MI355X; % FP64 peak"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_state(n: int, seed: int):
+    from bia_3d_gpu_amd import ic
+    return ic.plummer_with_binaries(n, seed)
+
+
+def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup):
+    import torch
+
+    from bia_3d_gpu_amd import flops_per_step
+
+    dev = "cuda:0"
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+    s = torch.from_numpy(st_host).to(dev)
+    d = torch.empty_like(s)
+    # untimed stats step on the initial state: branch mix and iteration counts
+    stats = torch.zeros(16, dtype=torch.int64, device=dev)
+    ctx.step_(s, d, eps, dt, mode=mode, itr=itr, stats=stats)
+    torch.cuda.synchronize()
+    st = stats.cpu().numpy().astype(np.uint64)
+    flops = flops_per_step(st)
+    for _ in range(warmup):
+        ctx.step_(s, d, eps, dt, mode=mode, itr=itr)
+        s.add_(d)
+    ctx.set_profiling(True)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kms = []
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for k in range(steps):
+        ctx.step_(s, d, eps, dt, mode=mode, itr=itr)
+        kms.append(ctx.last_kernel_ms())  # waits for this step's pair kernel
+        s.add_(d)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    ms = e0.elapsed_time(e1) / steps
+    return dict(ms_per_step=ms, wall_s=wall, kernel_ms=float(np.mean(kms)), stats=st.tolist(), flops=flops)
+
+
+def cpu_baseline(st_host, eps, dt, itr, budget_s: float):
+    """Reference Fortran CPU pair loop (bia.f90:142-196 with the reference's find_* routines),
+    single thread, rows [0, r) of the same system until ~budget_s seconds have elapsed."""
+    import oracle as O
+
+    n = st_host.shape[1]
+    kind = "reference" if O.ref_available() else "port"
+    rows, pairs, t = 0, 0, 0.0
+    chunk = max(1, int(2e6 / n))
+    while t < budget_s and rows < n:
+        r1 = min(n, rows + chunk)
+        t0 = time.perf_counter()
+        if kind == "reference":
+            O.ref_cpu_rows(st_host, eps, dt, 1, itr, rows=(rows, r1))
+        else:
+            O.bia_step(st_host, eps, dt, 1, itr, rows=(rows, r1))
+        t += time.perf_counter() - t0
+        pairs += sum(n - 1 - i for i in range(rows, r1))
+        rows = r1
+    return dict(value=pairs / t, unit="pair-interactions/s", cores=1, kind=kind,
+                sample=f"rows 0..{rows - 1} of the same N={n} system ({pairs} pairs, {t:.1f} s), "
+                       f"one step, {'amdflang -O3 build of the reference Fortran CPU routines' if kind == 'reference' else 'C oracle'}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--seed", type=int, default=65536)
+    ap.add_argument("--dt", type=float, default=1e-3)
+    ap.add_argument("--itr", type=int, default=18)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the pair hot path of one system per GPU); "
+              "--gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from bia_3d_gpu_amd import Context, MODE_LEARN, MODE_PLAIN
+
+    assert torch.cuda.is_available()
+    n = args.n
+    eps = -2.0 / n
+    log(f"building state N={n} seed={args.seed}")
+    st = make_state(n, args.seed)
+    pairs = n * (n - 1) / 2
+    ctx = Context(0)
+    log("plain mode")
+    plain = run_mode(ctx, st, eps, args.dt, args.itr, MODE_PLAIN, args.steps, args.warmup)
+    plain["pairs_per_s"] = pairs / (plain["ms_per_step"] / 1e3)
+    log(f"plain: {plain['ms_per_step']:.2f} ms/step, kernel {plain['kernel_ms']:.2f} ms")
+    learned = None
+    try:
+        learned = run_mode(ctx, st, eps, args.dt, args.itr, MODE_LEARN, args.steps, args.warmup)
+        learned["pairs_per_s"] = pairs / (learned["ms_per_step"] / 1e3)
+        learned["speedup_vs_plain"] = learned["pairs_per_s"] / plain["pairs_per_s"]
+        log(f"learned: {learned['ms_per_step']:.2f} ms/step")
+    except Exception as e:  # learned weights absent: report plain only
+        log(f"learned mode unavailable: {e}")
+    head = learned if learned is not None else plain
+    # roofline of the pair kernel: useful work = the plain reference algorithm's flops
+    achieved = plain["flops"] / (head["kernel_ms"] / 1e3) / 1e12
+    roof = {"bound": "mfma", "pipe": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "flops_per_launch": plain["flops"], "kernel_ms": head["kernel_ms"]}
+    cpu = None if args.no_cpu else cpu_baseline(st, eps, args.dt, args.itr, args.cpu_budget)
+    out = {
+        "metric": METRIC, "value": head["pairs_per_s"], "unit": "pair-interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Plummer N={n} with 20% hard binaries, G=1, m=1/N, dt={args.dt}, one BIA-ALG "
+                               f"step per timed step, seed {args.seed}",
+                   "n": n, "dt": args.dt, "eps": eps, "itr": args.itr,
+                   "mode": "learned" if learned is not None else "plain"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "plain": {k: plain[k] for k in ("pairs_per_s", "ms_per_step", "kernel_ms", "stats")},
+        "learned": None if learned is None else {k: learned[k] for k in ("pairs_per_s", "ms_per_step", "kernel_ms",
+                                                                          "stats", "speedup_vs_plain")},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

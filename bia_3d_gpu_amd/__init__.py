@@ -1,0 +1,16 @@
+"""bia_3d_gpu_amd -- MI355X (gfx950) library for the pairwise hot path of the Binary
+Interaction Approximation N-body method.
+
+The compute lives in ``libbia_hip.so`` (hand-written HIP kernels, C ABI in
+``include/bia_hip.h``).  This package is the Python host side: ctypes bindings that mirror
+the reference's Fortran call interface (:mod:`.api`), the reference's file formats
+(:mod:`.io`) and seeded synthetic initial conditions (:mod:`.ic`).
+"""
+from . import io, ic  # noqa: F401
+from .api import (  # noqa: F401
+    METHOD_ALG, METHOD_RKF, MODE_LEARN, MODE_PLAIN, STATS_LEN, Context, find_dxyuv_kernel, flops_per_step,
+    invariance_gpu, reference_step, rkf_c,
+)
+from ._lib import LIB_PATH, load  # noqa: F401
+
+__version__ = "1.0.0"

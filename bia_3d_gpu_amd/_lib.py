@@ -1,0 +1,87 @@
+"""ctypes binding of libbia_hip.so (the gfx950 HIP library).
+
+There is no fallback: if the library has not been built (``python -c "import
+__graft_entry__ as g; g.build()"`` or ``make -C bia_3d_gpu_amd/csrc``) importing a
+function that needs it raises ``RuntimeError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbia_hip.so")
+
+BIA_OK, BIA_ERR_ARG, BIA_ERR_HIP, BIA_ERR_OOM, BIA_ERR_STATE, BIA_ERR_RKF = 0, 1, 2, 3, 4, 5
+METHOD_RKF, METHOD_ALG = 0, 1
+MODE_PLAIN, MODE_LEARN = 0, 1
+STATS_LEN = 16
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = C.POINTER(C.c_int)
+_dptr = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# name -> (restype, argtypes); every exported symbol declared in include/bia_hip.h
+SIGNATURES = {
+    "find_dxyuv_kernel_": (None, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                  _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "invariance_gpu_": (None, [C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp] + [_dptr] * 11),
+    "rkf_c_": (None, [C.c_int, C.c_double, _dptr, _dptr, _dp, _ip, _dptr, _dptr, _dp, _dp, _ip, _ip, _dptr,
+                      _dp, _dp, _dp, _dp]),
+    "sincos_": (None, [_dptr, _dptr, _dptr]),
+    "bia_version": (C.c_int, []),
+    "bia_error_string": (C.c_char_p, [C.c_int]),
+    "bia_last_error": (C.c_char_p, [_vp]),
+    "bia_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "bia_destroy": (C.c_int, [_vp]),
+    "bia_set_stream": (C.c_int, [_vp, _vp]),
+    "bia_synchronize": (C.c_int, [_vp]),
+    "bia_set_profiling": (C.c_int, [_vp, C.c_int]),
+    "bia_last_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "bia_step_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _vp, C.c_int,
+                               _vp, _vp]),
+    "bia_step_host": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "bia_invariants_dev": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_int, _vp]),
+    "bia_invariants_host": (C.c_int, [_vp, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "bia_dim_rkf45_host": (C.c_int, [_vp, C.c_int, C.c_double, _dptr, C.c_double, _dp, _ip, C.c_double,
+                                     C.c_double, _dptr, _vp, _ip, _ip, _ip, _ip]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libbia_hip.so (raises RuntimeError when it is missing or incomplete)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `make -C bia_3d_gpu_amd/csrc` (hipcc, gfx950)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)  # AttributeError here means an export is missing
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def declared_symbols() -> list[str]:
+    return list(SIGNATURES)
+
+
+class BiaError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str = ""):
+        msg = load().bia_error_string(status).decode()
+        super().__init__(f"{where}: {msg}" + (f" ({detail})" if detail else ""))
+        self.status = status
+
+
+def check(status: int, where: str, ctx=None):
+    if status != BIA_OK:
+        detail = load().bia_last_error(ctx).decode() if ctx else ""
+        raise BiaError(status, where, detail)

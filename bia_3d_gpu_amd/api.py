@@ -1,0 +1,220 @@
+"""Python host interface of libbia_hip.so.
+
+Two layers, mirroring include/bia_hip.h:
+
+* reference-named wrappers (``find_dxyuv_kernel``, ``invariance_gpu``, ``rkf_c``) with the
+  argument meaning of the reference's Fortran call sites (bia/bia.f90:67,135-139;
+  dim/dim_3D.f90:30), operating in place on numpy arrays like the Fortran does;
+* :class:`Context` for the device-resident whole-step, invariant and DIM entry points.
+
+State layout for the new entry points: structure of arrays ``[6, n]`` (x, y, z, u, v, w).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import METHOD_ALG, METHOD_RKF, MODE_LEARN, MODE_PLAIN, STATS_LEN, check, load
+
+__all__ = ["find_dxyuv_kernel", "invariance_gpu", "rkf_c", "reference_step", "Context",
+           "METHOD_ALG", "METHOD_RKF", "MODE_PLAIN", "MODE_LEARN", "STATS_LEN", "flops_per_step"]
+
+INVARIANT_NAMES = ("KE", "PE", "Lx", "Ly", "Lz", "Ua", "Va", "Wa", "Xa", "Ya", "Za")
+
+
+def _f64(a) -> np.ndarray:
+    a = np.asarray(a)
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous float64 array (updated in place)")
+    return a
+
+
+# ------------------------------------------------------------------------------------------------
+# reference-named wrappers
+# ------------------------------------------------------------------------------------------------
+def find_dxyuv_kernel(METHOD: int, N: int, I: int, EPS: float, DT: float, ITR: int,
+                      dX, dY, dZ, dU, dV, dW, RXYZUVW, dXYZUVW) -> None:
+    """Per-particle pair kernel, bia/find_dXYUV_kernel.cu:526-561 semantics.
+
+    ``I`` is 1-based; ``RXYZUVW[6j:6j+6] = state_I - state_j``.  Accumulates the pair deltas into
+    ``dX..dW`` in place (+ at I-1, - at j > I-1) and writes ``dXYZUVW[6j:6j+6]`` for j > I-1.
+    """
+    arrs = [_f64(a) for a in (dX, dY, dZ, dU, dV, dW, RXYZUVW, dXYZUVW)]
+    if any(a.size < N for a in arrs[:6]) or arrs[6].size < 6 * N or arrs[7].size < 6 * N:
+        raise ValueError("array shorter than N")
+    load().find_dxyuv_kernel_(int(METHOD), int(N), int(I), float(EPS), float(DT), int(ITR), *arrs)
+
+
+def invariance_gpu(N: int, eps_th: float, X, Y, Z, U, V, W) -> dict:
+    """Invariants, bia/invariance.cu:50-110; returns KE, PE, Lx, Ly, Lz, Ua, Va, Wa, Xa, Ya, Za."""
+    cols = [np.ascontiguousarray(a, dtype=np.float64) for a in (X, Y, Z, U, V, W)]
+    if any(c.size < N for c in cols):
+        raise ValueError("array shorter than N")
+    outs = [C.c_double() for _ in range(11)]
+    load().invariance_gpu_(int(N), float(eps_th), *cols, *[C.byref(o) for o in outs])
+    return {k: o.value for k, o in zip(INVARIANT_NAMES, outs)}
+
+
+def rkf_c(neq: int, eps_th: float, X0: float, XE: float, YH0, INIT: int = 1, RELERR: float = 0.0,
+          ABSERR: float = 1e-16, H: float = 0.0) -> dict:
+    """DIM direct-sum RKF45, dim/rkf_kernel.cu:194-495 semantics (dim/dim_3D.f90:30 call).
+
+    Note: as in the reference, an integration failure prints the reason and exits the process
+    with status 0; use :meth:`Context.dim_rkf45` for an error return instead.
+    """
+    yh0 = np.ascontiguousarray(YH0, dtype=np.float64).copy()
+    yhn = np.zeros_like(yh0)
+    ese = np.zeros_like(yh0)
+    wl0, wln, werr = np.zeros_like(yh0), np.zeros_like(yh0), np.zeros_like(yh0)
+    wrk = np.zeros(11 * neq)
+    x0, xe, h = C.c_double(X0), C.c_double(XE), C.c_double(H)
+    init, nde, ier = C.c_int(INIT), C.c_int(0), C.c_int(0)
+    rel, ab = C.c_double(RELERR), C.c_double(ABSERR)
+    load().rkf_c_(int(neq), float(eps_th), C.byref(x0), C.byref(xe), yh0, C.byref(init), C.byref(rel),
+                  C.byref(ab), yhn, ese, C.byref(nde), C.byref(ier), C.byref(h), wl0, wln, werr, wrk)
+    return dict(YHN=yhn, YH0=yh0, ESE=ese, X0=x0.value, NDE=nde.value, IER=ier.value, H=h.value, INIT=init.value)
+
+
+def reference_step(state_soa: np.ndarray, eps: float, dt: float, method: int = METHOD_ALG,
+                   itr: int = 18) -> np.ndarray:
+    """One BIA step driven exactly like the reference host loop (bia/bia.f90:126-196): for each
+    particle i, build the relative AoS row, call ``find_dxyuv_kernel``, then add the drift.
+    Returns the deltas ``[6, n]``."""
+    st = np.ascontiguousarray(state_soa, dtype=np.float64).reshape(6, -1)
+    n = st.shape[1]
+    d = np.zeros((6, n))
+    dX, dY, dZ, dU, dV, dW = (np.ascontiguousarray(d[c]) for c in range(6))
+    rel = np.zeros(6 * n)
+    out = np.zeros(6 * n)
+    for i in range(n):
+        rel.reshape(n, 6)[:] = (st[:, i:i + 1] - st).T
+        find_dxyuv_kernel(method, n, i + 1, eps, dt, itr, dX, dY, dZ, dU, dV, dW, rel, out)
+        dX[i] += st[3, i] * dt
+        dY[i] += st[4, i] * dt
+        dZ[i] += st[5, i] * dt
+    return np.stack([dX, dY, dZ, dU, dV, dW])
+
+
+# ------------------------------------------------------------------------------------------------
+# algorithmic work per step (flop convention: + - x / sqrt = 1, transcendental = 20)
+# ------------------------------------------------------------------------------------------------
+def flops_per_step(stats) -> float:
+    """Algorithmic FP64 flops of one step from its BIA_STATS counters.
+
+    Per pair, with k = Newton corrections after the two set-up evaluations (hyperbola) or
+    Newton corrections (ellipse, parabola), accepted steps s (RKF), counted from the reference
+    GPU source: hyperbola 635 + 122 k, ellipse 524 + 47 k, parabola 472 + 76 k, RKF 80 + 790 s.
+    """
+    s = np.asarray(stats, dtype=np.float64)
+    n_h, n_p, n_e, n_r = s[0:4]
+    it_h, it_p, it_e, it_r = s[4:8]
+    k_h = it_h - 2 * n_h
+    return (635 * n_h + 122 * k_h) + (472 * n_p + 76 * it_p) + (524 * n_e + 47 * it_e) + (80 * n_r + 790 * it_r)
+
+
+# ------------------------------------------------------------------------------------------------
+# context API
+# ------------------------------------------------------------------------------------------------
+class Context:
+    """Owns a bia_ctx (device workspace, stream).  One per host thread."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        h = C.c_void_p()
+        check(L.bia_create(C.byref(h), int(device)), "bia_create")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().bia_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_handle: int):
+        check(load().bia_set_stream(self._h, C.c_void_p(int(stream_handle))), "bia_set_stream", self._h)
+
+    def set_profiling(self, on: bool = True):
+        check(load().bia_set_profiling(self._h, int(on)), "bia_set_profiling", self._h)
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float()
+        check(load().bia_last_kernel_ms(self._h, C.byref(ms)), "bia_last_kernel_ms", self._h)
+        return ms.value
+
+    def synchronize(self):
+        check(load().bia_synchronize(self._h), "bia_synchronize", self._h)
+
+    # -- whole step on device memory (torch tensors) -------------------------------------------
+    def step_(self, state, delta, eps: float, dt: float, method: int = METHOD_ALG, mode: int = MODE_PLAIN,
+              itr: int = 18, stats=None):
+        """Device step: ``state`` and ``delta`` are CUDA/HIP float64 tensors ``[6, ld]``."""
+        import torch
+
+        assert state.dtype == torch.float64 and delta.dtype == torch.float64
+        assert state.is_cuda and delta.is_cuda and state.is_contiguous() and delta.is_contiguous()
+        n = state.shape[1]
+        assert delta.shape == state.shape
+        sp = 0 if stats is None else stats.data_ptr()
+        check(load().bia_step_dev(self._h, method, mode, n, eps, dt, itr, state.data_ptr(), n, delta.data_ptr(),
+                                  sp or None), "bia_step_dev", self._h)
+        return delta
+
+    # -- whole step on host arrays --------------------------------------------------------------
+    def step_host(self, state_soa: np.ndarray, eps: float, dt: float, method: int = METHOD_ALG,
+                  mode: int = MODE_PLAIN, itr: int = 18, with_stats: bool = False):
+        st = np.ascontiguousarray(state_soa, dtype=np.float64).reshape(6, -1)
+        n = st.shape[1]
+        cols = [np.ascontiguousarray(st[c]) for c in range(6)]
+        outs = [np.zeros(n) for _ in range(6)]
+        stats = np.zeros(STATS_LEN, dtype=np.uint64)
+        check(load().bia_step_host(self._h, method, mode, n, eps, dt, itr, *cols, *outs,
+                                   stats.ctypes.data_as(C.c_void_p)), "bia_step_host", self._h)
+        d = np.stack(outs)
+        return (d, stats) if with_stats else d
+
+    def invariants(self, state_soa: np.ndarray, eps: float) -> dict:
+        st = np.ascontiguousarray(state_soa, dtype=np.float64).reshape(6, -1)
+        out = np.zeros(11)
+        cols = [np.ascontiguousarray(st[c]) for c in range(6)]
+        check(load().bia_invariants_host(self._h, st.shape[1], eps, *cols, out), "bia_invariants_host", self._h)
+        return dict(zip(INVARIANT_NAMES, out.tolist()))
+
+    def invariants_(self, state, out, eps: float):
+        """Device invariants: ``state`` [6, n] float64 tensor, ``out`` 11-element float64 tensor."""
+        check(load().bia_invariants_dev(self._h, state.shape[1], eps, state.data_ptr(), state.shape[1],
+                                        out.data_ptr()), "bia_invariants_dev", self._h)
+        return out
+
+    def dim_rkf45(self, y_aos: np.ndarray, eps: float, x0: float, xe: float, relerr: float = 0.0,
+                  abserr: float = 1e-16, h: float = 0.0, init: int = 1) -> dict:
+        y = np.ascontiguousarray(y_aos, dtype=np.float64).reshape(-1).copy()
+        ese = np.zeros_like(y)
+        x = C.c_double(x0)
+        hh = C.c_double(h)
+        ini = C.c_int(init)
+        nde, nacc, nrej, ier = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        s = load().bia_dim_rkf45_host(self._h, y.size // 6, eps, C.byref(x), xe, y, C.byref(ini), relerr, abserr,
+                                      C.byref(hh), ese.ctypes.data_as(C.c_void_p), C.byref(nde), C.byref(nacc),
+                                      C.byref(nrej), C.byref(ier))
+        if s not in (_lib.BIA_OK, _lib.BIA_ERR_RKF):
+            check(s, "bia_dim_rkf45_host", self._h)
+        return dict(y=y, ese=ese, x=x.value, h=hh.value, nde=nde.value, naccept=nacc.value,
+                    nreject=nrej.value, ier=ier.value)

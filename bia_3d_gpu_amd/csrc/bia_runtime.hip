@@ -26,6 +26,19 @@ void legacy_fail(const char* where, int code, bia_ctx* ctx) {
   abort();
 }
 
+int prof_begin(bia_ctx* ctx) {
+  if (!ctx->profiling) return BIA_OK;
+  BIA_HIP_TRY(ctx, hipEventRecord(ctx->ev_begin, ctx->stream));
+  return BIA_OK;
+}
+
+int prof_end(bia_ctx* ctx) {
+  if (!ctx->profiling) return BIA_OK;
+  BIA_HIP_TRY(ctx, hipEventRecord(ctx->ev_end, ctx->stream));
+  ctx->ev_pending = 1;
+  return BIA_OK;
+}
+
 int ensure_tiles(bia_ctx* ctx, int nt) {
   if (ctx->tiles_nt == nt) return BIA_OK;
   const size_t ntiles = (size_t)nt * (nt + 1) / 2;
@@ -84,6 +97,8 @@ int bia_destroy(bia_ctx* c) {
                          &c->inv_state, &c->inv_part, &c->inv_out, &c->dim_work, &c->dim_ctl,
                          &c->mlp_w, &c->mlp_ws};
   for (auto* b : bufs) b->release();
+  if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
+  if (c->ev_end) (void)hipEventDestroy(c->ev_end);
   delete c;
   return BIA_OK;
 }
@@ -92,6 +107,28 @@ int bia_set_stream(bia_ctx* c, void* stream) {
   if (!c) return BIA_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
   c->stream = static_cast<hipStream_t>(stream);
+  return BIA_OK;
+}
+
+int bia_set_profiling(bia_ctx* c, int on) {
+  if (!c) return BIA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  BIA_HIP_TRY(c, hipSetDevice(c->device));
+  if (on && !c->ev_begin) {
+    BIA_HIP_TRY(c, hipEventCreate(&c->ev_begin));
+    BIA_HIP_TRY(c, hipEventCreate(&c->ev_end));
+  }
+  c->profiling = on ? 1 : 0;
+  c->ev_pending = 0;
+  return BIA_OK;
+}
+
+int bia_last_kernel_ms(bia_ctx* c, float* ms) {
+  if (!c || !ms) return BIA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!c->ev_pending) return BIA_ERR_STATE;
+  BIA_HIP_TRY(c, hipEventSynchronize(c->ev_end));
+  BIA_HIP_TRY(c, hipEventElapsedTime(ms, c->ev_begin, c->ev_end));
   return BIA_OK;
 }
 

@@ -59,6 +59,10 @@ struct bia_ctx {
   // learned initialiser weights (device copy) and batch workspace
   bia::DevBuf mlp_w, mlp_ws;
   int mlp_ready = 0;
+  // optional timing of the dominant (pair) kernel with HIP events on the context's stream
+  int profiling = 0;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  int ev_pending = 0;
   char last_error[256] = {0};
 };
 
@@ -88,6 +92,9 @@ namespace bia {
 bia_ctx* default_ctx();
 // abort with a message: the legacy symbols have no error return, so failures are loud
 [[noreturn]] void legacy_fail(const char* where, int code, bia_ctx* ctx);
+// record the profiling events around the pair kernel (no-ops unless profiling is on)
+int prof_begin(bia_ctx* ctx);
+int prof_end(bia_ctx* ctx);
 // upper-triangle list of 64-wide tile pairs (I <= J) for nt tiles, cached on the context
 int ensure_tiles(bia_ctx* ctx, int nt);
 // learned-initialiser step (bia_learn.hip); caller holds ctx->mu

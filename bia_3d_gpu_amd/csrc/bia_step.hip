@@ -165,6 +165,7 @@ static int step_dev_locked(bia_ctx* ctx, int method, int mode, int n, double eps
   const int2* tiles = ctx->tiles.as<int2>();
   dim3 g(ntiles), b(kWave);
   const bool stats = d_stats != nullptr;
+  BIA_TRY(ctx, prof_begin(ctx));
   if (method == BIA_METHOD_ALG) {
     if (stats)
       hipLaunchKernelGGL((k_step_tiles<1, true>), g, b, 0, ctx->stream, n, ld, ldp, eps, dt, itr, d_state, tiles,
@@ -181,6 +182,7 @@ static int step_dev_locked(bia_ctx* ctx, int method, int mode, int n, double eps
                          part, d_stats);
   }
   BIA_HIP_TRY(ctx, hipGetLastError());
+  BIA_TRY(ctx, prof_end(ctx));
   hipLaunchKernelGGL(k_step_reduce, dim3((n + 255) / 256, 6), dim3(256), 0, ctx->stream, n, ld, ldp, ld, nt, dt,
                      d_state, part, d_delta);
   BIA_HIP_TRY(ctx, hipGetLastError());

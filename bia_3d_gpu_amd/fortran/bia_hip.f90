@@ -1,0 +1,53 @@
+! bia_hip.f90 -- ISO_C_BINDING interface to libbia_hip.so's new entry points
+! (include/bia_hip.h).  The reference-compatible symbols (find_dxyuv_kernel_, invariance_gpu_,
+! rkf_c_) need no interface: the reference's implicit %VAL calls bind to them directly.
+module bia_hip
+  use iso_c_binding
+  implicit none
+  integer(c_int), parameter :: BIA_OK = 0, BIA_METHOD_RKF = 0, BIA_METHOD_ALG = 1
+  integer(c_int), parameter :: BIA_MODE_PLAIN = 0, BIA_MODE_LEARN = 1, BIA_STATS_LEN = 16
+
+  interface
+    integer(c_int) function bia_create(ctx, device) bind(C, name='bia_create')
+      import :: c_int, c_ptr
+      type(c_ptr) :: ctx
+      integer(c_int), value :: device
+    end function
+
+    integer(c_int) function bia_destroy(ctx) bind(C, name='bia_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function bia_step_host(ctx, method, mode, n, eps, dt, itr, X, Y, Z, U, V, W, &
+                                          dX, dY, dZ, dU, dV, dW, stats) bind(C, name='bia_step_host')
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: method, mode, n, itr
+      real(c_double), value :: eps, dt
+      real(c_double), intent(in) :: X(*), Y(*), Z(*), U(*), V(*), W(*)
+      real(c_double), intent(out) :: dX(*), dY(*), dZ(*), dU(*), dV(*), dW(*)
+      integer(c_long_long), intent(out) :: stats(*)
+    end function
+
+    integer(c_int) function bia_invariants_host(ctx, n, eps, X, Y, Z, U, V, W, out) &
+        bind(C, name='bia_invariants_host')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+      real(c_double), value :: eps
+      real(c_double), intent(in) :: X(*), Y(*), Z(*), U(*), V(*), W(*)
+      real(c_double), intent(out) :: out(11)
+    end function
+
+    integer(c_int) function bia_dim_rkf45_host(ctx, np, eps, x0, xe, y, init, relerr, abserr, h, ese, &
+                                               nde, naccept, nreject, ier) bind(C, name='bia_dim_rkf45_host')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: np
+      real(c_double), value :: eps, xe, relerr, abserr
+      real(c_double) :: x0, y(*), h, ese(*)
+      integer(c_int) :: init, nde, naccept, nreject, ier
+    end function
+  end interface
+end module bia_hip

@@ -101,6 +101,11 @@ int bia_destroy(bia_ctx *ctx);
 /* stream is a hipStream_t (0 = null stream); subsequent calls are ordered on it */
 int bia_set_stream(bia_ctx *ctx, void *stream);
 int bia_synchronize(bia_ctx *ctx);
+/* Timing of the dominant kernel.  With profiling on, each step records HIP events on the
+ * context's stream immediately around its pair kernel; bia_last_kernel_ms waits for the last
+ * such pair and returns the elapsed milliseconds (BIA_ERR_STATE if none was recorded). */
+int bia_set_profiling(bia_ctx *ctx, int on);
+int bia_last_kernel_ms(bia_ctx *ctx, float *ms);
 
 /* One whole BIA step of every unordered pair of an n-particle system.  No counterpart in the
  * reference ABI: it replaces the per-particle loop of bia/bia.f90:128-202 (N calls of

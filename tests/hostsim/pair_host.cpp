@@ -1,0 +1,21 @@
+// pair_host.cpp -- test-only host build of the library's per-pair device code
+// (bia_3d_gpu_amd/csrc/bia_pair.hpp), so its control flow can be checked against the oracle on
+// the CPU.  Never loaded by the package; built by __graft_entry__.build() into tests/hostsim/.
+#include "../../bia_3d_gpu_amd/csrc/bia_pair.hpp"
+
+extern "C" void hs_pairs(int method, int64_t n, double eps, double dt, int itr, const double* rel, double* out,
+                         int* branch, int* iters, unsigned* flags) {
+  double hist[bia::kMaxItr];
+  for (int64_t k = 0; k < n; k++) {
+    bia::Hist h{hist, 1};
+    int br = 0, it = 0;
+    unsigned fl = 0;
+    if (method == 1)
+      bia::pair_plain<1>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h);
+    else
+      bia::pair_plain<0>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h);
+    branch[k] = br;
+    iters[k] = it;
+    flags[k] = fl;
+  }
+}

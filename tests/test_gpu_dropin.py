@@ -1,0 +1,62 @@
+"""The reference's unmodified Fortran host programs (bia/bia.f90, dim/dim_3D.f90), compiled by
+oracle/Makefile and linked against libbia_hip.so, run on the reference's N=1332 fixture in the
+directory layout they expect (../OUT relative to the working directory)."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+REF_BIN = os.path.join(ROOT, "oracle", "_ref")
+
+
+def _layout(tmp_path, with_dim=True):
+    out = tmp_path / "OUT"
+    run = tmp_path / "run"
+    out.mkdir()
+    run.mkdir()
+    names = ["0001332_10^0.bbb"] + (["0001332_10^0.DIM"] if with_dim else [])
+    for nm in names:
+        shutil.copy(os.path.join(GOLDEN, nm), out / nm)
+    return out, run
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_BIN, "bia_gpu_alg")), reason="reference build absent")
+def test_reference_bia_f90_on_library(tmp_path):
+    from bia_3d_gpu_amd import io
+    out, run = _layout(tmp_path)
+    p = subprocess.run([os.path.join(REF_BIN, "bia_gpu_alg")], cwd=run, capture_output=True, text=True, timeout=600)
+    print(p.stdout[-3000:])
+    assert p.returncode == 0, p.stderr[-2000:]
+    (dt, got), = io.read_timed(str(out / "0001332_10^0.BIA"))
+    (_, ref), = io.read_timed(os.path.join(GOLDEN, "0001332_10^0.BIA"))
+    dp, dv = np.abs(got[:3] - ref[:3]).max(), np.abs(got[3:] - ref[3:]).max()
+    print(f"reference bia.f90 + libbia_hip vs OUT/0001332_10^0.BIA: pos {dp:.3e} vel {dv:.3e}")
+    assert dt == 1.0 and dp < 3e-11 and dv < 1e-14
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_BIN, "dim_gpu")), reason="reference build absent")
+def test_reference_dim_f90_on_library(tmp_path):
+    from bia_3d_gpu_amd import io
+    out, run = _layout(tmp_path, with_dim=False)
+    trace = tmp_path / "trace.DIM"
+    env = dict(os.environ, BIA_DIM_TRACE=str(trace))
+    p = subprocess.run([os.path.join(REF_BIN, "dim_gpu")], cwd=run, capture_output=True, text=True, timeout=600,
+                       env=env)
+    print(p.stdout[-3000:])
+    assert p.returncode == 0, p.stderr[-2000:]
+    (t, got), = io.read_timed(str(out / "0001332_10^0.DIM"))
+    (_, ref), = io.read_timed(os.path.join(GOLDEN, "0001332_10^0.DIM"))
+    dp, dv = np.abs(got[:3] - ref[:3]).max(), np.abs(got[3:] - ref[3:]).max()
+    print(f"reference dim_3D.f90 + libbia_hip vs OUT/0001332_10^0.DIM: pos {dp:.3e} vel {dv:.3e}")
+    assert t == 1.0 and dp < 1e-13 and dv < 1e-13
+    mine = np.loadtxt(trace, skiprows=1)
+    theirs = np.loadtxt(os.path.join(GOLDEN, "0001332.DIM"), skiprows=1)
+    assert mine.shape == theirs.shape
+    # accepted step times: the controller works on error ratios at the rounding floor
+    # (ABSERR 1e-16), so step sizes agree to ~1e-9 relative, not to the last bit
+    assert np.abs(mine[:, 0] - theirs[:, 0]).max() < 1e-7
