@@ -5,8 +5,13 @@ G = 1, m = 1/N) with 20 % of the particles in hard binaries (a in [1e-3, 1e-2], 
 dt = 1e-3, eps = -2Gm, seed 65536.  One "step" = one BIA step of every unordered pair
 (N(N-1)/2 pair interactions) followed by the state update, state resident in HBM.
 
-Printed: one JSON line (rank 0).  ``value`` is the pair-interactions/s of the headline mode
-(learned initialiser when trained weights are available, else plain).  ``roofline`` prices the
+Modes measured on the same input: plain (the reference's guess and iteration control), guess (the
+reference's closed-form guess + bracketed Newton with a rounding-aware stop, no network) and
+learned (the network's guess + the same bracketed Newton).  The network is trained on the GPU
+first, from samples the solver records on other systems (training time reported separately).
+
+Printed: one JSON line (rank 0).  ``value`` is the pair-interactions/s of the learned mode
+(plain if training is skipped with --no-learn).  ``roofline`` prices the
 dominant kernel (the pair kernel) with the algorithmic FP64 flop count of the reference
 algorithm (stats counters of an untimed step on the same input) over its average duration,
 measured with HIP events recorded on the kernel's own stream.  ``cpu_baseline`` times the
@@ -110,6 +115,9 @@ def main():
     ap.add_argument("--itr", type=int, default=18)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learn", action="store_true")
+    ap.add_argument("--train-samples", type=int, default=10_000_000)
+    ap.add_argument("--train-steps", type=int, default=3000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the pair hot path of one system per GPU); "
@@ -118,7 +126,7 @@ def main():
 
     import torch
 
-    from bia_3d_gpu_amd import Context, MODE_LEARN, MODE_PLAIN
+    from bia_3d_gpu_amd import Context, MODE_GUESS, MODE_LEARN, MODE_PLAIN, learn
 
     assert torch.cuda.is_available()
     n = args.n
@@ -127,19 +135,26 @@ def main():
     st = make_state(n, args.seed)
     pairs = n * (n - 1) / 2
     ctx = Context(0)
-    log("plain mode")
-    plain = run_mode(ctx, st, eps, args.dt, args.itr, MODE_PLAIN, args.steps, args.warmup)
-    plain["pairs_per_s"] = pairs / (plain["ms_per_step"] / 1e3)
-    log(f"plain: {plain['ms_per_step']:.2f} ms/step, kernel {plain['kernel_ms']:.2f} ms")
-    learned = None
-    try:
-        learned = run_mode(ctx, st, eps, args.dt, args.itr, MODE_LEARN, args.steps, args.warmup)
-        learned["pairs_per_s"] = pairs / (learned["ms_per_step"] / 1e3)
-        learned["speedup_vs_plain"] = learned["pairs_per_s"] / plain["pairs_per_s"]
-        log(f"learned: {learned['ms_per_step']:.2f} ms/step")
-    except Exception as e:  # learned weights absent: report plain only
-        log(f"learned mode unavailable: {e}")
-    head = learned if learned is not None else plain
+    res = {}
+    for name, mode in (("plain", MODE_PLAIN), ("guess", MODE_GUESS)):
+        r = run_mode(ctx, st, eps, args.dt, args.itr, mode, args.steps, args.warmup)
+        r["pairs_per_s"] = pairs / (r["ms_per_step"] / 1e3)
+        res[name] = r
+        log(f"{name}: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms")
+    train = None
+    if not args.no_learn:
+        t0 = time.perf_counter()
+        w, curve, rows = learn.train_default(ctx, nsamples=args.train_samples, steps=args.train_steps, log=log)
+        torch.cuda.synchronize()
+        train = {"seconds": time.perf_counter() - t0, "samples": int(rows.shape[0]), "steps": args.train_steps,
+                 "loss_first": float(curve[0]), "loss_last": float(np.mean(curve[-20:]))}
+        del rows
+        r = run_mode(ctx, st, eps, args.dt, args.itr, MODE_LEARN, args.steps, args.warmup)
+        r["pairs_per_s"] = pairs / (r["ms_per_step"] / 1e3)
+        res["learned"] = r
+        log(f"learned: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms")
+    plain = res["plain"]
+    head = res.get("learned", plain)
     # roofline of the pair kernel: useful work = the plain reference algorithm's flops
     achieved = plain["flops"] / (head["kernel_ms"] / 1e3) / 1e12
     roof = {"bound": "mfma", "pipe": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
@@ -154,12 +169,12 @@ def main():
         "config": {"workload": f"Plummer N={n} with 20% hard binaries, G=1, m=1/N, dt={args.dt}, one BIA-ALG "
                                f"step per timed step, seed {args.seed}",
                    "n": n, "dt": args.dt, "eps": eps, "itr": args.itr,
-                   "mode": "learned" if learned is not None else "plain"},
+                   "mode": "learned" if "learned" in res else "plain"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "plain": {k: plain[k] for k in ("pairs_per_s", "ms_per_step", "kernel_ms", "stats")},
-        "learned": None if learned is None else {k: learned[k] for k in ("pairs_per_s", "ms_per_step", "kernel_ms",
-                                                                          "stats", "speedup_vs_plain")},
+        "modes": {k: {q: v[q] for q in ("pairs_per_s", "ms_per_step", "kernel_ms", "stats")} for k, v in res.items()},
+        "speedup_vs_plain": {k: v["pairs_per_s"] / plain["pairs_per_s"] for k, v in res.items()},
+        "training": train,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

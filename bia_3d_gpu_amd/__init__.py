@@ -8,9 +8,9 @@ the reference's Fortran call interface (:mod:`.api`), the reference's file forma
 """
 from . import io, ic  # noqa: F401
 from .api import (  # noqa: F401
-    METHOD_ALG, METHOD_RKF, MODE_LEARN, MODE_PLAIN, STATS_LEN, Context, find_dxyuv_kernel, flops_per_step,
+    METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, STATS_LEN, Context, find_dxyuv_kernel, flops_per_step,
     invariance_gpu, reference_step, rkf_c,
 )
-from ._lib import LIB_PATH, load  # noqa: F401
+from ._lib import COUPLING_COULOMB, COUPLING_GRAVITY, COUPLING_SCALAR, LIB_PATH, load  # noqa: F401
 
 __version__ = "1.0.0"

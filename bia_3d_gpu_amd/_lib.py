@@ -16,10 +16,13 @@ LIB_PATH = os.path.join(HERE, "libbia_hip.so")
 
 BIA_OK, BIA_ERR_ARG, BIA_ERR_HIP, BIA_ERR_OOM, BIA_ERR_STATE, BIA_ERR_RKF = 0, 1, 2, 3, 4, 5
 METHOD_RKF, METHOD_ALG = 0, 1
-MODE_PLAIN, MODE_LEARN = 0, 1
+MODE_PLAIN, MODE_LEARN, MODE_GUESS = 0, 1, 2
 STATS_LEN = 16
+COUPLING_SCALAR, COUPLING_GRAVITY, COUPLING_COULOMB = 0, 1, 2
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _ip = C.POINTER(C.c_int)
 _dptr = C.POINTER(C.c_double)
 _vp = C.c_void_p
@@ -45,8 +48,24 @@ SIGNATURES = {
                                _vp, _vp]),
     "bia_step_host": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "bia_step_batched_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _ip32, _dp, C.c_double, C.c_double,
+                                       C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "bia_autotune_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _ip32, _dp, C.c_double, C.c_double,
+                                   C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.POINTER(C.c_float), _ip]),
+    "bia_run_steps_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _ip32, _dp, C.c_double, C.c_double,
+                                    C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int]),
     "bia_invariants_dev": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_int, _vp]),
     "bia_invariants_host": (C.c_int, [_vp, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "bia_mlp_nparam": (C.c_int, []),
+    "bia_mlp_init": (C.c_int, [_vp, C.c_ulonglong]),
+    "bia_mlp_set_weights": (C.c_int, [_vp, _fp, C.c_int]),
+    "bia_mlp_get_weights": (C.c_int, [_vp, _fp, C.c_int]),
+    "bia_mlp_forward_dev": (C.c_int, [_vp, _vp, C.c_int, _vp]),
+    "bia_mlp_grad_dev": (C.c_int, [_vp, _vp, C.c_int, _vp]),
+    "bia_mlp_train_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _vp]),
+    "bia_mlp_build_table": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "bia_record_samples_dev": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int,
+                                         C.c_ulonglong, _vp]),
     "bia_dim_rkf45_host": (C.c_int, [_vp, C.c_int, C.c_double, _dptr, C.c_double, _dp, _ip, C.c_double,
                                      C.c_double, _dptr, _vp, _ip, _ip, _ip, _ip]),
 }
@@ -61,6 +80,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built: run `make -C bia_3d_gpu_amd/csrc` (hipcc, gfx950)")
+    # PyTorch-ROCm ships its own HIP runtime under the same SONAME (libamdhip64.so.7).  Loading
+    # torch first makes this library bind to that runtime, so a process holds one HIP runtime
+    # whichever of the two touches the GPU first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)  # AttributeError here means an export is missing

@@ -16,10 +16,10 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import METHOD_ALG, METHOD_RKF, MODE_LEARN, MODE_PLAIN, STATS_LEN, check, load
+from ._lib import METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, STATS_LEN, check, load
 
 __all__ = ["find_dxyuv_kernel", "invariance_gpu", "rkf_c", "reference_step", "Context",
-           "METHOD_ALG", "METHOD_RKF", "MODE_PLAIN", "MODE_LEARN", "STATS_LEN", "flops_per_step"]
+           "METHOD_ALG", "METHOD_RKF", "MODE_PLAIN", "MODE_LEARN", "MODE_GUESS", "STATS_LEN", "flops_per_step"]
 
 INVARIANT_NAMES = ("KE", "PE", "Lx", "Ly", "Lz", "Ua", "Va", "Wa", "Xa", "Ya", "Za")
 
@@ -201,6 +201,85 @@ class Context:
         """Device invariants: ``state`` [6, n] float64 tensor, ``out`` 11-element float64 tensor."""
         check(load().bia_invariants_dev(self._h, state.shape[1], eps, state.data_ptr(), state.shape[1],
                                         out.data_ptr()), "bia_invariants_dev", self._h)
+        return out
+
+    # -- batched / coupled steps, autotuner, multi-step driver ---------------------------------
+    def _batched_args(self, sizes, eps, coupling):
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        e = np.ascontiguousarray(np.zeros(len(sizes)) if eps is None else np.broadcast_to(eps, len(sizes)),
+                                 dtype=np.float64)
+        return sizes, e
+
+    def step_batched_(self, state, delta, sizes, eps=None, dt: float = 1e-3, method: int = METHOD_ALG,
+                      mode: int = MODE_PLAIN, coupling: int = 0, coup: float = 1.0, mass=None, charge=None,
+                      itr: int = 18, stats=None):
+        """Systems of the given sizes concatenated column-wise in state/delta ([6, ld] CUDA f64)."""
+        sz, e = self._batched_args(sizes, eps, coupling)
+        ptr = lambda t: None if t is None else t.data_ptr()
+        check(load().bia_step_batched_dev(self._h, method, mode, coupling, len(sz), sz, e, coup, dt, itr,
+                                          state.data_ptr(), state.shape[1], ptr(mass), ptr(charge), delta.data_ptr(),
+                                          ptr(stats)), "bia_step_batched_dev", self._h)
+        return delta
+
+    def autotune_(self, state, delta, sizes, eps=None, dt: float = 1e-3, method: int = METHOD_ALG,
+                  mode: int = MODE_PLAIN, coupling: int = 0, coup: float = 1.0, mass=None, charge=None, itr: int = 18):
+        sz, e = self._batched_args(sizes, eps, coupling)
+        ptr = lambda t: None if t is None else t.data_ptr()
+        ms, g = C.c_float(), C.c_int()
+        check(load().bia_autotune_dev(self._h, method, mode, coupling, len(sz), sz, e, coup, dt, itr,
+                                      state.data_ptr(), state.shape[1], ptr(mass), ptr(charge), delta.data_ptr(),
+                                      C.byref(ms), C.byref(g)), "bia_autotune_dev", self._h)
+        return ms.value, g.value
+
+    def run_steps_(self, state, delta, sizes, nsteps: int, eps=None, dt: float = 1e-3, method: int = METHOD_ALG,
+                   mode: int = MODE_PLAIN, coupling: int = 0, coup: float = 1.0, mass=None, charge=None,
+                   itr: int = 18, use_graph: bool = True):
+        sz, e = self._batched_args(sizes, eps, coupling)
+        ptr = lambda t: None if t is None else t.data_ptr()
+        check(load().bia_run_steps_dev(self._h, method, mode, coupling, len(sz), sz, e, coup, dt, itr,
+                                       state.data_ptr(), state.shape[1], ptr(mass), ptr(charge), delta.data_ptr(),
+                                       nsteps, int(use_graph)), "bia_run_steps_dev", self._h)
+        return state
+
+    # -- learned initialiser ---------------------------------------------------------------------
+    def mlp_init(self, seed: int = 1):
+        check(load().bia_mlp_init(self._h, int(seed)), "bia_mlp_init", self._h)
+
+    def mlp_set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        check(load().bia_mlp_set_weights(self._h, w, w.size), "bia_mlp_set_weights", self._h)
+
+    def mlp_get_weights(self) -> np.ndarray:
+        n = load().bia_mlp_nparam()
+        w = np.zeros(n, dtype=np.float32)
+        check(load().bia_mlp_get_weights(self._h, w, n), "bia_mlp_get_weights", self._h)
+        return w
+
+    def mlp_forward_(self, rows, y):
+        """rows: float32 CUDA tensor [B, 4]; y: float32 CUDA tensor [B]."""
+        check(load().bia_mlp_forward_dev(self._h, rows.data_ptr(), rows.shape[0], y.data_ptr()),
+              "bia_mlp_forward_dev", self._h)
+        return y
+
+    def mlp_grad_(self, rows, grad):
+        """Full-batch gradient of mean((y - t)^2): grad is a float32 CUDA tensor of nparam + 1."""
+        check(load().bia_mlp_grad_dev(self._h, rows.data_ptr(), rows.shape[0], grad.data_ptr()),
+              "bia_mlp_grad_dev", self._h)
+        return grad
+
+    def mlp_train_(self, rows, steps: int, batch: int = 8192, lr: float = 3e-3, seed: int = 7) -> np.ndarray:
+        curve = np.zeros(steps, dtype=np.float32)
+        check(load().bia_mlp_train_dev(self._h, rows.data_ptr(), rows.shape[0], steps, batch, lr, int(seed),
+                                       curve.ctypes.data_as(C.c_void_p)), "bia_mlp_train_dev", self._h)
+        return curve
+
+    def mlp_build_table(self, nx: int = 1024, nv: int = 256):
+        check(load().bia_mlp_build_table(self._h, nx, nv), "bia_mlp_build_table", self._h)
+
+    def record_samples_(self, state, eps: float, dt: float, nsamples: int, seed: int, out):
+        """out: float32 CUDA tensor [nsamples, 4]."""
+        check(load().bia_record_samples_dev(self._h, state.shape[1], eps, dt, state.data_ptr(), state.shape[1],
+                                            nsamples, int(seed), out.data_ptr()), "bia_record_samples_dev", self._h)
         return out
 
     def dim_rkf45(self, y_aos: np.ndarray, eps: float, x0: float, xe: float, relerr: float = 0.0,

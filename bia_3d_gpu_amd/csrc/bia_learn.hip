@@ -1,12 +1,471 @@
-// bia_learn.hip -- learned-initialiser step (filled in by the MLP work).
+// bia_learn.hip -- the learned initialiser: a 3 -> 64 -> 64 -> 1 GELU network trained on samples
+// the solver records, with hand-written BF16 MFMA forward / backward kernels (FP32
+// accumulation), Adam, the sample recorder, and the tabulation the pair kernel reads.
+//
+// Network input (one hyperbolic pair): x1 = theta_gss / theta_0, v = e^(-1/4), branch flag
+// +1 attractive / -1 repulsive.  Output: t = (theta / theta_0 - x1) * e (see bia_pair.hpp).
+//
+// Parameters (fp32 master copy, 4481 floats): W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3.
+// Matrix products run on v_mfma_f32_32x32x16_bf16 with BF16 operands: one wave computes a
+// 64 x 64 output (2 x 2 blocks of 32 x 32) from operands staged in LDS.
+//
+// Fragment layout of v_mfma_f32_32x32x16_bf16 (wave64), lane l, r = l % 32, h = l / 32:
+//   A (32 x 16):  lane holds A[r][8h .. 8h+7]      B (16 x 32): lane holds B[8h .. 8h+7][r]
+//   D (32 x 32):  register i of lane l holds D[8 (i/4) + 4h + (i%4)][r]
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "bia_pair.hpp"
 #include "bia_runtime.hpp"
 
 namespace bia {
-int step_learn_locked(bia_ctx* ctx, int, double, double, int, const double*, int, double*, unsigned long long*) {
-  if (!ctx->mlp_ready) {
-    snprintf(ctx->last_error, sizeof(ctx->last_error), "learned mode needs trained weights");
-    return BIA_ERR_STATE;
+
+constexpr int kH = 64;                      // hidden width
+constexpr int kNParam = 3 * kH + kH + kH * kH + kH + kH + 1;  // 4481
+constexpr int oW1 = 0, oB1 = 192, oW2 = 256, oB2 = 4352, oW3 = 4416, oB3 = 4480;
+
+typedef __bf16 bf16;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+struct Acc {
+  f32x16 c[2][2];
+  __device__ void zero() {
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int i = 0; i < 16; i++) c[a][b][i] = 0.f;
   }
-  return BIA_ERR_STATE;
+};
+
+// C[64 x 64] += A[64 x K] B[K x 64]; A(m,k) = A[m*am + k*ak], B(k,n) = B[k*bk + n*bn] (LDS, bf16)
+__device__ __forceinline__ void gemm64(const bf16* A, int am, int ak, const bf16* B, int bk, int bn, int K, Acc& acc) {
+  const int l = threadIdx.x & 63, r = l & 31, h = l >> 5;
+  for (int kb = 0; kb < K; kb += 16) {
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) a[mi][j] = A[(mi * 32 + r) * am + (kb + h * 8 + j) * ak];
+#pragma unroll
+    for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) b[ni][j] = B[(kb + h * 8 + j) * bk + (ni * 32 + r) * bn];
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+      for (int ni = 0; ni < 2; ni++)
+        acc.c[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[mi], b[ni], acc.c[mi][ni], 0, 0, 0);
+  }
 }
+__device__ __forceinline__ int acc_row(int mi, int i) { return mi * 32 + 8 * (i / 4) + 4 * ((threadIdx.x & 63) >> 5) + (i % 4); }
+__device__ __forceinline__ int acc_col(int ni) { return ni * 32 + (threadIdx.x & 31); }
+
+// GELU, tanh form: 0.5 z (1 + tanh(c (z + 0.044715 z^3))), c = sqrt(2/pi)
+__host__ __device__ inline float gelu_f(float z) {
+  const float c = 0.7978845608028654f;
+  return 0.5f * z * (1.f + tanhf(c * (z + 0.044715f * z * z * z)));
+}
+__host__ __device__ inline float gelu_df(float z) {
+  const float c = 0.7978845608028654f;
+  const float th = tanhf(c * (z + 0.044715f * z * z * z));
+  return 0.5f * (1.f + th) + 0.5f * z * (1.f - th * th) * c * (1.f + 3.f * 0.044715f * z * z);
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+
+// sample rows: [x1, v, b (0 attractive / 1 repulsive), t]
+struct Batch {
+  const float* samples;
+  int nsamples;
+  int batch;                // samples per step
+  unsigned long long seed;  // 0: sequential rows (inference / full-batch gradient)
+  long long step;
+};
+
+__device__ __forceinline__ int batch_row(const Batch& B, int i) {
+  if (B.seed == 0) return i;
+  return (int)(mix64(B.seed ^ (0x9e3779b97f4a7c15ULL * (unsigned long long)(B.step * B.batch + i))) %
+               (unsigned long long)B.nsamples);
+}
+
+// One wave per workgroup, grid-stride over 64-sample tiles of the batch.
+//   FWD_ONLY: y[i] = network(sample i)
+//   otherwise: per-workgroup gradient of loss = mean((y - t)^2) into part[wg][kNParam + 1] (last = loss)
+template <bool FWD_ONLY>
+__global__ __launch_bounds__(64) void k_mlp(const float* __restrict__ w, Batch B, float* __restrict__ y_out,
+                                            float* __restrict__ part) {
+  __shared__ bf16 sW1[kH * 16], sW2[kH * kH], sX[64 * 16], sH1[64 * kH], sD[64 * kH];
+  __shared__ float sZ1[64 * kH], sZ2[64 * kH], sH2[64 * kH];
+  __shared__ float sXf[64 * 4], sDy[64], sB1[kH], sB2[kH], sW3[kH];
+  __shared__ float gW1[kH * 3], gB1[kH], gB2[kH], gW3[kH];
+  const int l = threadIdx.x;
+  for (int q = l; q < kH * 16; q += 64) {
+    const int n = q / 16, k = q % 16;
+    sW1[q] = (bf16)(k < 3 ? w[oW1 + n * 3 + k] : 0.f);
+  }
+  for (int q = l; q < kH * kH; q += 64) sW2[q] = (bf16)w[oW2 + q];
+  sB1[l] = w[oB1 + l];
+  sB2[l] = w[oB2 + l];
+  sW3[l] = w[oW3 + l];
+  const float b3 = w[oB3];
+  if (!FWD_ONLY) {
+    for (int k = 0; k < 3; k++) gW1[l * 3 + k] = 0.f;
+    gB1[l] = gB2[l] = gW3[l] = 0.f;
+  }
+  float gb3 = 0.f, loss = 0.f;
+  Acc accW2;
+  accW2.zero();
+  const float inv_b = 1.f / (float)B.batch;
+  __syncthreads();
+  const int ntile = (B.batch + 63) / 64;
+  for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const int i = tile * 64 + l;
+    const bool ok = i < B.batch;
+    float x1 = 0, v = 0, br = 0, tt = 0;
+    if (ok) {
+      const float* s = B.samples + (size_t)batch_row(B, i) * 4;
+      x1 = s[0];
+      v = s[1];
+      br = 1.f - 2.f * s[2];
+      tt = s[3];
+    }
+    sXf[l * 4 + 0] = x1;
+    sXf[l * 4 + 1] = v;
+    sXf[l * 4 + 2] = br;
+    for (int k = 0; k < 16; k++) sX[l * 16 + k] = (bf16)(k == 0 ? x1 : k == 1 ? v : k == 2 ? br : 0.f);
+    __syncthreads();
+    // layer 1: Z1 = X W1^T + b1
+    Acc acc;
+    acc.zero();
+    gemm64(sX, 16, 1, sW1, 1, 16, 16, acc);
+    for (int mi = 0; mi < 2; mi++)
+      for (int ni = 0; ni < 2; ni++)
+        for (int q = 0; q < 16; q++) {
+          const int m = acc_row(mi, q), n = acc_col(ni);
+          const float z = acc.c[mi][ni][q] + sB1[n];
+          sZ1[m * kH + n] = z;
+          sH1[m * kH + n] = (bf16)gelu_f(z);
+        }
+    __syncthreads();
+    // layer 2: Z2 = H1 W2^T + b2
+    acc.zero();
+    gemm64(sH1, kH, 1, sW2, 1, kH, kH, acc);
+    for (int mi = 0; mi < 2; mi++)
+      for (int ni = 0; ni < 2; ni++)
+        for (int q = 0; q < 16; q++) {
+          const int m = acc_row(mi, q), n = acc_col(ni);
+          const float z = acc.c[mi][ni][q] + sB2[n];
+          sZ2[m * kH + n] = z;
+          sH2[m * kH + n] = gelu_f(z);
+        }
+    __syncthreads();
+    // layer 3 (lane = sample)
+    float yv = b3;
+    for (int n = 0; n < kH; n++) yv += sW3[n] * sH2[l * kH + n];
+    if (FWD_ONLY) {
+      if (ok) y_out[i] = yv;
+      __syncthreads();
+      continue;
+    }
+    const float dy = ok ? 2.f * (yv - tt) * inv_b : 0.f;
+    if (ok) loss += (yv - tt) * (yv - tt) * inv_b;
+    sDy[l] = dy;
+    gb3 += dy;
+    __syncthreads();
+    // lane = hidden unit n from here on (conflict-free rows)
+    {
+      const int n = l;
+      float g3 = 0.f, gb = 0.f;
+      for (int m = 0; m < 64; m++) {
+        const float dym = sDy[m];
+        g3 += dym * sH2[m * kH + n];
+        const float dz = dym * sW3[n] * gelu_df(sZ2[m * kH + n]);
+        sZ2[m * kH + n] = dz;
+        sD[m * kH + n] = (bf16)dz;
+        gb += dz;
+      }
+      gW3[n] += g3;
+      gB2[n] += gb;
+    }
+    __syncthreads();
+    // dW2 += dZ2^T H1 (reduction over the tile's samples), kept in MFMA accumulators
+    gemm64(sD, 1, kH, sH1, kH, 1, 64, accW2);
+    // dH1 = dZ2 W2
+    acc.zero();
+    gemm64(sD, kH, 1, sW2, kH, 1, kH, acc);
+    for (int mi = 0; mi < 2; mi++)
+      for (int ni = 0; ni < 2; ni++)
+        for (int q = 0; q < 16; q++) {
+          const int m = acc_row(mi, q), n = acc_col(ni);
+          sZ1[m * kH + n] = acc.c[mi][ni][q] * gelu_df(sZ1[m * kH + n]);
+        }
+    __syncthreads();
+    {
+      const int n = l;
+      float gb = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f;
+      for (int m = 0; m < 64; m++) {
+        const float dz = sZ1[m * kH + n];
+        gb += dz;
+        g0 += dz * sXf[m * 4 + 0];
+        g1 += dz * sXf[m * 4 + 1];
+        g2 += dz * sXf[m * 4 + 2];
+      }
+      gB1[n] += gb;
+      gW1[n * 3 + 0] += g0;
+      gW1[n * 3 + 1] += g1;
+      gW1[n * 3 + 2] += g2;
+    }
+    __syncthreads();
+  }
+  if (FWD_ONLY) return;
+  // write this workgroup's gradient
+  float* P = part + (size_t)blockIdx.x * (kNParam + 1);
+  for (int mi = 0; mi < 2; mi++)
+    for (int ni = 0; ni < 2; ni++)
+      for (int q = 0; q < 16; q++) P[oW2 + acc_row(mi, q) * kH + acc_col(ni)] = accW2.c[mi][ni][q];
+  for (int k = 0; k < 3; k++) P[oW1 + l * 3 + k] = gW1[l * 3 + k];
+  P[oB1 + l] = gB1[l];
+  P[oB2 + l] = gB2[l];
+  P[oW3 + l] = gW3[l];
+  for (int off = 32; off > 0; off >>= 1) {
+    gb3 += __shfl_xor(gb3, off, 64);
+    loss += __shfl_xor(loss, off, 64);
+  }
+  if (l == 0) {
+    P[oB3] = gb3;
+    P[kNParam] = loss;
+  }
+}
+
+// sum the workgroup partials in a fixed order; optionally apply Adam
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ part, int nwg, float* __restrict__ grad,
+                                                    float* __restrict__ w, float* __restrict__ m1,
+                                                    float* __restrict__ m2, float lr, float beta1, float beta2,
+                                                    float bc1, float bc2, int apply) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kNParam) return;
+  float g = 0.f;
+  for (int q = 0; q < nwg; q++) g += part[(size_t)q * (kNParam + 1) + p];
+  grad[p] = g;  // grad[kNParam] = loss
+  if (!apply || p == kNParam) return;
+  const float a = beta1 * m1[p] + (1.f - beta1) * g;
+  const float b = beta2 * m2[p] + (1.f - beta2) * g * g;
+  m1[p] = a;
+  m2[p] = b;
+  w[p] -= lr * (a / bc1) / (sqrtf(b / bc2) + 1e-8f);
+}
+
+// grid of the table: rows [x1, v, b, 0]
+__global__ void k_table_grid(int nx, int nv, float* __restrict__ rows) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 2 * nx * nv) return;
+  const int ix = q % nx, iv = (q / nx) % nv, b = q / (nx * nv);
+  float* r = rows + (size_t)q * 4;
+  r[0] = -1.f + 2.f * ix / (nx - 1);
+  r[1] = (float)iv / (nv - 1);
+  r[2] = (float)b;
+  r[3] = 0.f;
+}
+
+// Recorder: random pairs (i, j) of a system; hyperbolic pairs are solved to convergence from the
+// reference guess with the safeguarded Newton and written as [x1, v, b, t]; other pairs get
+// x1 = NaN.
+__global__ __launch_bounds__(64) void k_record(int n, int ld, double eps, double dt, const double* __restrict__ st,
+                                               int nsamples, unsigned long long seed, float* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsamples) return;
+  const unsigned long long h = mix64(seed ^ (0xd1b54a32d192ed03ULL * (unsigned long long)(s + 1)));
+  const int i = (int)(h % (unsigned long long)n);
+  int j = (int)((h >> 32) % (unsigned long long)(n - 1));
+  if (j >= i) j++;
+  double rel[6];
+  for (int c = 0; c < 6; c++) rel[c] = st[(size_t)c * ld + i] - st[(size_t)c * ld + j];
+  double exi[3], eeta[3], yz[4];
+  frame(rel, rel + 3, exi, eeta, yz);
+  float* o = out + (size_t)s * 4;
+  const double W = sqrt(yz[0] * yz[0] + yz[1] * yz[1]) * (yz[2] * yz[2] + yz[3] * yz[3]) + 2 * eps;
+  o[0] = __builtin_nanf("");
+  o[1] = o[2] = o[3] = 0.f;
+  if (!(W > 0)) return;
+  HypState S;
+  unsigned fl = 0;
+  hyp_setup(eps, dt, yz, S, &fl);
+  int ev = 0;
+  bool okc = false;
+  const double th = hyp_solve_safe(S, S.th_gss, 40, &ev, &okc);
+  if (!okc) return;
+  double x1, v;
+  int b;
+  hyp_features(S, &x1, &v, &b);
+  const double t = (th / S.P.th0 - x1) / fabs(S.P.cs0);
+  if (!isfinite(t) || !isfinite(x1)) return;
+  o[0] = (float)x1;
+  o[1] = (float)v;
+  o[2] = (float)b;
+  o[3] = (float)t;
+}
+
+static int mlp_bufs(bia_ctx* ctx) {
+  // [w | m1 | m2 | grad(+loss)] floats
+  return ctx->mlp_w.ensure((size_t)4 * (kNParam + 4) * sizeof(float));
+}
+static float* W_(bia_ctx* c) { return c->mlp_w.as<float>(); }
+static float* M1_(bia_ctx* c) { return c->mlp_w.as<float>() + (kNParam + 4); }
+static float* M2_(bia_ctx* c) { return c->mlp_w.as<float>() + 2 * (kNParam + 4); }
+static float* G_(bia_ctx* c) { return c->mlp_w.as<float>() + 3 * (kNParam + 4); }
+
+static int mlp_nwg(int batch) {
+  const int ntile = (batch + 63) / 64;
+  return ntile < 1024 ? ntile : 1024;
+}
+
+static int mlp_grad_locked(bia_ctx* ctx, const Batch& B, int apply, float lr, long long step) {
+  const int nwg = mlp_nwg(B.batch);
+  BIA_TRY(ctx, ctx->mlp_ws.ensure((size_t)nwg * (kNParam + 1) * sizeof(float)));
+  hipLaunchKernelGGL((k_mlp<false>), dim3(nwg), dim3(64), 0, ctx->stream, W_(ctx), B, nullptr, ctx->mlp_ws.as<float>());
+  BIA_HIP_TRY(ctx, hipGetLastError());
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float bc1 = 1.f - powf(b1, (float)(step + 1)), bc2 = 1.f - powf(b2, (float)(step + 1));
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, ctx->stream,
+                     ctx->mlp_ws.as<float>(), nwg, G_(ctx), W_(ctx), M1_(ctx), M2_(ctx), lr, b1, b2, bc1, bc2, apply);
+  BIA_HIP_TRY(ctx, hipGetLastError());
+  return BIA_OK;
+}
+
+static int mlp_forward_locked(bia_ctx* ctx, const float* d_rows, int nrows, float* d_y) {
+  Batch B{d_rows, nrows, nrows, 0ULL, 0};
+  const int nwg = mlp_nwg(nrows);
+  hipLaunchKernelGGL((k_mlp<true>), dim3(nwg), dim3(64), 0, ctx->stream, W_(ctx), B, d_y, nullptr);
+  BIA_HIP_TRY(ctx, hipGetLastError());
+  return BIA_OK;
+}
+
 }  // namespace bia
+
+using namespace bia;
+
+#define BIA_LOCK(ctx)                              \
+  if (!(ctx)) return BIA_ERR_ARG;                  \
+  std::lock_guard<std::mutex> lk_((ctx)->mu);      \
+  (ctx)->last_error[0] = 0;                        \
+  BIA_HIP_TRY((ctx), hipSetDevice((ctx)->device))
+
+extern "C" {
+
+int bia_mlp_nparam(void) { return kNParam; }
+
+int bia_mlp_init(bia_ctx* ctx, unsigned long long seed) {
+  BIA_LOCK(ctx);
+  BIA_TRY(ctx, mlp_bufs(ctx));
+  std::vector<float> w(kNParam, 0.f);
+  unsigned long long s = seed * 0x9e3779b97f4a7c15ULL + 1;
+  auto unif = [&]() {  // uniform in (-1, 1)
+    s = s * 6364136223846793005ULL + 1442695040888963407ULL;
+    return (float)((double)(s >> 11) / 9007199254740992.0 * 2.0 - 1.0);
+  };
+  for (int q = 0; q < 192; q++) w[oW1 + q] = unif() * 1.0f;               // fan-in 3
+  for (int q = 0; q < 4096; q++) w[oW2 + q] = unif() * 0.2165f;           // sqrt(3/64)
+  for (int q = 0; q < 64; q++) w[oW3 + q] = unif() * 0.2165f;
+  std::vector<float> all(4 * (kNParam + 4), 0.f);
+  memcpy(all.data(), w.data(), kNParam * sizeof(float));
+  BIA_HIP_TRY(ctx, hipMemcpy(ctx->mlp_w.p, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice));
+  ctx->mlp_init = 1;
+  ctx->mlp_ready = 0;
+  ctx->mlp_step = 0;
+  return BIA_OK;
+}
+
+int bia_mlp_set_weights(bia_ctx* ctx, const float* w, int n) {
+  BIA_LOCK(ctx);
+  if (!w || n != kNParam) return BIA_ERR_ARG;
+  BIA_TRY(ctx, mlp_bufs(ctx));
+  std::vector<float> all(4 * (kNParam + 4), 0.f);
+  memcpy(all.data(), w, kNParam * sizeof(float));
+  BIA_HIP_TRY(ctx, hipMemcpy(ctx->mlp_w.p, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice));
+  ctx->mlp_init = 1;
+  ctx->mlp_ready = 0;
+  ctx->mlp_step = 0;
+  return BIA_OK;
+}
+
+int bia_mlp_get_weights(bia_ctx* ctx, float* w, int n) {
+  BIA_LOCK(ctx);
+  if (!w || n != kNParam || !ctx->mlp_init) return BIA_ERR_ARG;
+  BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  BIA_HIP_TRY(ctx, hipMemcpy(w, ctx->mlp_w.p, kNParam * sizeof(float), hipMemcpyDeviceToHost));
+  return BIA_OK;
+}
+
+int bia_mlp_forward_dev(bia_ctx* ctx, const float* d_rows, int nrows, float* d_y) {
+  BIA_LOCK(ctx);
+  if (!ctx->mlp_init || !d_rows || !d_y || nrows < 1) return BIA_ERR_ARG;
+  return mlp_forward_locked(ctx, d_rows, nrows, d_y);
+}
+
+int bia_mlp_grad_dev(bia_ctx* ctx, const float* d_rows, int nrows, float* d_grad) {
+  BIA_LOCK(ctx);
+  if (!ctx->mlp_init || !d_rows || !d_grad || nrows < 1) return BIA_ERR_ARG;
+  Batch B{d_rows, nrows, nrows, 0ULL, 0};
+  BIA_TRY(ctx, mlp_grad_locked(ctx, B, 0, 0.f, 0));
+  BIA_HIP_TRY(ctx, hipMemcpyAsync(d_grad, G_(ctx), (kNParam + 1) * sizeof(float), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+  return BIA_OK;
+}
+
+int bia_mlp_train_dev(bia_ctx* ctx, const float* d_rows, int nrows, int steps, int batch, float lr,
+                      unsigned long long seed, float* loss_curve) {
+  BIA_LOCK(ctx);
+  if (!ctx->mlp_init || !d_rows || nrows < 1 || steps < 1 || batch < 64) return BIA_ERR_ARG;
+  for (int k = 0; k < steps; k++) {
+    // cosine decay to lr/20
+    const float lk = lr * (0.05f + 0.95f * 0.5f * (1.f + cosf(3.14159265f * k / steps)));
+    Batch B{d_rows, nrows, batch, seed | 1ULL, ctx->mlp_step};
+    BIA_TRY(ctx, mlp_grad_locked(ctx, B, 1, lk, ctx->mlp_step));
+    ctx->mlp_step++;
+    if (loss_curve)
+      BIA_HIP_TRY(ctx, hipMemcpyAsync(loss_curve + k, G_(ctx) + kNParam, sizeof(float), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+  }
+  BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->mlp_ready = 0;
+  return BIA_OK;
+}
+
+int bia_mlp_build_table(bia_ctx* ctx, int nx, int nv) {
+  BIA_LOCK(ctx);
+  if (!ctx->mlp_init || nx < 2 || nv < 2 || (size_t)nx * nv > (1u << 24)) return BIA_ERR_ARG;
+  const int rows = 2 * nx * nv;
+  BIA_TRY(ctx, ctx->mlp_table.ensure((size_t)rows * sizeof(float) * 5));
+  float* tab = ctx->mlp_table.as<float>();
+  float* grid = tab + rows;  // rows x 4 scratch after the table
+  hipLaunchKernelGGL(k_table_grid, dim3((rows + 255) / 256), dim3(256), 0, ctx->stream, nx, nv, grid);
+  BIA_HIP_TRY(ctx, hipGetLastError());
+  BIA_TRY(ctx, mlp_forward_locked(ctx, grid, rows, tab));
+  BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->table_nx = nx;
+  ctx->table_nv = nv;
+  ctx->mlp_ready = 1;
+  return BIA_OK;
+}
+
+int bia_record_samples_dev(bia_ctx* ctx, int n, double eps, double dt, const double* d_state, int ld, int nsamples,
+                           unsigned long long seed, float* d_out) {
+  BIA_LOCK(ctx);
+  if (n < 2 || ld < n || !d_state || !d_out || nsamples < 1) return BIA_ERR_ARG;
+  hipLaunchKernelGGL(k_record, dim3((nsamples + 63) / 64), dim3(64), 0, ctx->stream, n, ld, eps, dt, d_state, nsamples,
+                     seed, d_out);
+  BIA_HIP_TRY(ctx, hipGetLastError());
+  return BIA_OK;
+}
+
+}  // extern "C"

@@ -54,7 +54,17 @@ enum Flag : unsigned {
   kFallback = 128u     // learned path fell back to the reference guess
 };
 
-enum Mode : int { kPlain = 0, kLearn = 1 };
+// kPlain: the reference's guess selection and iteration control.
+// kLearn: learned guess (network tabulated over its input features) + safeguarded Newton.
+// kGuess: the reference's closed-form guess + the same safeguarded Newton (no network).
+enum Mode : int { kPlain = 0, kLearn = 1, kGuess = 2 };
+
+// Learned-initialiser table: the network's output t(x1, v) for the two hyperbolic branches on a
+// regular grid, x1 in [-1, 1] (nx points), v in [0, 1] (nv points), bilinear interpolation.
+struct LearnTable {
+  const float* t;  // [2][nv][nx]
+  int nx, nv;
+};
 
 // Residual history used by the reference's exact-repeat exit.  On the device it lives in
 // LDS (one column per lane); on the host it is a plain array.
@@ -228,6 +238,71 @@ BIA_HDF double hyp_solve_plain(const HypState& S, int itr, Hist hist, int* iters
   if (!updated) *fl |= kStockDefault;
   *iters = ir + 2;  // two set-up Newton evaluations + the loop
   return th_stock;
+}
+
+// ------------------------------------------------------------------------------------
+// learned initial guess and safeguarded Newton (no reference counterpart)
+//
+// Features of a hyperbolic pair: x1 = theta_gss / theta_0 (the reference's closed-form guess
+// (:172), a function of the dimensionless mean anomaly TE and e), v = e^(-1/4) = |cs0|^(1/4),
+// branch b = 0 attractive (cs0 < 0) / 1 repulsive.  Target t = (theta/theta_0 - x1) * e, the
+// guess's scaled residual, O(1) and smooth; theta_L = theta_0 (x1 + t / e).
+// ------------------------------------------------------------------------------------
+BIA_HDF void hyp_features(const HypState& S, double* x1, double* v, int* b) {
+  *x1 = S.th_gss / S.P.th0;
+  *v = sqrt(sqrt(fabs(S.P.cs0)));
+  *b = (S.P.cs0 < 0) ? 0 : 1;
+}
+
+BIA_HDF double table_lookup(const LearnTable& T, int b, double x1, double v) {
+  double fx = (x1 + 1) * 0.5 * (T.nx - 1);
+  double fv = v * (T.nv - 1);
+  fx = fmin(fmax(fx, 0.0), T.nx - 1.000001);
+  fv = fmin(fmax(fv, 0.0), T.nv - 1.000001);
+  const int ix = (int)fx, iv = (int)fv;
+  const double ax = fx - ix, av = fv - iv;
+  const float* p = T.t + ((size_t)b * T.nv + iv) * T.nx + ix;
+  const double t00 = p[0], t01 = p[1], t10 = p[T.nx], t11 = p[T.nx + 1];
+  return (1 - av) * ((1 - ax) * t00 + ax * t01) + av * ((1 - ax) * t10 + ax * t11);
+}
+
+// Newton on the reference's residual, kept inside a bracket of (-theta_0, theta_0): the residual
+// is monotone there, so each evaluation moves one end of the bracket, and a step leaving the
+// bracket (or a non-finite residual near a pole) is replaced by bisection.  Stops after a Newton
+// step |d theta| <= 1e-8 theta_0: quadratic convergence leaves ~1e-16 theta_0 after applying it.
+BIA_HDF double hyp_solve_safe(const HypState& S, double th, int kmax, int* evals, bool* ok) {
+  double lo = -S.P.th0, hi = S.P.th0;
+  if (!(th > lo && th < hi)) th = 0.5 * (lo + hi) + 0.5 * (S.th_gss - 0.5 * (lo + hi));
+  *ok = false;
+  int k = 0;
+  for (; k < kmax; k++) {
+    double err, d;
+    double th_new = hyp_newton(th, S.TE, S.P, &err, &d);
+    if (err == 0) {
+      *ok = true;
+      *evals = k + 1;
+      return th;
+    }
+    const bool newton_ok = isfinite(err) && isfinite(d) && d != 0 && isfinite(th_new);
+    // converged: a Newton step this small leaves ~1e-16 theta_0 once applied (tested before the
+    // bracket, whose end may sit exactly at th when the step rounds to zero)
+    if (newton_ok && fabs(th_new - th) <= 1e-8 * S.P.th0) {
+      *ok = true;
+      *evals = k + 1;
+      return th_new;
+    }
+    if (!newton_ok) {
+      if (th > 0) hi = th;  // the only poles sit at the ends of the interval
+      else lo = th;
+    } else if ((err > 0) == (d > 0)) {
+      hi = th;  // residual increasing through zero above th
+    } else {
+      lo = th;
+    }
+    th = (newton_ok && th_new > lo && th_new < hi) ? th_new : 0.5 * (lo + hi);
+  }
+  *evals = k;
+  return th;
 }
 
 // ------------------------------------------------------------------------------------
@@ -508,9 +583,9 @@ BIA_HD void rkf_pair(double eps, double dt, const double yz[4], double yn[4], in
 // one pair: frame, classifier (:503-506), 2D solve, half-split epilogue (:507-515)
 // rel = (r_i - r_j, g_i - g_j); d = the six half-deltas credited to i (j gets -d)
 // ------------------------------------------------------------------------------------
-template <int METHOD>  // 0 = RKF45, 1 = ALG (reference METHOD codes)
-BIA_HDF void pair_plain(double eps, double dt, int itr, const double rel[6], double d[6], int* branch,
-                        int* iters, unsigned* fl, Hist hist) {
+template <int METHOD, int MODE>  // METHOD: 0 = RKF45, 1 = ALG (reference METHOD codes)
+BIA_HDF void pair_eval(double eps, double dt, int itr, const double rel[6], double d[6], int* branch, int* iters,
+                       unsigned* fl, Hist hist, const LearnTable& T) {
   double exi[3], eeta[3], yz[4], yn[4];
   frame(rel, rel + 3, exi, eeta, yz);
   int it = 0;
@@ -523,7 +598,33 @@ BIA_HDF void pair_plain(double eps, double dt, int itr, const double rel[6], dou
       *branch = kHyperbola;
       HypState S;
       hyp_setup(eps, dt, yz, S, fl);
-      double th = hyp_solve_plain(S, itr, hist, &it, fl);
+      double th;
+      if (MODE == kPlain) {
+        th = hyp_solve_plain(S, itr, hist, &it, fl);
+      } else {
+        double th0 = S.th_gss;
+        if (MODE == kLearn) {
+          double x1, v;
+          int b;
+          hyp_features(S, &x1, &v, &b);
+          const double tp = table_lookup(T, b, x1, v);
+          const double thl = S.P.th0 * (x1 + fabs(S.P.cs0) * tp);
+          // the reference guess judges the learned one: a non-finite or out-of-range proposal
+          // is replaced by it
+          if (isfinite(thl) && fabs(thl) < S.P.th0) {
+            th0 = thl;
+            *fl |= kLearned;
+          }
+        }
+        bool ok;
+        th = hyp_solve_safe(S, th0, 24, &it, &ok);
+        if (!ok) {  // fallback: the reference's own guess and iteration control
+          int it2 = 0;
+          th = hyp_solve_plain(S, itr, hist, &it2, fl);
+          it += it2;
+          *fl |= kFallback;
+        }
+      }
       hyp_finish(S, th, yz, yn);
     } else if (W == 0) {
       *branch = kParabola;
@@ -543,6 +644,13 @@ BIA_HDF void pair_plain(double eps, double dt, int itr, const double rel[6], dou
   bool fin = true;
   for (int c = 0; c < 6; c++) fin = fin && isfinite(d[c]);
   if (!fin) *fl |= kNonFinite;
+}
+
+template <int METHOD>
+BIA_HDF void pair_plain(double eps, double dt, int itr, const double rel[6], double d[6], int* branch, int* iters,
+                        unsigned* fl, Hist hist) {
+  const LearnTable none{nullptr, 0, 0};
+  pair_eval<METHOD, kPlain>(eps, dt, itr, rel, d, branch, iters, fl, hist, none);
 }
 
 }  // namespace bia

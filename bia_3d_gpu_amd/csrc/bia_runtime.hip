@@ -95,8 +95,14 @@ int bia_destroy(bia_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   bia::DevBuf* bufs[] = {&c->part, &c->tiles, &c->state, &c->delta, &c->stats, &c->leg_in, &c->leg_out,
                          &c->inv_state, &c->inv_part, &c->inv_out, &c->dim_work, &c->dim_ctl,
-                         &c->mlp_w, &c->mlp_ws};
+                         &c->mlp_w, &c->mlp_ws, &c->mlp_table};
   for (auto* b : bufs) b->release();
+  c->plan_tiles.release();
+  c->plan_sys.release();
+  c->plan_chunks.release();
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
   if (c->ev_end) (void)hipEventDestroy(c->ev_end);
   delete c;

@@ -5,11 +5,31 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <map>
 #include <mutex>
+#include <vector>
 
 #include "../../include/bia_hip.h"
 
 namespace bia {
+
+// one system of a (possibly batched) step: particles [off, off + n) of the SoA state, nt = ceil(n/64)
+// tiles, partial sums at part + poff laid out [nt][6][64 nt]
+struct SysDesc {
+  int off, n, nt, pad;
+  long long poff;
+  double eps;
+};
+
+// cached launch plan for a list of system sizes
+struct Plan {
+  std::vector<int> sizes;
+  int nsys = 0, ntiles = 0, nchunks = 0, ntotal = 0;
+  size_t part_doubles = 0;
+  std::vector<SysDesc> hsys;  // host copy (eps updated per call)
+  int grid = 0;               // pair-kernel workgroups (0: one per tile); set by the autotuner
+  double tuned_ms = 0;
+};
 
 // grow-only device buffer; reused across calls so steady-state steps allocate nothing
 struct DevBuf {
@@ -46,19 +66,31 @@ struct DevBuf {
 struct bia_ctx {
   int device = 0;
   hipStream_t stream = nullptr;  // default: the null stream (torch's default stream too)
+  hipStream_t cap_stream = nullptr;  // private blocking stream for graph capture when stream is null
   std::mutex mu;                 // one call at a time per context
   // step workspace
   bia::DevBuf part, tiles, state, delta, stats;
   int tiles_nt = -1;
+  // batched plans: device tile list (int4 I, J, sys, 0), system descriptors, reduce chunks
+  bia::Plan plan;
+  bia::DevBuf plan_tiles, plan_sys, plan_chunks;
+  std::map<std::vector<int>, int> tuned_grid;  // autotuner cache: sizes -> grid
+  // multi-step driver graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<long long> graph_key;
   // legacy per-row buffers
   bia::DevBuf leg_in, leg_out;
   // invariance workspace
   bia::DevBuf inv_state, inv_part, inv_out;
   // DIM workspace
   bia::DevBuf dim_work, dim_ctl;
-  // learned initialiser weights (device copy) and batch workspace
-  bia::DevBuf mlp_w, mlp_ws;
-  int mlp_ready = 0;
+  // learned initialiser: fp32 master weights + Adam moments, training workspace, tabulated output
+  bia::DevBuf mlp_w, mlp_ws, mlp_table;
+  int mlp_init = 0;   // weights present
+  int mlp_ready = 0;  // table built from the current weights
+  int table_nx = 0, table_nv = 0;
+  long long mlp_step = 0;
   // optional timing of the dominant (pair) kernel with HIP events on the context's stream
   int profiling = 0;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -97,7 +129,4 @@ int prof_begin(bia_ctx* ctx);
 int prof_end(bia_ctx* ctx);
 // upper-triangle list of 64-wide tile pairs (I <= J) for nt tiles, cached on the context
 int ensure_tiles(bia_ctx* ctx, int nt);
-// learned-initialiser step (bia_learn.hip); caller holds ctx->mu
-int step_learn_locked(bia_ctx* ctx, int n, double eps, double dt, int itr, const double* d_state, int ld,
-                      double* d_delta, unsigned long long* d_stats);
 }  // namespace bia

@@ -1,16 +1,29 @@
-// bia_step.hip -- whole-step pair kernels (ALG / RKF45) and the reference-compatible
+// bia_step.hip -- whole-step pair kernels (ALG / RKF45) for one system or a batch of systems,
+// the multi-step driver (hipGraph), the launch autotuner, and the reference-compatible
 // per-particle entry point find_dxyuv_kernel_.
 //
-// Whole step (bia_step_dev): the i<j triangle is cut into 64x64 tiles (I <= J).  One wave of
-// 64 lanes owns one tile: lane l holds row particle i = 64 I + l in registers and walks the
-// 64 column particles j of tile J in order (j is wave-uniform, so its six doubles are
-// scalar loads).  Each lane accumulates its row sum in registers; the column sum of each j
-// is a 64-lane butterfly reduction.  Results go to a partial buffer part[K][c][i] in which
-// every (tile K, particle i) slot has exactly one writer:
+// Tiling.  The i<j triangle of each system is cut into 64x64 tiles (I <= J).  One wave owns one
+// tile: lane l holds row particle i = 64 I + l in registers and walks the 64 column particles j
+// of tile J in order (j is wave-uniform, so its state comes in through scalar loads).  Each lane
+// keeps its row sum in registers; the column sum of each j is a 64-lane butterfly reduction.
+// Results go to a per-system partial buffer part[K][c][i] in which every (tile K, particle i)
+// slot has exactly one writer:
 //   tile (I,J), I<J : rows of I  -> part[J][c][i],  columns of J -> part[I][c][j]
 //   tile (I,I)      : both sides -> part[I][c][i]
-// so no atomics and no clearing are needed, and a second pass sums part over K in a fixed
-// order (deterministic results).  The drift v*dt is added in that pass.
+// so there are no atomics and nothing to clear, and a second kernel sums part over K in a
+// fixed order (deterministic).  The drift v dt is added there.
+//
+// Batching.  The tiles of all systems form one list (I, J, system); the pair kernel is a
+// persistent grid-stride launch over it, so a batch of systems of uneven N is one launch.
+//
+// Couplings.  SCALAR: the reference's r'' = eps r/|r|^3 with eps per system and the equal-mass
+// half split (find_dXYUV_kernel.cu:510-515).  GRAVITY: eps_ij = -G (m_i + m_j).  COULOMB:
+// eps_ij = k q_i q_j (1/m_i + 1/m_j).  With masses the relative change is split by the centre of
+// mass: m_j/(m_i+m_j) to i and m_i/(m_i+m_j) to j.
+#include <string.h>
+
+#include <algorithm>
+
 #include "bia_pair.hpp"
 #include "bia_runtime.hpp"
 
@@ -38,7 +51,6 @@ struct PairStats {
     for (int k = 0; k < 8; k++) fl[k] = 0;
   }
   __device__ void add(int br, int it, unsigned f) {
-    // branch-indexed without dynamic register indexing
     for (int k = 0; k < 4; k++) {
       cnt[k] += (br == k);
       its[k] += (br == k) ? (unsigned)it : 0u;
@@ -58,76 +70,117 @@ struct PairStats {
   }
 };
 
-template <int METHOD, bool STATS>
-__global__ __launch_bounds__(kWave) void k_step_tiles(int n, int lds, int ldp, double eps, double dt, int itr,
-                                                      const double* __restrict__ st,
-                                                      const int2* __restrict__ tiles,
-                                                      double* __restrict__ part,
-                                                      unsigned long long* __restrict__ stats) {
+enum Coupling : int { kScalar = 0, kGravity = 1, kCoulomb = 2 };
+
+template <int METHOD, int MODE, int COUP, bool STATS>
+__global__ __launch_bounds__(kWave) void k_tiles(int ntiles, int lds, const int4* __restrict__ tiles,
+                                                 const SysDesc* __restrict__ sys, const double* __restrict__ st,
+                                                 const double* __restrict__ mass, const double* __restrict__ charge,
+                                                 double coup, double dt, int itr, double* __restrict__ part,
+                                                 unsigned long long* __restrict__ stats, LearnTable T) {
   __shared__ double hist_lds[kMaxItr * kWave];
   const int lane = threadIdx.x;
-  const int2 t = tiles[blockIdx.x];
-  const int I = t.x, J = t.y;
-  const int i = I * kWave + lane;
-  const bool row_ok = i < n;
-  double ri[6];
-#pragma unroll
-  for (int c = 0; c < 6; c++) ri[c] = row_ok ? st[(size_t)c * lds + i] : 0.0;
-  double acc[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
+  Hist hist{hist_lds + lane, kWave};
   PairStats ps;
   if (STATS) ps.clear();
-  Hist hist{hist_lds + lane, kWave};
-  const int j0 = J * kWave;
-  const int jn = min(kWave, n - j0);
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int4 td = tiles[t];
+    const SysDesc S = sys[td.z];
+    const int I = td.x, J = td.y;
+    const int il = I * kWave + lane;  // local row index
+    const bool row_ok = il < S.n;
+    const int gi = S.off + il;
+    double ri[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) ri[c] = row_ok ? st[(size_t)c * lds + gi] : 0.0;
+    double mi = 1, qi = 0;
+    if (COUP != kScalar) {
+      mi = (row_ok && mass) ? mass[gi] : 1.0;
+      if (COUP == kCoulomb) qi = row_ok ? charge[gi] : 0.0;
+    }
+    double acc[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
+    const int j0 = J * kWave;
+    const int jn = min(kWave, S.n - j0);
 #pragma unroll 1
-  for (int jj = 0; jj < jn; ++jj) {
-    const int j = j0 + jj;
-    double rj[6];
+    for (int jj = 0; jj < jn; ++jj) {
+      const int gj = S.off + j0 + jj;
+      double rj[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) rj[c] = st[(size_t)c * lds + j];
-    double d[6] = {0, 0, 0, 0, 0, 0};
-    if (row_ok && (I != J || lane < jj)) {
-      double rel[6];
+      for (int c = 0; c < 6; c++) rj[c] = st[(size_t)c * lds + gj];
+      double d[6] = {0, 0, 0, 0, 0, 0};
+      double wi = 1, wj = 1;  // multiples of the half deltas credited to i and debited from j
+      if (row_ok && (I != J || lane < jj)) {
+        double eps = S.eps;
+        if (COUP != kScalar) {
+          const double mj = mass ? mass[gj] : 1.0;
+          if (COUP == kGravity) eps = -coup * (mi + mj);
+          else eps = coup * qi * charge[gj] * (1 / mi + 1 / mj);
+          wi = 2 * mj / (mi + mj);
+          wj = 2 * mi / (mi + mj);
+        }
+        double rel[6];
 #pragma unroll
-      for (int c = 0; c < 6; c++) rel[c] = ri[c] - rj[c];
-      int br = 0, it = 0;
-      unsigned fl = 0;
-      pair_plain<METHOD>(eps, dt, itr, rel, d, &br, &it, &fl, hist);
-      if (STATS) ps.add(br, it, fl);
+        for (int c = 0; c < 6; c++) rel[c] = ri[c] - rj[c];
+        int br = 0, it = 0;
+        unsigned fl = 0;
+        pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T);
+        if (STATS) ps.add(br, it, fl);
+      }
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        if (COUP == kScalar) {
+          acc[c] += d[c];
+          const double s = wave_sum(d[c]);
+          if (lane == jj) col[c] = -s;
+        } else {
+          acc[c] += wi * d[c];
+          const double s = wave_sum(wj * d[c]);
+          if (lane == jj) col[c] = -s;
+        }
+      }
     }
+    const int ldp = S.nt * kWave;
+    double* P = part + S.poff;
+    if (I == J) {
 #pragma unroll
-    for (int c = 0; c < 6; c++) {
-      acc[c] += d[c];
-      double s = wave_sum(d[c]);
-      if (lane == jj) col[c] = -s;
-    }
-  }
-  if (I == J) {
+      for (int c = 0; c < 6; c++) P[((size_t)I * 6 + c) * ldp + il] = acc[c] + col[c];
+    } else {
 #pragma unroll
-    for (int c = 0; c < 6; c++) part[((size_t)I * 6 + c) * ldp + i] = acc[c] + col[c];
-  } else {
-#pragma unroll
-    for (int c = 0; c < 6; c++) {
-      part[((size_t)J * 6 + c) * ldp + i] = acc[c];
-      part[((size_t)I * 6 + c) * ldp + j0 + lane] = col[c];
+      for (int c = 0; c < 6; c++) {
+        P[((size_t)J * 6 + c) * ldp + il] = acc[c];
+        P[((size_t)I * 6 + c) * ldp + j0 + lane] = col[c];
+      }
     }
   }
   if (STATS) ps.flush(stats);
 }
 
-// delta[c][i] = sum_K part[K][c][i] (+ v*dt for positions), K ascending
-__global__ __launch_bounds__(256) void k_step_reduce(int n, int lds, int ldp, int ldd, int nt, double dt,
-                                                     const double* __restrict__ st,
-                                                     const double* __restrict__ part,
-                                                     double* __restrict__ delta) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// delta[c][g] = sum_K part[K][c][local] (+ v dt for positions), K ascending; one block per
+// 256-particle chunk {system, first local index}
+__global__ __launch_bounds__(256) void k_reduce(const int2* __restrict__ chunks, const SysDesc* __restrict__ sys,
+                                                int lds, int ldd, double dt, const double* __restrict__ st,
+                                                const double* __restrict__ part, double* __restrict__ delta) {
+  const int2 ch = chunks[blockIdx.x];
+  const SysDesc S = sys[ch.x];
+  const int il = ch.y + threadIdx.x;
+  if (il >= S.n) return;
   const int c = blockIdx.y;
-  if (i >= n) return;
+  const int ldp = S.nt * kWave;
+  const double* p = part + S.poff + (size_t)c * ldp + il;
   double s = 0;
-  const double* p = part + (size_t)c * ldp + i;
-  for (int K = 0; K < nt; ++K) s += p[(size_t)K * 6 * ldp];
-  if (c < 3) s += st[(size_t)(3 + c) * lds + i] * dt;
-  delta[(size_t)c * ldd + i] = s;
+  for (int K = 0; K < S.nt; ++K) s += p[(size_t)K * 6 * ldp];
+  const int g = S.off + il;
+  if (c < 3) s += st[(size_t)(3 + c) * lds + g] * dt;
+  delta[(size_t)c * ldd + g] = s;
+}
+
+// state += delta (the update of bia/bia.f90:203-204), SoA [6][ld], first n columns
+__global__ __launch_bounds__(256) void k_update(int n, int lds, int ldd, double* __restrict__ st,
+                                                const double* __restrict__ delta) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+#pragma unroll
+  for (int c = 0; c < 6; c++) st[(size_t)c * lds + g] += delta[(size_t)c * ldd + g];
 }
 
 // reference per-particle form: thread per j > i on an AoS relative-state row
@@ -149,43 +202,261 @@ __global__ __launch_bounds__(kWave) void k_legacy_row(int n, int i, double eps, 
   for (int c = 0; c < 6; c++) out_aos[(size_t)6 * j + c] = d[c];
 }
 
-// the step on device-resident SoA state; caller holds ctx->mu
-static int step_dev_locked(bia_ctx* ctx, int method, int mode, int n, double eps, double dt, int itr,
-                           const double* d_state, int ld, double* d_delta, unsigned long long* d_stats) {
-  if (n < 1 || ld < n || !d_state || !d_delta) return BIA_ERR_ARG;
-  if (method != BIA_METHOD_ALG && method != BIA_METHOD_RKF) return BIA_ERR_ARG;
-  if (mode != BIA_MODE_PLAIN) return BIA_ERR_ARG;  // learned mode dispatches elsewhere
-  if (itr < 0 || itr > kMaxItr) return BIA_ERR_ARG;
-  const int nt = (n + kWave - 1) / kWave;
-  const int ldp = nt * kWave;  // partial rows are padded to whole tiles
-  BIA_TRY(ctx, ensure_tiles(ctx, nt));
-  BIA_TRY(ctx, ctx->part.ensure((size_t)nt * 6 * ldp * sizeof(double)));
-  const unsigned ntiles = (unsigned)((size_t)nt * (nt + 1) / 2);
-  double* part = ctx->part.as<double>();
-  const int2* tiles = ctx->tiles.as<int2>();
-  dim3 g(ntiles), b(kWave);
-  const bool stats = d_stats != nullptr;
-  BIA_TRY(ctx, prof_begin(ctx));
-  if (method == BIA_METHOD_ALG) {
-    if (stats)
-      hipLaunchKernelGGL((k_step_tiles<1, true>), g, b, 0, ctx->stream, n, ld, ldp, eps, dt, itr, d_state, tiles,
-                         part, d_stats);
-    else
-      hipLaunchKernelGGL((k_step_tiles<1, false>), g, b, 0, ctx->stream, n, ld, ldp, eps, dt, itr, d_state, tiles,
-                         part, d_stats);
-  } else {
-    if (stats)
-      hipLaunchKernelGGL((k_step_tiles<0, true>), g, b, 0, ctx->stream, n, ld, ldp, eps, dt, itr, d_state, tiles,
-                         part, d_stats);
-    else
-      hipLaunchKernelGGL((k_step_tiles<0, false>), g, b, 0, ctx->stream, n, ld, ldp, eps, dt, itr, d_state, tiles,
-                         part, d_stats);
+// ------------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------------
+struct StepArgs {
+  int method, mode, coupling;
+  int nsys;
+  const int* sizes;   // host, nsys
+  const double* eps;  // host, nsys (SCALAR coupling), may be NULL for the others
+  double coup;        // G or k
+  double dt;
+  int itr;
+  const double* d_state;
+  int ld;
+  const double* d_mass;
+  const double* d_charge;
+  double* d_delta;
+  unsigned long long* d_stats;
+};
+
+// build (or reuse) the plan for this list of sizes; upload system descriptors when eps changed
+static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
+  Plan& P = ctx->plan;
+  std::vector<int> sizes(a.sizes, a.sizes + a.nsys);
+  if (P.sizes != sizes) {
+    Plan Q;
+    Q.sizes = sizes;
+    Q.nsys = a.nsys;
+    std::vector<int4> tiles;
+    std::vector<int2> chunks;
+    long long off = 0, poff = 0;
+    for (int s = 0; s < a.nsys; s++) {
+      const int n = sizes[s];
+      if (n < 1) return BIA_ERR_ARG;
+      const int nt = (n + kWave - 1) / kWave;
+      SysDesc d{(int)off, n, nt, 0, poff, 0.0};
+      Q.hsys.push_back(d);
+      for (int I = 0; I < nt; I++)
+        for (int J = I; J < nt; J++) tiles.push_back(make_int4(I, J, s, 0));
+      for (int c0 = 0; c0 < n; c0 += 256) chunks.push_back(make_int2(s, c0));
+      off += n;
+      poff += (long long)nt * 6 * nt * kWave;
+    }
+    if (off > 0x7fffffffLL || tiles.size() > 0x7fffffffULL) return BIA_ERR_ARG;
+    Q.ntotal = (int)off;
+    Q.ntiles = (int)tiles.size();
+    Q.nchunks = (int)chunks.size();
+    Q.part_doubles = (size_t)poff;
+    BIA_TRY(ctx, ctx->plan_tiles.ensure(tiles.size() * sizeof(int4)));
+    BIA_TRY(ctx, ctx->plan_chunks.ensure(chunks.size() * sizeof(int2)));
+    BIA_TRY(ctx, ctx->plan_sys.ensure(Q.hsys.size() * sizeof(SysDesc)));
+    BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_tiles.p, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+    BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_chunks.p, chunks.data(), chunks.size() * sizeof(int2), hipMemcpyHostToDevice));
+    auto it = ctx->tuned_grid.find(sizes);
+    Q.grid = (it != ctx->tuned_grid.end()) ? it->second : 0;
+    P = Q;
+    for (auto& d : P.hsys) d.eps = __builtin_nan("");  // force the descriptor upload below
   }
+  bool dirty = false;
+  for (int s = 0; s < a.nsys; s++) {
+    const double e = a.eps ? a.eps[s] : 0.0;
+    if (!(P.hsys[s].eps == e)) {
+      P.hsys[s].eps = e;
+      dirty = true;
+    }
+  }
+  if (dirty)
+    BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_sys.p, P.hsys.data(), P.hsys.size() * sizeof(SysDesc), hipMemcpyHostToDevice));
+  BIA_TRY(ctx, ctx->part.ensure(P.part_doubles * sizeof(double)));
+  return BIA_OK;
+}
+
+static int validate(bia_ctx* ctx, const StepArgs& a) {
+  if (a.nsys < 1 || !a.sizes || !a.d_state || !a.d_delta) return BIA_ERR_ARG;
+  if (a.method != BIA_METHOD_ALG && a.method != BIA_METHOD_RKF) return BIA_ERR_ARG;
+  if (a.mode != BIA_MODE_PLAIN && a.mode != BIA_MODE_LEARN && a.mode != BIA_MODE_GUESS) return BIA_ERR_ARG;
+  if (a.mode != BIA_MODE_PLAIN && a.method != BIA_METHOD_ALG) return BIA_ERR_ARG;
+  if (a.coupling < BIA_COUPLING_SCALAR || a.coupling > BIA_COUPLING_COULOMB) return BIA_ERR_ARG;
+  if (a.coupling == BIA_COUPLING_SCALAR && !a.eps) return BIA_ERR_ARG;
+  if (a.coupling == BIA_COUPLING_GRAVITY && !a.d_mass) return BIA_ERR_ARG;
+  if (a.coupling == BIA_COUPLING_COULOMB && !a.d_charge) return BIA_ERR_ARG;
+  if (a.itr < 0 || a.itr > kMaxItr) return BIA_ERR_ARG;
+  long long tot = 0;
+  for (int s = 0; s < a.nsys; s++) tot += a.sizes[s];
+  if (a.ld < tot) return BIA_ERR_ARG;
+  if (a.mode == BIA_MODE_LEARN && !ctx->mlp_ready) {
+    snprintf(ctx->last_error, sizeof(ctx->last_error), "learned mode needs a trained network (bia_mlp_build_table)");
+    return BIA_ERR_STATE;
+  }
+  return BIA_OK;
+}
+
+// enqueue pair kernel + reduction on ctx->stream (no synchronisation, no allocation once planned)
+static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override) {
+  const Plan& P = ctx->plan;
+  const LearnTable T{ctx->mlp_table.as<float>(), ctx->table_nx, ctx->table_nv};
+  int grid = grid_override > 0 ? grid_override : (P.grid > 0 ? P.grid : P.ntiles);
+  grid = std::min(grid, P.ntiles);
+  const int4* tiles = ctx->plan_tiles.as<int4>();
+  const SysDesc* sys = ctx->plan_sys.as<SysDesc>();
+  double* part = ctx->part.as<double>();
+  const bool st = a.d_stats != nullptr;
+  BIA_TRY(ctx, prof_begin(ctx));
+#define L_(M, MO, CP, S)                                                                                         \
+  hipLaunchKernelGGL((k_tiles<M, MO, CP, S>), dim3(grid), dim3(kWave), 0, ctx->stream, P.ntiles, a.ld, tiles, sys, \
+                     a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, a.d_stats, T)
+#define L_COUP(M, MO, S)                                                  \
+  do {                                                                    \
+    if (a.coupling == BIA_COUPLING_SCALAR) L_(M, MO, kScalar, S);         \
+    else if (a.coupling == BIA_COUPLING_GRAVITY) L_(M, MO, kGravity, S);  \
+    else L_(M, MO, kCoulomb, S);                                          \
+  } while (0)
+  if (a.method == BIA_METHOD_RKF) {
+    if (st) L_COUP(0, kPlain, true);
+    else L_COUP(0, kPlain, false);
+  } else if (a.mode == BIA_MODE_PLAIN) {
+    if (st) L_COUP(1, kPlain, true);
+    else L_COUP(1, kPlain, false);
+  } else if (a.mode == BIA_MODE_LEARN) {
+    if (st) L_COUP(1, kLearn, true);
+    else L_COUP(1, kLearn, false);
+  } else {
+    if (st) L_COUP(1, kGuess, true);
+    else L_COUP(1, kGuess, false);
+  }
+#undef L_COUP
+#undef L_
   BIA_HIP_TRY(ctx, hipGetLastError());
   BIA_TRY(ctx, prof_end(ctx));
-  hipLaunchKernelGGL(k_step_reduce, dim3((n + 255) / 256, 6), dim3(256), 0, ctx->stream, n, ld, ldp, ld, nt, dt,
-                     d_state, part, d_delta);
+  hipLaunchKernelGGL(k_reduce, dim3(P.nchunks, 6), dim3(256), 0, ctx->stream, ctx->plan_chunks.as<int2>(), sys, a.ld,
+                     a.ld, a.dt, a.d_state, part, a.d_delta);
   BIA_HIP_TRY(ctx, hipGetLastError());
+  return BIA_OK;
+}
+
+static int step_locked(bia_ctx* ctx, const StepArgs& a) {
+  BIA_TRY(ctx, validate(ctx, a));
+  BIA_TRY(ctx, ensure_plan(ctx, a));
+  return launch_step(ctx, a, 0);
+}
+
+// Time candidate grid sizes of the persistent pair kernel on the caller's input (one step each,
+// outputs written to d_delta) and keep the fastest for this list of sizes.
+static int autotune_locked(bia_ctx* ctx, const StepArgs& a, float* best_ms, int* best_grid) {
+  BIA_TRY(ctx, validate(ctx, a));
+  BIA_TRY(ctx, ensure_plan(ctx, a));
+  int ncu = 0;
+  BIA_HIP_TRY(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const int ntiles = ctx->plan.ntiles;
+  std::vector<int> cands = {ntiles};
+  for (int m : {4, 8, 16, 32, 64})
+    if (m * ncu < ntiles) cands.push_back(m * ncu);
+  hipEvent_t e0, e1;
+  BIA_HIP_TRY(ctx, hipEventCreate(&e0));
+  BIA_HIP_TRY(ctx, hipEventCreate(&e1));
+  StepArgs b = a;
+  b.d_stats = nullptr;
+  const int saved_prof = ctx->profiling;
+  ctx->profiling = 0;
+  float best = 1e30f;
+  int bestg = ntiles;
+  int status = BIA_OK;
+  for (int g : cands) {
+    if ((status = launch_step(ctx, b, g)) != BIA_OK) break;  // warm-up
+    (void)hipEventRecord(e0, ctx->stream);
+    if ((status = launch_step(ctx, b, g)) != BIA_OK) break;
+    (void)hipEventRecord(e1, ctx->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best) {
+      best = ms;
+      bestg = g;
+    }
+  }
+  ctx->profiling = saved_prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (status != BIA_OK) return status;
+  ctx->plan.grid = bestg;
+  ctx->plan.tuned_ms = best;
+  ctx->tuned_grid[ctx->plan.sizes] = bestg;
+  if (best_ms) *best_ms = best;
+  if (best_grid) *best_grid = bestg;
+  return BIA_OK;
+}
+
+// nsteps of (step, state += delta) on the stream; with use_graph the step is captured once into a
+// hipGraph (captured again when the arguments change) and replayed
+static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, int nsteps);
+
+static int run_steps_locked(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, int nsteps, int use_graph) {
+  BIA_TRY(ctx, validate(ctx, a));
+  BIA_TRY(ctx, ensure_plan(ctx, a));
+  const int ntot = ctx->plan.ntotal;
+  if (use_graph) {
+    // capture is not possible on the null stream: use a private blocking stream, which stays
+    // ordered with null-stream work on both sides
+    hipStream_t user = ctx->stream;
+    if (!user) {
+      if (!ctx->cap_stream) BIA_HIP_TRY(ctx, hipStreamCreate(&ctx->cap_stream));
+      ctx->stream = ctx->cap_stream;
+    }
+    const int s = run_steps_graph(ctx, a, d_state_rw, nsteps);
+    ctx->stream = user;
+    return s;
+  }
+  {
+    for (int k = 0; k < nsteps; k++) {
+      BIA_TRY(ctx, launch_step(ctx, a, 0));
+      hipLaunchKernelGGL(k_update, dim3((ntot + 255) / 256), dim3(256), 0, ctx->stream, ntot, a.ld, a.ld, d_state_rw,
+                         a.d_delta);
+      BIA_HIP_TRY(ctx, hipGetLastError());
+    }
+    return BIA_OK;
+  }
+}
+
+static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, int nsteps) {
+  const int ntot = ctx->plan.ntotal;
+  std::vector<long long> key = {a.method, a.mode, a.coupling, a.itr, (long long)(size_t)d_state_rw,
+                                (long long)(size_t)a.d_delta, (long long)(size_t)a.d_mass,
+                                (long long)(size_t)a.d_charge, a.ld, (long long)(size_t)ctx->stream, ctx->plan.grid,
+                                ctx->mlp_ready, (long long)(size_t)ctx->part.p,
+                                (long long)(size_t)ctx->mlp_table.p};
+  long long dtb, cb;
+  memcpy(&dtb, &a.dt, 8);
+  memcpy(&cb, &a.coup, 8);
+  key.push_back(dtb);
+  key.push_back(cb);
+  for (int s : ctx->plan.sizes) key.push_back(s);
+  if (key != ctx->graph_key || !ctx->graph_exec) {
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    const int saved_prof = ctx->profiling;
+    ctx->profiling = 0;  // no event records inside the graph
+    BIA_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int s = launch_step(ctx, a, 0);
+    if (s == BIA_OK)
+      hipLaunchKernelGGL(k_update, dim3((ntot + 255) / 256), dim3(256), 0, ctx->stream, ntot, a.ld, a.ld, d_state_rw,
+                         a.d_delta);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->profiling = saved_prof;
+    if (s != BIA_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    BIA_HIP_TRY(ctx, e);
+    ctx->graph = g;
+    BIA_HIP_TRY(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+    ctx->graph_key = key;
+  }
+  for (int k = 0; k < nsteps; k++) BIA_HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
   return BIA_OK;
 }
 
@@ -193,63 +464,78 @@ static int step_dev_locked(bia_ctx* ctx, int method, int mode, int n, double eps
 
 using namespace bia;
 
+#define BIA_LOCKED(ctx)                       \
+  if (!(ctx)) return BIA_ERR_ARG;             \
+  std::lock_guard<std::mutex> lk_((ctx)->mu); \
+  (ctx)->last_error[0] = 0;                   \
+  BIA_HIP_TRY((ctx), hipSetDevice((ctx)->device))
+
 extern "C" {
 
-int bia_step_dev(bia_ctx* ctx, int method, int mode, int n, double eps, double dt, int itr,
-                 const double* d_state, int ld, double* d_delta, unsigned long long* d_stats) {
-  if (!ctx) return BIA_ERR_ARG;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  ctx->last_error[0] = 0;
-  BIA_HIP_TRY(ctx, hipSetDevice(ctx->device));
-  if (mode == BIA_MODE_LEARN) {
-    if (method != BIA_METHOD_ALG) return BIA_ERR_ARG;
-    return step_learn_locked(ctx, n, eps, dt, itr, d_state, ld, d_delta, d_stats);
-  }
-  return step_dev_locked(ctx, method, mode, n, eps, dt, itr, d_state, ld, d_delta, d_stats);
+int bia_step_dev(bia_ctx* ctx, int method, int mode, int n, double eps, double dt, int itr, const double* d_state,
+                 int ld, double* d_delta, unsigned long long* d_stats) {
+  BIA_LOCKED(ctx);
+  StepArgs a{method, mode, BIA_COUPLING_SCALAR, 1, &n, &eps, 0.0, dt, itr, d_state, ld, nullptr, nullptr, d_delta,
+             d_stats};
+  return step_locked(ctx, a);
+}
+
+int bia_step_batched_dev(bia_ctx* ctx, int method, int mode, int coupling, int nsys, const int* sizes,
+                         const double* eps, double coup, double dt, int itr, const double* d_state, int ld,
+                         const double* d_mass, const double* d_charge, double* d_delta, unsigned long long* d_stats) {
+  BIA_LOCKED(ctx);
+  StepArgs a{method, mode, coupling, nsys, sizes, eps, coup, dt, itr, d_state, ld, d_mass, d_charge, d_delta, d_stats};
+  return step_locked(ctx, a);
+}
+
+int bia_autotune_dev(bia_ctx* ctx, int method, int mode, int coupling, int nsys, const int* sizes, const double* eps,
+                     double coup, double dt, int itr, const double* d_state, int ld, const double* d_mass,
+                     const double* d_charge, double* d_delta, float* best_ms, int* best_grid) {
+  BIA_LOCKED(ctx);
+  StepArgs a{method, mode, coupling, nsys, sizes, eps, coup, dt, itr, d_state, ld, d_mass, d_charge, d_delta, nullptr};
+  return autotune_locked(ctx, a, best_ms, best_grid);
+}
+
+int bia_run_steps_dev(bia_ctx* ctx, int method, int mode, int coupling, int nsys, const int* sizes, const double* eps,
+                      double coup, double dt, int itr, double* d_state, int ld, const double* d_mass,
+                      const double* d_charge, double* d_delta, int nsteps, int use_graph) {
+  BIA_LOCKED(ctx);
+  if (nsteps < 0) return BIA_ERR_ARG;
+  StepArgs a{method, mode, coupling, nsys, sizes, eps, coup, dt, itr, d_state, ld, d_mass, d_charge, d_delta, nullptr};
+  return run_steps_locked(ctx, a, d_state, nsteps, use_graph);
 }
 
 int bia_step_host(bia_ctx* ctx, int method, int mode, int n, double eps, double dt, int itr, const double* X,
-                  const double* Y, const double* Z, const double* U, const double* V, const double* W,
-                  double* dX, double* dY, double* dZ, double* dU, double* dV, double* dW,
-                  unsigned long long* stats) {
+                  const double* Y, const double* Z, const double* U, const double* V, const double* W, double* dX,
+                  double* dY, double* dZ, double* dU, double* dV, double* dW, unsigned long long* stats) {
   if (!ctx || n < 1) return BIA_ERR_ARG;
   const double* in[6] = {X, Y, Z, U, V, W};
   double* out[6] = {dX, dY, dZ, dU, dV, dW};
   for (int c = 0; c < 6; c++)
     if (!in[c] || !out[c]) return BIA_ERR_ARG;
-  const int ld = ((n + kWave - 1) / kWave) * kWave;
-  {
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    ctx->last_error[0] = 0;
-    BIA_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    BIA_TRY(ctx, ctx->delta.ensure((size_t)6 * ld * sizeof(double) * 2 + BIA_STATS_LEN * 8));
-    // host staging: [state 6*ld][delta 6*ld][stats]
-    double* dst = ctx->delta.as<double>();
-    double* ddl = dst + (size_t)6 * ld;
-    unsigned long long* dsa = reinterpret_cast<unsigned long long*>(ddl + (size_t)6 * ld);
-    BIA_HIP_TRY(ctx, hipMemsetAsync(dst, 0, (size_t)6 * ld * sizeof(double), ctx->stream));
-    for (int c = 0; c < 6; c++)
-      BIA_HIP_TRY(ctx, hipMemcpyAsync(dst + (size_t)c * ld, in[c], (size_t)n * sizeof(double),
-                                      hipMemcpyHostToDevice, ctx->stream));
-    BIA_HIP_TRY(ctx, hipMemsetAsync(dsa, 0, BIA_STATS_LEN * 8, ctx->stream));
-  }
+  BIA_LOCKED(ctx);
+  const int ld = n;
+  // device staging: [state 6n][delta 6n][stats]
+  BIA_TRY(ctx, ctx->delta.ensure((size_t)12 * ld * sizeof(double) + BIA_STATS_LEN * 8));
   double* dst = ctx->delta.as<double>();
   double* ddl = dst + (size_t)6 * ld;
   unsigned long long* dsa = reinterpret_cast<unsigned long long*>(ddl + (size_t)6 * ld);
-  int s = bia_step_dev(ctx, method, mode, n, eps, dt, itr, dst, ld, ddl, dsa);
-  if (s != BIA_OK) return s;
-  std::lock_guard<std::mutex> lk(ctx->mu);
   for (int c = 0; c < 6; c++)
-    BIA_HIP_TRY(ctx, hipMemcpyAsync(out[c], ddl + (size_t)c * ld, (size_t)n * sizeof(double),
-                                    hipMemcpyDeviceToHost, ctx->stream));
-  if (stats)
-    BIA_HIP_TRY(ctx, hipMemcpyAsync(stats, dsa, BIA_STATS_LEN * 8, hipMemcpyDeviceToHost, ctx->stream));
+    BIA_HIP_TRY(ctx, hipMemcpyAsync(dst + (size_t)c * ld, in[c], (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                                    ctx->stream));
+  BIA_HIP_TRY(ctx, hipMemsetAsync(dsa, 0, BIA_STATS_LEN * 8, ctx->stream));
+  StepArgs a{method, mode, BIA_COUPLING_SCALAR, 1, &n, &eps, 0.0, dt, itr, dst, ld, nullptr, nullptr, ddl, dsa};
+  BIA_TRY(ctx, step_locked(ctx, a));
+  for (int c = 0; c < 6; c++)
+    BIA_HIP_TRY(ctx, hipMemcpyAsync(out[c], ddl + (size_t)c * ld, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+  if (stats) BIA_HIP_TRY(ctx, hipMemcpyAsync(stats, dsa, BIA_STATS_LEN * 8, hipMemcpyDeviceToHost, ctx->stream));
   BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return BIA_OK;
 }
 
-void find_dxyuv_kernel_(int METHOD, int N, int I, double EPS, double DT, int ITR, double* dX, double* dY,
-                        double* dZ, double* dU, double* dV, double* dW, double* RXYZUVW, double* dXYZUVW) {
+void find_dxyuv_kernel_(int METHOD, int N, int I, double EPS, double DT, int ITR, double* dX, double* dY, double* dZ,
+                        double* dU, double* dV, double* dW, double* RXYZUVW, double* dXYZUVW) {
   bia_ctx* ctx = default_ctx();
   const int i = I - 1;  // Fortran index
   if (N < 1 || i < 0 || i >= N || ITR < 0 || ITR > kMaxItr || (METHOD != 0 && METHOD != 1)) {
@@ -262,8 +548,7 @@ void find_dxyuv_kernel_(int METHOD, int N, int I, double EPS, double DT, int ITR
   if ((s = ctx->leg_in.ensure(bytes)) != BIA_OK) legacy_fail("find_dxyuv_kernel_ allocation", s, ctx);
   if ((s = ctx->leg_out.ensure(bytes)) != BIA_OK) legacy_fail("find_dxyuv_kernel_ allocation", s, ctx);
   hipError_t e = hipSetDevice(ctx->device);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(ctx->leg_in.p, RXYZUVW, bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ctx->leg_in.p, RXYZUVW, bytes, hipMemcpyHostToDevice, ctx->stream);
   const int cnt = N - (i + 1);
   if (e == hipSuccess && cnt > 0) {
     dim3 g((cnt + kWave - 1) / kWave), b(kWave);

@@ -42,6 +42,12 @@ extern "C" {
 /* initialiser modes for the analytic hyperbola branch */
 #define BIA_MODE_PLAIN 0  /* the reference's starting guess and iteration control */
 #define BIA_MODE_LEARN 1  /* learned starting guess + safeguarded Newton, reference guess as fallback */
+#define BIA_MODE_GUESS 2  /* the reference's closed-form guess + the same safeguarded Newton (control) */
+
+/* pair couplings of the batched entry points */
+#define BIA_COUPLING_SCALAR 0  /* the reference's r'' = eps r/|r|^3, eps per system, equal-mass half split */
+#define BIA_COUPLING_GRAVITY 1 /* eps_ij = -G (m_i + m_j), centre-of-mass split m_j/(m_i+m_j) : m_i/(m_i+m_j) */
+#define BIA_COUPLING_COULOMB 2 /* eps_ij = k q_i q_j (1/m_i + 1/m_j), same split (unit masses if mass NULL) */
 
 /* per-step statistics written by the step entry points (uint64 counters, BIA_STATS_LEN long) */
 #define BIA_STATS_LEN 16
@@ -117,6 +123,29 @@ int bia_last_kernel_ms(bia_ctx *ctx, float *ms);
 int bia_step_dev(bia_ctx *ctx, int method, int mode, int n, double eps, double dt, int itr,
                  const double *d_state, int ld, double *d_delta, unsigned long long *d_stats);
 
+/* Batched step: nsys independent systems of uneven sizes in one persistent launch.  The systems
+ * are concatenated column-wise in one SoA state [6][ld] (system s occupies the next sizes[s]
+ * columns); d_delta has the same layout.  No counterpart in the reference ABI (its only ensemble
+ * is the sequential N_calc loop of bia/bia.f90:62).
+ *   coupling   BIA_COUPLING_*; eps (host, nsys) for SCALAR, coup = G or k for the others
+ *   d_mass, d_charge  per-particle arrays (device, ld) for GRAVITY / COULOMB, else NULL */
+int bia_step_batched_dev(bia_ctx *ctx, int method, int mode, int coupling, int nsys, const int *sizes,
+                         const double *eps, double coup, double dt, int itr, const double *d_state, int ld,
+                         const double *d_mass, const double *d_charge, double *d_delta, unsigned long long *d_stats);
+
+/* Autotuner: times the pair kernel's persistent grid size on this input (one step per candidate,
+ * written to d_delta) and keeps the fastest for this list of sizes in the context. */
+int bia_autotune_dev(bia_ctx *ctx, int method, int mode, int coupling, int nsys, const int *sizes, const double *eps,
+                     double coup, double dt, int itr, const double *d_state, int ld, const double *d_mass,
+                     const double *d_charge, double *d_delta, float *best_ms, int *best_grid);
+
+/* Device-resident multi-step driver: nsteps of (step, state += delta) enqueued on the context's
+ * stream with no host synchronisation; with use_graph the step is captured once as a hipGraph and
+ * replayed.  d_state is updated in place; d_delta holds the last step's deltas. */
+int bia_run_steps_dev(bia_ctx *ctx, int method, int mode, int coupling, int nsys, const int *sizes, const double *eps,
+                      double coup, double dt, int itr, double *d_state, int ld, const double *d_mass,
+                      const double *d_charge, double *d_delta, int nsteps, int use_graph);
+
 /* Host-array variant of bia_step_dev for Fortran callers (bind(C) interface in
  * bia_3d_gpu_amd/fortran/bia_hip.f90).  dX..dW are overwritten; stats (may be NULL) is
  * BIA_STATS_LEN counters for this call. */
@@ -138,6 +167,34 @@ int bia_invariants_host(bia_ctx *ctx, int n, double eps, const double *X, const 
 int bia_dim_rkf45_host(bia_ctx *ctx, int np, double eps, double *x0, double xe, double *y, int *init,
                        double relerr, double abserr, double *h, double *ese, int *nde, int *naccept,
                        int *nreject, int *ier);
+
+/* ---------------------------------------------------------------------------------------- */
+/* learned initialiser (no counterpart in the reference)                                     */
+/* ---------------------------------------------------------------------------------------- */
+/* A 3 -> 64 -> 64 -> 1 GELU network proposes the starting true anomaly of the hyperbola
+ * Newton solve (BIA_MODE_LEARN).  Inputs of one hyperbolic pair: x1 = theta_gss / theta_0 (the
+ * reference's closed-form guess), v = e^(-1/4), b = 0 attractive / 1 repulsive; output
+ * t = (theta / theta_0 - x1) e.  Sample rows are float [x1, v, b, t].  Weights are 4481 floats:
+ * W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3.  Products run on BF16 MFMA with FP32
+ * accumulation; the master weights are FP32. */
+int bia_mlp_nparam(void);
+int bia_mlp_init(bia_ctx *ctx, unsigned long long seed);
+int bia_mlp_set_weights(bia_ctx *ctx, const float *w, int n);
+int bia_mlp_get_weights(bia_ctx *ctx, float *w, int n);
+/* d_rows [nrows][4] (device; the 4th column is ignored) -> d_y [nrows] (device) */
+int bia_mlp_forward_dev(bia_ctx *ctx, const float *d_rows, int nrows, float *d_y);
+/* gradient of mean((y - t)^2) over all rows; d_grad (device) receives 4481 floats + the loss */
+int bia_mlp_grad_dev(bia_ctx *ctx, const float *d_rows, int nrows, float *d_grad);
+/* Adam, cosine-decayed learning rate, random mini-batches of `batch` rows per step;
+ * loss_curve (host, may be NULL) receives the loss of each step */
+int bia_mlp_train_dev(bia_ctx *ctx, const float *d_rows, int nrows, int steps, int batch, float lr,
+                      unsigned long long seed, float *loss_curve);
+/* tabulate the network on an nx x nv grid per branch for the pair kernel; enables BIA_MODE_LEARN */
+int bia_mlp_build_table(bia_ctx *ctx, int nx, int nv);
+/* record nsamples random pairs of a system: rows [x1, v, b, t] for hyperbolic pairs solved to
+ * convergence, x1 = NaN for the others (device output) */
+int bia_record_samples_dev(bia_ctx *ctx, int n, double eps, double dt, const double *d_state, int ld, int nsamples,
+                           unsigned long long seed, float *d_out);
 
 #ifdef __cplusplus
 }

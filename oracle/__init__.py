@@ -51,6 +51,10 @@ def lib():
                                     C.POINTER(C.c_int), C.c_void_p, C.c_void_p, C.c_int]
         L.orc_dim_rkf45.restype = C.c_int
         L.orc_dim_func.argtypes = [C.c_int, C.c_double, _dp, _dp]
+        L.orc_bia_rows_coupled.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp,
+                                           C.c_void_p, C.c_void_p, C.c_int, C.c_int, _dp]
+        L.orc_sensitivity.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, C.c_void_p,
+                                      C.c_void_p, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -137,6 +141,20 @@ def bia_step(state_soa: np.ndarray, eps: float, dt: float, method: int = 1, itr:
     return (d, h) if hist else d
 
 
+def bia_step_coupled(state_soa: np.ndarray, coupling: int, coup: float, dt: float, mass=None, charge=None,
+                     method: int = 1, itr: int = 18) -> np.ndarray:
+    """Step with per-pair couplings (1 gravity with masses, 2 Coulomb); deltas SoA [6, n]."""
+    st = np.ascontiguousarray(state_soa, dtype=np.float64).reshape(6, -1)
+    n = st.shape[1]
+    d = np.zeros((6, n))
+    m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+    q = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64)
+    lib().orc_bia_rows_coupled(method, n, coupling, coup, dt, itr, st,
+                               None if m is None else m.ctypes.data_as(C.c_void_p),
+                               None if q is None else q.ctypes.data_as(C.c_void_p), 0, n, d)
+    return d
+
+
 def invariance(state_soa: np.ndarray, eps: float, kind: str = "gpu") -> np.ndarray:
     """KE, PE, Lx, Ly, Lz, Ua, Va, Wa, Xa, Ya, Za (kind: gpu order, cpu order, quad)."""
     st = np.ascontiguousarray(state_soa, dtype=np.float64).reshape(6, -1)
@@ -201,3 +219,46 @@ def ref_cpu_rows(state_soa: np.ndarray, eps: float, dt: float, method: int = 1, 
     d = np.zeros((6, n))
     ref().ref_cpu_rows(method, n, eps, dt, itr, st, i0, i1, d)
     return d
+
+
+# ------------------------------------------------------------------------------------------
+# parity bound: the reference formulas' own sensitivity to one rounding unit
+# ------------------------------------------------------------------------------------------
+def sensitivity(state_soa: np.ndarray, eps: float, dt: float, method: int = 1, itr: int = 18, samples: int = 6,
+                coupling: int = 0, mass=None, charge=None):
+    """Per-particle sums over its pairs of the largest change of the pair's (position, velocity)
+    half-deltas under random 1-ulp perturbations of the pair's six relative-state inputs.
+
+    Two correct evaluations of the reference formulas whose libm results differ by one ulp land
+    about this far apart.  It is large where the reference's formulation is ill-conditioned:
+    acos(q) with q near 1 (pairs starting near periapsis, bia/find_dXYUV_kernel.cu:124) and
+    near-parabolic hyperbolae (W = g^2 + 2 eps/r cancels, :98-107)."""
+    st = np.ascontiguousarray(state_soa, dtype=np.float64).reshape(6, -1)
+    n = st.shape[1]
+    bp, bv = np.zeros(n), np.zeros(n)
+    m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+    q = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64)
+    lib().orc_sensitivity(method, n, coupling, eps, dt, itr, st, None if m is None else m.ctypes.data_as(C.c_void_p),
+                          None if q is None else q.ctypes.data_as(C.c_void_p), samples, bp, bv)
+    return bp, bv
+
+
+def within_condition_bound(state_soa, d, ref, eps, dt: float = None, factor: float = 10.0, floor: float = 1e-13,
+                           method: int = 1, bounds=None):
+    """Per-particle check of deltas d against ref: |d - ref| <= factor * sensitivity + floor *
+    max|x + ref| (positions and velocities separately).  Returns (ok, worst ratio pos, vel).
+
+    factor = 10: on the reference's own N = 1332 fixture its CUDA output (OUT/*.BIA) and the
+    glibc evaluation of the same formulas differ by up to 7x this sensitivity estimate."""
+    st = np.asarray(state_soa, dtype=np.float64).reshape(6, -1)
+    if bounds is None:
+        if dt is None:
+            raise ValueError("dt is needed to evaluate the sensitivity")
+        bounds = sensitivity(st, eps, dt, method)
+    bp, bv = bounds
+    ep = np.abs(d[:3] - ref[:3]).max(0)
+    ev = np.abs(d[3:] - ref[3:]).max(0)
+    ap = factor * bp + floor * np.abs(st[:3] + ref[:3]).max()
+    av = factor * bv + floor * np.abs(st[3:] + ref[3:]).max()
+    rp, rv = (ep / ap).max(), (ev / av).max()
+    return bool(rp <= 1 and rv <= 1), float(rp), float(rv)

@@ -61,6 +61,14 @@ int orc_dim_rkf45(int np, double eps, double *x0, double xe, double *y, double r
                   double *ese, int *nde, int *naccept, int *nreject, double *step_t, double *step_y1,
                   int cap);
 
+/* coupled step: coupling 1 gravity with masses, 2 Coulomb with charges (and masses) */
+void orc_bia_rows_coupled(int method, int n, int coupling, double coup, double dt, int itr, const double *st,
+                          const double *mass, const double *charge, int i0, int i1, double *delta);
+
+/* per-particle sums of the pair formulas' sensitivity to one rounding unit of their inputs */
+void orc_sensitivity(int method, int n, int coupling, double eps, double dt, int itr, const double *st,
+                     const double *mass, const double *charge, int samples, double *bpos, double *bvel);
+
 /* one direct-sum derivative evaluation f = (v, a) on an AoS state */
 void orc_dim_func(int np, double eps, const double *y, double *f);
 

@@ -4,16 +4,21 @@
 #include "../../bia_3d_gpu_amd/csrc/bia_pair.hpp"
 
 extern "C" void hs_pairs(int method, int64_t n, double eps, double dt, int itr, const double* rel, double* out,
-                         int* branch, int* iters, unsigned* flags) {
+                         int* branch, int* iters, unsigned* flags, int mode, const float* table, int nx, int nv) {
   double hist[bia::kMaxItr];
+  const bia::LearnTable T{table, nx, nv};
   for (int64_t k = 0; k < n; k++) {
     bia::Hist h{hist, 1};
     int br = 0, it = 0;
     unsigned fl = 0;
-    if (method == 1)
-      bia::pair_plain<1>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h);
-    else
+    if (method == 0)
       bia::pair_plain<0>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h);
+    else if (mode == 1)
+      bia::pair_eval<1, bia::kLearn>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h, T);
+    else if (mode == 2)
+      bia::pair_eval<1, bia::kGuess>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h, T);
+    else
+      bia::pair_plain<1>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h);
     branch[k] = br;
     iters[k] = it;
     flags[k] = fl;

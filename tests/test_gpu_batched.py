@@ -1,0 +1,130 @@
+"""Batched (ensemble) steps, per-particle couplings, the hipGraph multi-step driver and the
+autotuner, through the C ABI on the GPU."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cat(states):
+    return np.ascontiguousarray(np.concatenate(states, axis=1))
+
+
+def test_batched_equals_single_system_calls(ctx):
+    import torch
+    from bia_3d_gpu_amd import ic
+    sizes = [100, 333, 1000]
+    states = [ic.plummer_system(n, seed=50 + k) for k, n in enumerate(sizes)]
+    eps = [-2.0 / n for n in sizes]
+    s = torch.from_numpy(_cat(states)).cuda()
+    d = torch.empty_like(s)
+    ctx.step_batched_(s, d, sizes, eps=eps, dt=1e-3)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    off = 0
+    for st, n, e in zip(states, sizes, eps):
+        single = ctx.step_host(st, e, 1e-3)
+        assert np.array_equal(got[:, off:off + n], single)  # same tiles, same reduction order
+        off += n
+
+
+def test_gravity_coupling_equal_masses_is_the_reference_step(ctx):
+    import torch
+    from bia_3d_gpu_amd import ic, COUPLING_GRAVITY
+    n = 700
+    st = ic.plummer_system(n, seed=9)
+    s = torch.from_numpy(st).cuda()
+    m = torch.full((n,), 1.0 / n, dtype=torch.float64, device="cuda")
+    d = torch.empty_like(s)
+    ctx.step_batched_(s, d, [n], dt=1e-3, coupling=COUPLING_GRAVITY, coup=1.0, mass=m)
+    torch.cuda.synchronize()
+    assert np.array_equal(d.cpu().numpy(), ctx.step_host(st, -2.0 / n, 1e-3))
+
+
+@pytest.mark.parametrize("kind", ["gravity", "coulomb"])
+def test_coupled_step_vs_oracle(ctx, kind):
+    import torch
+    from bia_3d_gpu_amd import ic, COUPLING_GRAVITY, COUPLING_COULOMB
+    n = 512
+    if kind == "gravity":
+        st = ic.plummer_system(n, seed=12)
+        m = np.random.default_rng(2).uniform(0.1, 10, n)
+        m /= m.sum()
+        q, coupling, coup, dt = None, COUPLING_GRAVITY, 1.0, 1e-3
+    else:
+        st, q = ic.coulomb_box(n, seed=13)
+        m, coupling, coup, dt = None, COUPLING_COULOMB, 1.0, 5e-4
+    s = torch.from_numpy(st).cuda()
+    d = torch.empty_like(s)
+    tm = None if m is None else torch.from_numpy(m).cuda()
+    tq = None if q is None else torch.from_numpy(q).cuda()
+    stats = torch.zeros(16, dtype=torch.int64, device="cuda")
+    ctx.step_batched_(s, d, [n], dt=dt, coupling=coupling, coup=coup, mass=tm, charge=tq, stats=stats)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    ref = O.bia_step_coupled(st, coupling, coup, dt, mass=m, charge=q)
+    b = O.sensitivity(st, coup, dt, coupling=coupling, mass=m, charge=q)
+    ok, rp, rv = O.within_condition_bound(st, got, ref, coup, bounds=b)
+    sc = stats.cpu().numpy()
+    print(kind, "branches", sc[:4], "flags", sc[8:14], f"error/allowed {rp:.2f} {rv:.2f}")
+    assert ok
+    if kind == "coulomb":
+        assert sc[0] > 0 and sc[2] > 0  # attractive bound pairs and repulsive / unbound pairs
+
+
+def test_graph_driver_matches_eager_steps(ctx):
+    import torch
+    from bia_3d_gpu_amd import ic
+    sizes = [300, 129]
+    st = _cat([ic.plummer_system(n, seed=20 + k) for k, n in enumerate(sizes)])
+    eps = [-2.0 / n for n in sizes]
+    a = torch.from_numpy(st).cuda()
+    b = a.clone()
+    da, db = torch.empty_like(a), torch.empty_like(b)
+    for k in range(4):
+        ctx.run_steps_(a, da, sizes, 1, eps=eps, dt=1e-3, use_graph=True)   # replay of the captured graph
+        ctx.step_batched_(b, db, sizes, eps=eps, dt=1e-3)                    # eager work in between
+        b.add_(db)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b), f"replay {k}"
+    ctx.run_steps_(a, da, sizes, 3, eps=eps, dt=1e-3, use_graph=True)
+    ctx.run_steps_(b, db, sizes, 3, eps=eps, dt=1e-3, use_graph=False)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+def test_autotuner_keeps_results(ctx):
+    import torch
+    from bia_3d_gpu_amd import ic
+    n = 2048
+    st = ic.plummer_system(n, seed=31)
+    s = torch.from_numpy(st).cuda()
+    d0, d1 = torch.empty_like(s), torch.empty_like(s)
+    ctx.step_batched_(s, d0, [n], eps=[-2.0 / n], dt=1e-3)
+    ms, grid = ctx.autotune_(s, d1, [n], eps=[-2.0 / n], dt=1e-3)
+    ctx.step_batched_(s, d1, [n], eps=[-2.0 / n], dt=1e-3)
+    torch.cuda.synchronize()
+    ntiles = (n // 64) * (n // 64 + 1) // 2
+    print(f"autotuner: grid {grid} of {ntiles} tiles, {ms:.3f} ms")
+    assert 0 < grid <= ntiles and ms > 0
+    assert torch.equal(d0, d1)  # grid size changes scheduling only
+
+
+def test_batched_rkf(ctx):
+    import torch
+    from bia_3d_gpu_amd import ic
+    sizes = [64, 96]
+    states = [ic.uniform_ball_virial(n, seed=40 + k) for k, n in enumerate(sizes)]
+    eps = [-2.0 / n for n in sizes]
+    s = torch.from_numpy(_cat(states)).cuda()
+    d = torch.empty_like(s)
+    ctx.step_batched_(s, d, sizes, eps=eps, dt=1e-2, method=0)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    off = 0
+    for st, n, e in zip(states, sizes, eps):
+        ref = O.bia_step(st, e, 1e-2, method=0)
+        assert O.within_condition_bound(st, got[:, off:off + n], ref, e, 1e-2, method=0)[0]
+        off += n

@@ -21,13 +21,16 @@ def hs():
     dp = np.ctypeslib.ndpointer(np.float64, flags="C")
     ip = np.ctypeslib.ndpointer(np.int32, flags="C")
     up = np.ctypeslib.ndpointer(np.uint32, flags="C")
-    L.hs_pairs.argtypes = [C.c_int, C.c_int64, C.c_double, C.c_double, C.c_int, dp, dp, ip, ip, up]
+    L.hs_pairs.argtypes = [C.c_int, C.c_int64, C.c_double, C.c_double, C.c_int, dp, dp, ip, ip, up, C.c_int,
+                           C.c_void_p, C.c_int, C.c_int]
 
-    def run(method, rel, eps, dt, itr=18):
+    def run(method, rel, eps, dt, itr=18, mode=0, table=None):
         m = rel.shape[0]
         out = np.zeros((m, 6))
         br, it, fl = np.zeros(m, np.int32), np.zeros(m, np.int32), np.zeros(m, np.uint32)
-        L.hs_pairs(method, m, eps, dt, itr, np.ascontiguousarray(rel), out, br, it, fl)
+        tp, nv, nx = (None, 0, 0) if table is None else (table.ctypes.data_as(C.c_void_p), table.shape[1],
+                                                         table.shape[2])
+        L.hs_pairs(method, m, eps, dt, itr, np.ascontiguousarray(rel), out, br, it, fl, mode, tp, nx, nv)
         return out, br, it, fl
     return run
 
@@ -78,3 +81,39 @@ def test_rkf_pairs(hs):
     a, br, it, fl = hs(0, rel, -2 / 40, 1e-2)
     b, bb, fb = _oracle_pairs(0, rel, -2 / 40, 1e-2)
     assert np.all(br == 3) and np.abs(a - b).max() < 1e-15
+
+
+def test_guess_mode_safeguarded_newton(hs):
+    """BIA_MODE_GUESS (the reference guess + bracketed Newton with a rounding-aware stop): same
+    results as the reference iteration wherever that converges, and converged results on the
+    strongly deflected pairs where the reference's Newton leaves (-theta_0, theta_0)."""
+    from bia_3d_gpu_amd import ic
+    st = ic.plummer_with_binaries(1024, seed=8)
+    rel = _pairs(st)[::7]
+    eps, dt = -2 / 65536, 1e-3
+    a, br, it, fl = hs(1, rel, eps, dt, mode=2)
+    b, bb, fb = _oracle_pairs(1, rel, eps, dt)
+    hyp = br == 0
+    assert hyp.mean() > 0.99
+    assert (fl[hyp] & 128).sum() == 0  # no fallback needed
+    assert it[hyp].mean() < 3.0        # evaluations per pair (the reference spends ~5.5)
+    good = (fb & (O.F_ITR_CAP | O.F_STOCK_DEFAULT)) == 0
+    err = (np.abs(a - b) / np.abs(rel).max(0)).max(1)
+    assert np.quantile(err[good & hyp], 0.999) < 1e-11
+    # strongly deflected pairs (e ~ 1-3): the reference's Newton fails on some; the bracketed
+    # one converges (checked against the independent RKF45 integration of the same pair)
+    eps2 = -1.0
+    rng = np.random.default_rng(1)
+    rels = []
+    for _ in range(200):
+        r = 0.9
+        g = rng.uniform(1.5, 2.5) * np.sqrt(-eps2 / r)
+        ang = rng.uniform(0.1, np.pi - 0.1)
+        rels.append(np.concatenate([[r, 0, 0], g * np.array([np.cos(ang), np.sin(ang), 0])]))
+    rels = np.array(rels)
+    a, br, it, fl = hs(1, rels, eps2, 0.25, mode=2)
+    k, _, _, _ = hs(0, rels, eps2, 0.25)
+    assert np.all(br == 0) and (fl & 128).sum() == 0
+    assert np.abs(a - k).max() < 1e-10
+    b, bb, fb = _oracle_pairs(1, rels, eps2, 0.25)
+    assert ((fb & O.F_ITR_CAP) != 0).sum() > 0  # the reference iteration does fail on some of these

@@ -1,0 +1,114 @@
+"""The learned initialiser: FP64 CPU oracle of the network (pinned by finite differences), the GPU
+BF16 MFMA forward/backward against it, training, and what the trained network gains over the
+same safeguarded solver started from the reference's closed-form guess."""
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import mlp_oracle as M
+
+
+def _weights(seed=3):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(M.NPARAM)
+    w[0:192] = rng.uniform(-1, 1, 192)
+    w[192:256] = rng.uniform(-0.3, 0.3, 64)
+    w[256:4352] = rng.uniform(-0.2165, 0.2165, 4096)
+    w[4352:4416] = rng.uniform(-0.1, 0.1, 64)
+    w[4416:4480] = rng.uniform(-0.2165, 0.2165, 64)
+    w[4480] = 0.05
+    return w
+
+
+def _rows(n, seed=4):
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.uniform(-1, 1, n), rng.uniform(0, 1, n), rng.integers(0, 2, n).astype(float),
+                     rng.uniform(-0.5, 0.5, n)], axis=1)
+
+
+def test_oracle_gradient_by_finite_differences():
+    w = _weights()
+    rows = _rows(300)
+    loss, g = M.loss_and_grad(w, rows)
+    rng = np.random.default_rng(5)
+    idx = np.concatenate([rng.choice(192, 8), 192 + rng.choice(64, 4), 256 + rng.choice(4096, 12),
+                          4352 + rng.choice(64, 4), 4416 + rng.choice(64, 6), [4480]])
+    for p in idx:
+        h = 1e-6 * max(1.0, abs(w[p]))
+        wp, wm = w.copy(), w.copy()
+        wp[p] += h
+        wm[p] -= h
+        fd = (M.loss_and_grad(wp, rows)[0] - M.loss_and_grad(wm, rows)[0]) / (2 * h)
+        assert abs(fd - g[p]) <= 1e-6 * max(1e-3, abs(g[p])) + 1e-9, (p, fd, g[p])
+
+
+def test_gelu_derivative():
+    z = np.linspace(-6, 6, 2001)
+    fd = (M.gelu(z + 1e-6) - M.gelu(z - 1e-6)) / 2e-6
+    assert np.abs(fd - M.gelu_d(z)).max() < 1e-8
+
+
+# ------------------------------------------------------------------------------------------------
+# GPU
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_mlp_forward_and_gradient_match_oracle(ctx):
+    import torch
+    w = _weights().astype(np.float32)
+    ctx.mlp_set_weights(w)
+    rows = _rows(4096)
+    r = torch.from_numpy(rows.astype(np.float32)).cuda()
+    y = torch.empty(4096, dtype=torch.float32, device="cuda")
+    ctx.mlp_forward_(r, y)
+    g = torch.empty(M.NPARAM + 1, dtype=torch.float32, device="cuda")
+    ctx.mlp_grad_(r, g)
+    torch.cuda.synchronize()
+    yo = M.forward(w.astype(np.float64), rows)
+    ey = np.abs(y.cpu().numpy() - yo).max() / np.abs(yo).max()
+    lo, go = M.loss_and_grad(w.astype(np.float64), rows)
+    gg = g.cpu().numpy().astype(np.float64)
+    rel = np.linalg.norm(gg[:M.NPARAM] - go) / np.linalg.norm(go)
+    cos = gg[:M.NPARAM] @ go / (np.linalg.norm(gg[:M.NPARAM]) * np.linalg.norm(go))
+    print(f"forward max rel err {ey:.2e}; gradient rel L2 {rel:.2e}, cosine {cos:.6f}; loss {gg[-1]:.6e} vs {lo:.6e}")
+    # BF16 operands (8-bit mantissa), FP32 accumulation
+    assert ey < 2e-2
+    assert rel < 5e-2 and cos > 0.998
+    assert abs(gg[-1] / lo - 1) < 2e-2
+    # per-block gradient agreement (each layer separately)
+    for a, b in ((0, 192), (192, 256), (256, 4352), (4352, 4416), (4416, 4480)):
+        assert np.linalg.norm(gg[a:b] - go[a:b]) <= 8e-2 * np.linalg.norm(go[a:b]) + 1e-7, (a, b)
+
+
+@pytest.mark.gpu
+def test_training_and_learned_gain(ctx):
+    """Train on recorded samples, then step a Plummer-with-binaries system (another seed) in the
+    three modes: all agree with the oracle's reference step within the reference formula's own
+    rounding sensitivity, and the learned guess needs fewer residual evaluations per hyperbolic
+    pair than the same solver started from the reference's guess."""
+    import torch
+    from bia_3d_gpu_amd import ic, learn, MODE_PLAIN, MODE_GUESS, MODE_LEARN
+    raw = learn.record(ctx, learn.training_systems(11), per_system=200_000, seed=11)
+    rows = learn.balance_log_e(raw, 400_000, seed=11)
+    ctx.mlp_init(11)
+    curve = ctx.mlp_train_(rows, steps=600, batch=8192, lr=3e-3, seed=11)
+    ctx.mlp_build_table(learn.NX, learn.NV)
+    var = float(rows[:, 3].var())
+    print(f"training: loss {curve[0]:.3e} -> {curve[-20:].mean():.3e} (target variance {var:.3e})")
+    assert curve[-20:].mean() < 0.01 * var
+    n = 2048
+    st = ic.plummer_with_binaries(n, seed=777)
+    eps, dt = -2.0 / 65536, 1e-3  # the 65536-body coupling, so e spans the benchmark's range
+    ref = O.bia_step(st, eps, dt)
+    bounds = O.sensitivity(st, eps, dt)
+    res = {}
+    for name, mode in (("plain", MODE_PLAIN), ("guess", MODE_GUESS), ("learn", MODE_LEARN)):
+        d, s = ctx.step_host(st, eps, dt, mode=mode, with_stats=True)
+        ok, rp, rv = O.within_condition_bound(st, d, ref, eps, bounds=bounds)
+        per = s[4] / max(s[0], 1)
+        res[name] = per
+        print(f"{name}: evaluations per hyperbolic pair {per:.3f}; learned {s[14]}, fallback {s[15]}; "
+              f"error/allowed {rp:.2f} {rv:.2f}")
+        assert ok
+        if mode != MODE_PLAIN:
+            assert s[15] <= 1e-3 * s[0]
+    assert res["learn"] < res["guess"] < res["plain"]

@@ -296,3 +296,36 @@ def test_fixture_energy_drift_of_reference_bia(fixture1332):
     e1 = O.invariance(st + fixture1332["bia"], eps, "quad")
     rel = abs((e1[0] + e1[1]) - (e0[0] + e0[1])) / abs(e0[0] + e0[1])
     assert 1e-13 < rel < 1e-12  # 4.4e-13
+
+
+def test_sensitivity_bound_covers_the_reference_itself(fixture1332):
+    """The parity bound used for the GPU (10x the formulas' sensitivity to one rounding unit)
+    accepts the reference's own CUDA output against the glibc evaluation of the same formulas."""
+    st, eps, dt = fixture1332["state"], fixture1332["eps"], fixture1332["dt"]
+    ok, rp, rv = O.within_condition_bound(st, fixture1332["bia"], O.bia_step(st, eps, dt), eps, dt)
+    assert ok and rp > 0.05  # the bound is tight enough to be exercised by the reference itself
+
+
+def test_coupled_step_reduces_to_reference_for_equal_masses():
+    from bia_3d_gpu_amd import ic
+    st = ic.uniform_ball_virial(150, seed=3)
+    a = O.bia_step(st, -2 / 150, 1e-3)
+    b = O.bia_step_coupled(st, 1, 1.0, 1e-3, mass=np.full(150, 1 / 150))
+    assert np.array_equal(a, b)
+
+
+def test_coupled_step_conserves_momentum_and_centre_of_mass():
+    from bia_3d_gpu_amd import ic
+    n = 150
+    st = ic.uniform_ball_virial(n, seed=4)
+    m = np.random.default_rng(1).uniform(0.1, 10, n)
+    m /= m.sum()
+    d = O.bia_step_coupled(st, 1, 1.0, 1e-3, mass=m)
+    assert np.abs((m * d[3:]).sum(1)).max() < 1e-15 * np.abs(m * d[3:]).sum()
+    # sum m dx = sum m v dt (the pair terms cancel by the centre-of-mass split)
+    drift = (m * st[3:]).sum(1) * 1e-3
+    assert np.abs((m * d[:3]).sum(1) - drift).max() < 1e-15
+    # Coulomb, unit masses: charges +-1, momentum conserved, like charges repel
+    q = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    dc = O.bia_step_coupled(st, 2, 1e-4, 1e-3, charge=q)
+    assert np.abs(dc[3:].sum(1)).max() < 1e-15 * np.abs(dc[3:]).sum()
