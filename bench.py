@@ -43,7 +43,7 @@ def make_state(n: int, seed: int):
     return ic.plummer_with_binaries(n, seed)
 
 
-def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup):
+def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
     import torch
 
     from bia_3d_gpu_amd import flops_per_step
@@ -59,6 +59,10 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup):
     torch.cuda.synchronize()
     st = stats.cpu().numpy().astype(np.uint64)
     flops = flops_per_step(st)
+    tuned = None
+    if autotune:  # persistent grid size and register-occupancy variant, timed on this input
+        tms, grid, occ = ctx.autotune_(s, d, [s.shape[1]], eps=[eps], dt=dt, mode=mode, itr=itr)
+        tuned = {"grid": grid, "occupancy_variant": occ, "ms": tms}
     for _ in range(warmup):
         ctx.step_(s, d, eps, dt, mode=mode, itr=itr)
         s.add_(d)
@@ -77,7 +81,8 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup):
     wall = time.perf_counter() - t0
     ctx.set_profiling(False)
     ms = e0.elapsed_time(e1) / steps
-    return dict(ms_per_step=ms, wall_s=wall, kernel_ms=float(np.mean(kms)), stats=st.tolist(), flops=flops)
+    return dict(ms_per_step=ms, wall_s=wall, kernel_ms=float(np.mean(kms)), stats=st.tolist(), flops=flops,
+                tuned=tuned)
 
 
 def cpu_baseline(st_host, eps, dt, itr, budget_s: float):
@@ -116,6 +121,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learn", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-steps", type=int, default=3000)
     args = ap.parse_args()
@@ -137,10 +143,10 @@ def main():
     ctx = Context(0)
     res = {}
     for name, mode in (("plain", MODE_PLAIN), ("guess", MODE_GUESS)):
-        r = run_mode(ctx, st, eps, args.dt, args.itr, mode, args.steps, args.warmup)
+        r = run_mode(ctx, st, eps, args.dt, args.itr, mode, args.steps, args.warmup, not args.no_autotune)
         r["pairs_per_s"] = pairs / (r["ms_per_step"] / 1e3)
         res[name] = r
-        log(f"{name}: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms")
+        log(f"{name}: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms, tuned {r['tuned']}")
     train = None
     if not args.no_learn:
         t0 = time.perf_counter()
@@ -149,10 +155,10 @@ def main():
         train = {"seconds": time.perf_counter() - t0, "samples": int(rows.shape[0]), "steps": args.train_steps,
                  "loss_first": float(curve[0]), "loss_last": float(np.mean(curve[-20:]))}
         del rows
-        r = run_mode(ctx, st, eps, args.dt, args.itr, MODE_LEARN, args.steps, args.warmup)
+        r = run_mode(ctx, st, eps, args.dt, args.itr, MODE_LEARN, args.steps, args.warmup, not args.no_autotune)
         r["pairs_per_s"] = pairs / (r["ms_per_step"] / 1e3)
         res["learned"] = r
-        log(f"learned: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms")
+        log(f"learned: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms, tuned {r['tuned']}")
     plain = res["plain"]
     head = res.get("learned", plain)
     # roofline of the pair kernel: useful work = the plain reference algorithm's flops
@@ -172,7 +178,7 @@ def main():
                    "mode": "learned" if "learned" in res else "plain"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "modes": {k: {q: v[q] for q in ("pairs_per_s", "ms_per_step", "kernel_ms", "stats")} for k, v in res.items()},
+        "modes": {k: {q: v[q] for q in ("pairs_per_s", "ms_per_step", "kernel_ms", "stats", "tuned")} for k, v in res.items()},
         "speedup_vs_plain": {k: v["pairs_per_s"] / plain["pairs_per_s"] for k, v in res.items()},
         "training": train,
     }

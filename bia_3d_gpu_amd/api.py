@@ -229,7 +229,7 @@ class Context:
         check(load().bia_autotune_dev(self._h, method, mode, coupling, len(sz), sz, e, coup, dt, itr,
                                       state.data_ptr(), state.shape[1], ptr(mass), ptr(charge), delta.data_ptr(),
                                       C.byref(ms), C.byref(g)), "bia_autotune_dev", self._h)
-        return ms.value, g.value
+        return ms.value, g.value // 10, g.value % 10
 
     def run_steps_(self, state, delta, sizes, nsteps: int, eps=None, dt: float = 1e-3, method: int = METHOD_ALG,
                    mode: int = MODE_PLAIN, coupling: int = 0, coup: float = 1.0, mass=None, charge=None,

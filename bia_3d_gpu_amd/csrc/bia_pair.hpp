@@ -275,6 +275,7 @@ BIA_HDF double hyp_solve_safe(const HypState& S, double th, int kmax, int* evals
   if (!(th > lo && th < hi)) th = 0.5 * (lo + hi) + 0.5 * (S.th_gss - 0.5 * (lo + hi));
   *ok = false;
   int k = 0;
+  double prev = 0;  // size of the last Newton step taken (0 after a bisection)
   for (; k < kmax; k++) {
     double err, d;
     double th_new = hyp_newton(th, S.TE, S.P, &err, &d);
@@ -284,9 +285,14 @@ BIA_HDF double hyp_solve_safe(const HypState& S, double th, int kmax, int* evals
       return th;
     }
     const bool newton_ok = isfinite(err) && isfinite(d) && d != 0 && isfinite(th_new);
-    // converged: a Newton step this small leaves ~1e-16 theta_0 once applied (tested before the
-    // bracket, whose end may sit exactly at th when the step rounds to zero)
-    if (newton_ok && fabs(th_new - th) <= 1e-8 * S.P.th0) {
+    // converged when the error left after applying this step is below 1e-16 theta_0.  Newton's
+    // error contracts as e' = C e^2 and each step measures the error it removes, so after two
+    // Newton steps of sizes p then s the remaining error is about C s^2 = s^3 / p^2.  The first
+    // step has no such estimate and must itself be below 1e-10 theta_0.  (Tested before the
+    // bracket update, whose end may sit exactly at th when the step rounds to zero.)
+    const double step = fabs(th_new - th);
+    if (newton_ok && (k == 0 || prev == 0 ? step <= 1e-10 * S.P.th0
+                                          : step * step * step <= 1e-16 * S.P.th0 * prev * prev)) {
       *ok = true;
       *evals = k + 1;
       return th_new;
@@ -299,7 +305,9 @@ BIA_HDF double hyp_solve_safe(const HypState& S, double th, int kmax, int* evals
     } else {
       lo = th;
     }
-    th = (newton_ok && th_new > lo && th_new < hi) ? th_new : 0.5 * (lo + hi);
+    const bool take = newton_ok && th_new > lo && th_new < hi;
+    th = take ? th_new : 0.5 * (lo + hi);
+    prev = take ? step : 0;
   }
   *evals = k;
   return th;

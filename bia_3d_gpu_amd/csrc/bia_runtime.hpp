@@ -27,8 +27,6 @@ struct Plan {
   int nsys = 0, ntiles = 0, nchunks = 0, ntotal = 0;
   size_t part_doubles = 0;
   std::vector<SysDesc> hsys;  // host copy (eps updated per call)
-  int grid = 0;               // pair-kernel workgroups (0: one per tile); set by the autotuner
-  double tuned_ms = 0;
 };
 
 // grow-only device buffer; reused across calls so steady-state steps allocate nothing
@@ -74,7 +72,7 @@ struct bia_ctx {
   // batched plans: device tile list (int4 I, J, sys, 0), system descriptors, reduce chunks
   bia::Plan plan;
   bia::DevBuf plan_tiles, plan_sys, plan_chunks;
-  std::map<std::vector<int>, int> tuned_grid;  // autotuner cache: sizes -> grid
+  std::map<std::vector<int>, std::pair<int, int>> tuned_grid;  // autotuner cache: (method, mode, coupling, sizes) -> (grid, occ)
   // multi-step driver graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;

@@ -72,8 +72,10 @@ struct PairStats {
 
 enum Coupling : int { kScalar = 0, kGravity = 1, kCoulomb = 2 };
 
-template <int METHOD, int MODE, int COUP, bool STATS>
-__global__ __launch_bounds__(kWave) void k_tiles(int ntiles, int lds, const int4* __restrict__ tiles,
+// OCC: minimum waves per SIMD asked of the register allocator (1 = its own choice, 3 waves for the
+// ALG kernels; 4 = at most 128 VGPRs, a few spills); picked per shape by the autotuner
+template <int METHOD, int MODE, int COUP, bool STATS, int OCC>
+__global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const int4* __restrict__ tiles,
                                                  const SysDesc* __restrict__ sys, const double* __restrict__ st,
                                                  const double* __restrict__ mass, const double* __restrict__ charge,
                                                  double coup, double dt, int itr, double* __restrict__ part,
@@ -221,6 +223,20 @@ struct StepArgs {
   unsigned long long* d_stats;
 };
 
+// autotuner cache key: the kernel variant (method, mode, coupling) and the list of sizes
+static std::vector<int> tune_key(const StepArgs& a) {
+  std::vector<int> k = {a.method, a.mode, a.coupling};
+  k.insert(k.end(), a.sizes, a.sizes + a.nsys);
+  return k;
+}
+
+// (grid, occupancy variant) the autotuner chose for this kernel variant and sizes, else (0, 1):
+// one workgroup per tile and the compiler's own register allocation
+static std::pair<int, int> tuned_for(const bia_ctx* ctx, const StepArgs& a) {
+  auto it = ctx->tuned_grid.find(tune_key(a));
+  return it != ctx->tuned_grid.end() ? it->second : std::pair<int, int>(0, 1);
+}
+
 // build (or reuse) the plan for this list of sizes; upload system descriptors when eps changed
 static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
   Plan& P = ctx->plan;
@@ -254,8 +270,6 @@ static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
     BIA_TRY(ctx, ctx->plan_sys.ensure(Q.hsys.size() * sizeof(SysDesc)));
     BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_tiles.p, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
     BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_chunks.p, chunks.data(), chunks.size() * sizeof(int2), hipMemcpyHostToDevice));
-    auto it = ctx->tuned_grid.find(sizes);
-    Q.grid = (it != ctx->tuned_grid.end()) ? it->second : 0;
     P = Q;
     for (auto& d : P.hsys) d.eps = __builtin_nan("");  // force the descriptor upload below
   }
@@ -294,19 +308,27 @@ static int validate(bia_ctx* ctx, const StepArgs& a) {
 }
 
 // enqueue pair kernel + reduction on ctx->stream (no synchronisation, no allocation once planned)
-static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override) {
+static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int occ_override = 0) {
   const Plan& P = ctx->plan;
+  const auto tuned = tuned_for(ctx, a);
+  const int tuned_grid = tuned.first, tuned_occ = tuned.second;
+  const int occ = occ_override > 0 ? occ_override : tuned_occ;
   const LearnTable T{ctx->mlp_table.as<float>(), ctx->table_nx, ctx->table_nv};
-  int grid = grid_override > 0 ? grid_override : (P.grid > 0 ? P.grid : P.ntiles);
+  int grid = grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : P.ntiles);
   grid = std::min(grid, P.ntiles);
   const int4* tiles = ctx->plan_tiles.as<int4>();
   const SysDesc* sys = ctx->plan_sys.as<SysDesc>();
   double* part = ctx->part.as<double>();
   const bool st = a.d_stats != nullptr;
   BIA_TRY(ctx, prof_begin(ctx));
-#define L_(M, MO, CP, S)                                                                                         \
-  hipLaunchKernelGGL((k_tiles<M, MO, CP, S>), dim3(grid), dim3(kWave), 0, ctx->stream, P.ntiles, a.ld, tiles, sys, \
+#define L_OCC(M, MO, CP, S, O)                                                                                     \
+  hipLaunchKernelGGL((k_tiles<M, MO, CP, S, O>), dim3(grid), dim3(kWave), 0, ctx->stream, P.ntiles, a.ld, tiles, sys, \
                      a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, a.d_stats, T)
+#define L_(M, MO, CP, S)                         \
+  do {                                           \
+    if (M == 1 && occ == 4) L_OCC(M, MO, CP, S, 4); \
+    else L_OCC(M, MO, CP, S, 1);                 \
+  } while (0)
 #define L_COUP(M, MO, S)                                                  \
   do {                                                                    \
     if (a.coupling == BIA_COUPLING_SCALAR) L_(M, MO, kScalar, S);         \
@@ -328,6 +350,7 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override) {
   }
 #undef L_COUP
 #undef L_
+#undef L_OCC
   BIA_HIP_TRY(ctx, hipGetLastError());
   BIA_TRY(ctx, prof_end(ctx));
   hipLaunchKernelGGL(k_reduce, dim3(P.nchunks, 6), dim3(256), 0, ctx->stream, ctx->plan_chunks.as<int2>(), sys, a.ld,
@@ -361,30 +384,31 @@ static int autotune_locked(bia_ctx* ctx, const StepArgs& a, float* best_ms, int*
   const int saved_prof = ctx->profiling;
   ctx->profiling = 0;
   float best = 1e30f;
-  int bestg = ntiles;
+  int bestg = ntiles, besto = 1;
   int status = BIA_OK;
-  for (int g : cands) {
-    if ((status = launch_step(ctx, b, g)) != BIA_OK) break;  // warm-up
-    (void)hipEventRecord(e0, ctx->stream);
-    if ((status = launch_step(ctx, b, g)) != BIA_OK) break;
-    (void)hipEventRecord(e1, ctx->stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (ms < best) {
-      best = ms;
-      bestg = g;
+  const std::vector<int> occs = (a.method == BIA_METHOD_ALG) ? std::vector<int>{1, 4} : std::vector<int>{1};
+  for (int o : occs)
+    for (int g : cands) {
+      if ((status = launch_step(ctx, b, g, o)) != BIA_OK) break;  // warm-up
+      (void)hipEventRecord(e0, ctx->stream);
+      if ((status = launch_step(ctx, b, g, o)) != BIA_OK) break;
+      (void)hipEventRecord(e1, ctx->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        bestg = g;
+        besto = o;
+      }
     }
-  }
   ctx->profiling = saved_prof;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (status != BIA_OK) return status;
-  ctx->plan.grid = bestg;
-  ctx->plan.tuned_ms = best;
-  ctx->tuned_grid[ctx->plan.sizes] = bestg;
+  ctx->tuned_grid[tune_key(a)] = {bestg, besto};
   if (best_ms) *best_ms = best;
-  if (best_grid) *best_grid = bestg;
+  if (best_grid) *best_grid = bestg * 10 + besto;
   return BIA_OK;
 }
 
@@ -421,9 +445,10 @@ static int run_steps_locked(bia_ctx* ctx, const StepArgs& a, double* d_state_rw,
 
 static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, int nsteps) {
   const int ntot = ctx->plan.ntotal;
+  const auto tuned = tuned_for(ctx, a);
   std::vector<long long> key = {a.method, a.mode, a.coupling, a.itr, (long long)(size_t)d_state_rw,
                                 (long long)(size_t)a.d_delta, (long long)(size_t)a.d_mass,
-                                (long long)(size_t)a.d_charge, a.ld, (long long)(size_t)ctx->stream, ctx->plan.grid,
+                                (long long)(size_t)a.d_charge, a.ld, (long long)(size_t)ctx->stream, tuned.first, tuned.second,
                                 ctx->mlp_ready, (long long)(size_t)ctx->part.p,
                                 (long long)(size_t)ctx->mlp_table.p};
   long long dtb, cb;

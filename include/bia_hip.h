@@ -133,8 +133,9 @@ int bia_step_batched_dev(bia_ctx *ctx, int method, int mode, int coupling, int n
                          const double *eps, double coup, double dt, int itr, const double *d_state, int ld,
                          const double *d_mass, const double *d_charge, double *d_delta, unsigned long long *d_stats);
 
-/* Autotuner: times the pair kernel's persistent grid size on this input (one step per candidate,
- * written to d_delta) and keeps the fastest for this list of sizes in the context. */
+/* Autotuner: times the pair kernel's persistent grid size and register-occupancy variant on this
+ * input (one step per candidate, written to d_delta) and keeps the fastest for this list of sizes
+ * in the context.  *best_grid receives grid * 10 + occupancy variant (1 default, 4). */
 int bia_autotune_dev(bia_ctx *ctx, int method, int mode, int coupling, int nsys, const int *sizes, const double *eps,
                      double coup, double dt, int itr, const double *d_state, int ld, const double *d_mass,
                      const double *d_charge, double *d_delta, float *best_ms, int *best_grid);
