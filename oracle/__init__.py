@@ -53,6 +53,7 @@ def lib():
         L.orc_dim_func.argtypes = [C.c_int, C.c_double, _dp, _dp]
         L.orc_bia_rows_coupled.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp,
                                            C.c_void_p, C.c_void_p, C.c_int, C.c_int, _dp]
+        L.orc_conic_indeterminate.argtypes = [C.c_double, _dp]
         L.orc_sensitivity.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, C.c_void_p,
                                       C.c_void_p, C.c_int, _dp, _dp]
         _lib = L
@@ -230,7 +231,9 @@ def sensitivity(state_soa: np.ndarray, eps: float, dt: float, method: int = 1, i
     half-deltas under random 1-ulp perturbations of the pair's six relative-state inputs.
 
     Two correct evaluations of the reference formulas whose libm results differ by one ulp land
-    about this far apart.  It is large where the reference's formulation is ill-conditioned:
+    about this far apart.  A pair whose conic type is itself decided by rounding (classifier
+    value W below 64 rounding units of its terms, orc_conic_indeterminate) has no reproducible
+    answer; its two particles get an infinite bound (``indeterminate`` counts them).  It is large where the reference's formulation is ill-conditioned:
     acos(q) with q near 1 (pairs starting near periapsis, bia/find_dXYUV_kernel.cu:124) and
     near-parabolic hyperbolae (W = g^2 + 2 eps/r cancels, :98-107)."""
     st = np.ascontiguousarray(state_soa, dtype=np.float64).reshape(6, -1)
@@ -241,6 +244,15 @@ def sensitivity(state_soa: np.ndarray, eps: float, dt: float, method: int = 1, i
     lib().orc_sensitivity(method, n, coupling, eps, dt, itr, st, None if m is None else m.ctypes.data_as(C.c_void_p),
                           None if q is None else q.ctypes.data_as(C.c_void_p), samples, bp, bv)
     return bp, bv
+
+
+def indeterminate(bounds) -> int:
+    """Number of particles excluded by sensitivity(): those with a pair of indeterminate type."""
+    return int(np.count_nonzero(~np.isfinite(bounds[0])))
+
+
+def conic_indeterminate(eps: float, rel) -> bool:
+    return bool(lib().orc_conic_indeterminate(eps, np.ascontiguousarray(rel, dtype=np.float64)))
 
 
 def within_condition_bound(state_soa, d, ref, eps, dt: float = None, factor: float = 10.0, floor: float = 1e-13,

@@ -66,6 +66,7 @@ void orc_bia_rows_coupled(int method, int n, int coupling, double coup, double d
                           const double *mass, const double *charge, int i0, int i1, double *delta);
 
 /* per-particle sums of the pair formulas' sensitivity to one rounding unit of their inputs */
+int orc_conic_indeterminate(double eps, const double rel[6]);
 void orc_sensitivity(int method, int n, int coupling, double eps, double dt, int itr, const double *st,
                      const double *mass, const double *charge, int samples, double *bpos, double *bvel);
 

@@ -68,8 +68,11 @@ def test_coupled_step_vs_oracle(ctx, kind):
     b = O.sensitivity(st, coup, dt, coupling=coupling, mass=m, charge=q)
     ok, rp, rv = O.within_condition_bound(st, got, ref, coup, bounds=b)
     sc = stats.cpu().numpy()
-    print(kind, "branches", sc[:4], "flags", sc[8:14], f"error/allowed {rp:.2f} {rv:.2f}")
+    skipped = O.indeterminate(b)
+    print(kind, "branches", sc[:4], "flags", sc[8:14], f"error/allowed {rp:.2f} {rv:.2f}",
+          f"particles with a pair of indeterminate conic type: {skipped}")
     assert ok
+    assert skipped <= n // 100  # near-parabolic pairs decided by rounding (W ~ 1 ulp) are rare
     if kind == "coulomb":
         assert sc[0] > 0 and sc[2] > 0  # attractive bound pairs and repulsive / unbound pairs
 
