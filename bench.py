@@ -5,17 +5,20 @@ G = 1, m = 1/N) with 20 % of the particles in hard binaries (a in [1e-3, 1e-2], 
 dt = 1e-3, eps = -2Gm, seed 65536.  One "step" = one BIA step of every unordered pair
 (N(N-1)/2 pair interactions) followed by the state update, state resident in HBM.
 
-Modes measured on the same input: plain (the reference's guess and iteration control), guess (the
-reference's closed-form guess + bracketed Newton with a rounding-aware stop, no network) and
-learned (the network's guess + the same bracketed Newton).  The network is trained on the GPU
-first, from samples the solver records on other systems (training time reported separately).
+Modes measured on the same input: plain (the reference's conic solvers, guesses and iteration
+control), guess (universal-variable Kepler drift, every pair from its series start, no network)
+and learned (the same drift; pairs that are not weak over dt start from the network's table).
+The network is trained on the GPU first, from samples the solver records on other systems
+(training time reported separately).
 
 Printed: one JSON line (rank 0).  ``value`` is the pair-interactions/s of the learned mode
-(plain if training is skipped with --no-learn).  ``roofline`` prices the
-dominant kernel (the pair kernel) with the algorithmic FP64 flop count of the reference
-algorithm (stats counters of an untimed step on the same input) over its average duration,
-measured with HIP events recorded on the kernel's own stream.  ``cpu_baseline`` times the
-reference's own Fortran CPU pair loop (oracle/_ref, single thread) on a bounded row sample.
+(guess if training is skipped with --no-learn).  ``roofline`` prices the dominant kernel (the
+pair kernel of that mode) with the algorithmic FP64 flop count of its own algorithm (stats
+counters of an untimed step on the same input; bia_3d_gpu_amd.flops_per_step) over its average
+duration, measured with HIP events recorded on the kernel's own stream.  ``energy`` gives
+|dE/E0| over the timed steps of each mode from the library's double-double invariants.
+``cpu_baseline`` times the reference's own Fortran CPU pair loop (oracle/_ref, single thread)
+on a bounded row sample.
 """
 from __future__ import annotations
 
@@ -58,7 +61,7 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
     ctx.step_(s, d, eps, dt, mode=mode, itr=itr, stats=stats)
     torch.cuda.synchronize()
     st = stats.cpu().numpy().astype(np.uint64)
-    flops = flops_per_step(st)
+    flops = flops_per_step(st, mode)
     tuned = None
     if autotune:  # persistent grid size and register-occupancy variant, timed on this input
         tms, grid, occ = ctx.autotune_(s, d, [s.shape[1]], eps=[eps], dt=dt, mode=mode, itr=itr)
@@ -66,6 +69,8 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
     for _ in range(warmup):
         ctx.step_(s, d, eps, dt, mode=mode, itr=itr)
         s.add_(d)
+    inv0 = torch.zeros(11, dtype=torch.float64, device=dev)
+    ctx.invariants_(s, inv0, eps)
     ctx.set_profiling(True)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -81,8 +86,13 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
     wall = time.perf_counter() - t0
     ctx.set_profiling(False)
     ms = e0.elapsed_time(e1) / steps
-    return dict(ms_per_step=ms, wall_s=wall, kernel_ms=float(np.mean(kms)), stats=st.tolist(), flops=flops,
-                tuned=tuned)
+    inv1 = torch.zeros(11, dtype=torch.float64, device=dev)
+    ctx.invariants_(s, inv1, eps)
+    a, b = inv0.cpu().numpy(), inv1.cpu().numpy()
+    e_0, e_1 = a[0] + a[1], b[0] + b[1]  # KE + PE = 2E/m
+    kms = float(np.mean(kms))
+    return dict(ms_per_step=ms, wall_s=wall, kernel_ms=kms, stats=st.tolist(), flops=flops, tuned=tuned,
+                tflops=flops / (kms / 1e3) / 1e12, dE_rel=abs(e_1 - e_0) / abs(e_0), dE_steps=steps)
 
 
 def cpu_baseline(st_host, eps, dt, itr, budget_s: float):
@@ -146,7 +156,8 @@ def main():
         r = run_mode(ctx, st, eps, args.dt, args.itr, mode, args.steps, args.warmup, not args.no_autotune)
         r["pairs_per_s"] = pairs / (r["ms_per_step"] / 1e3)
         res[name] = r
-        log(f"{name}: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms, tuned {r['tuned']}")
+        log(f"{name}: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms ({r['tflops']:.1f} TFLOP/s), "
+            f"|dE/E0| {r['dE_rel']:.2e} over {r['dE_steps']} steps, tuned {r['tuned']}")
     train = None
     if not args.no_learn:
         t0 = time.perf_counter()
@@ -158,14 +169,16 @@ def main():
         r = run_mode(ctx, st, eps, args.dt, args.itr, MODE_LEARN, args.steps, args.warmup, not args.no_autotune)
         r["pairs_per_s"] = pairs / (r["ms_per_step"] / 1e3)
         res["learned"] = r
-        log(f"learned: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms, tuned {r['tuned']}")
+        log(f"learned: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms ({r['tflops']:.1f} TFLOP/s), "
+            f"|dE/E0| {r['dE_rel']:.2e}, tuned {r['tuned']}")
     plain = res["plain"]
-    head = res.get("learned", plain)
-    # roofline of the pair kernel: useful work = the plain reference algorithm's flops
-    achieved = plain["flops"] / (head["kernel_ms"] / 1e3) / 1e12
+    head_name = "learned" if "learned" in res else "guess"
+    head = res[head_name]
+    # roofline of the dominant kernel (the pair kernel of the reported mode), its own algorithm's flops
+    achieved = head["tflops"]
     roof = {"bound": "mfma", "pipe": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-            "flops_per_launch": plain["flops"], "kernel_ms": head["kernel_ms"]}
+            "flops_per_launch": head["flops"], "kernel_ms": head["kernel_ms"], "kernel": f"pair kernel, {head_name} mode"}
     cpu = None if args.no_cpu else cpu_baseline(st, eps, args.dt, args.itr, args.cpu_budget)
     out = {
         "metric": METRIC, "value": head["pairs_per_s"], "unit": "pair-interactions/s", "n_gpus": 1,
@@ -175,10 +188,11 @@ def main():
         "config": {"workload": f"Plummer N={n} with 20% hard binaries, G=1, m=1/N, dt={args.dt}, one BIA-ALG "
                                f"step per timed step, seed {args.seed}",
                    "n": n, "dt": args.dt, "eps": eps, "itr": args.itr,
-                   "mode": "learned" if "learned" in res else "plain"},
+                   "mode": head_name},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "modes": {k: {q: v[q] for q in ("pairs_per_s", "ms_per_step", "kernel_ms", "stats", "tuned")} for k, v in res.items()},
+        "modes": {k: {q: v[q] for q in ("pairs_per_s", "ms_per_step", "kernel_ms", "tflops", "dE_rel", "dE_steps",
+                                         "stats", "tuned")} for k, v in res.items()},
         "speedup_vs_plain": {k: v["pairs_per_s"] / plain["pairs_per_s"] for k, v in res.items()},
         "training": train,
     }

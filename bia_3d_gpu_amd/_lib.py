@@ -63,7 +63,7 @@ SIGNATURES = {
     "bia_mlp_forward_dev": (C.c_int, [_vp, _vp, C.c_int, _vp]),
     "bia_mlp_grad_dev": (C.c_int, [_vp, _vp, C.c_int, _vp]),
     "bia_mlp_train_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _vp]),
-    "bia_mlp_build_table": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "bia_mlp_build_table": (C.c_int, [_vp, C.c_int]),
     "bia_record_samples_dev": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int,
                                          C.c_ulonglong, _vp]),
     "bia_dim_rkf45_host": (C.c_int, [_vp, C.c_int, C.c_double, _dptr, C.c_double, _dp, _ip, C.c_double,

@@ -100,14 +100,23 @@ def reference_step(state_soa: np.ndarray, eps: float, dt: float, method: int = M
 # ------------------------------------------------------------------------------------------------
 # algorithmic work per step (flop convention: + - x / sqrt = 1, transcendental = 20)
 # ------------------------------------------------------------------------------------------------
-def flops_per_step(stats) -> float:
-    """Algorithmic FP64 flops of one step from its BIA_STATS counters.
+def flops_per_step(stats, mode: int = MODE_PLAIN) -> float:
+    """Algorithmic FP64 flops of one step from its BIA_STATS counters (+ - x / sqrt = 1 flop,
+    each libm transcendental = 20).
 
-    Per pair, with k = Newton corrections after the two set-up evaluations (hyperbola) or
-    Newton corrections (ellipse, parabola), accepted steps s (RKF), counted from the reference
-    GPU source: hyperbola 635 + 122 k, ellipse 524 + 47 k, parabola 472 + 76 k, RKF 80 + 790 s.
+    Plain mode (the reference algorithm), per pair, with k = Newton corrections after the two
+    set-up evaluations (hyperbola) or Newton corrections (ellipse, parabola) and s accepted RKF
+    steps, counted from the reference GPU source: hyperbola 635 + 122 k, ellipse 524 + 47 k,
+    parabola 472 + 76 k, RKF 80 + 790 s.
+
+    Guess and learned modes (universal-variable Kepler drift, csrc/bia_pair.hpp kepler_pair),
+    counted from that source: 126 per pair (relative state 6, set-up 23, series start 6, output
+    with its Stumpff evaluation 79, accumulation 12) + 75 per evaluation of the time equation,
+    + 120 per learned start (features, three asinh, trilinear lookup, exp).
     """
     s = np.asarray(stats, dtype=np.float64)
+    if mode != MODE_PLAIN:
+        return 126 * s[0:4].sum() + 75 * s[4:8].sum() + 120 * s[14]
     n_h, n_p, n_e, n_r = s[0:4]
     it_h, it_p, it_e, it_r = s[4:8]
     k_h = it_h - 2 * n_h
@@ -273,8 +282,8 @@ class Context:
                                        curve.ctypes.data_as(C.c_void_p)), "bia_mlp_train_dev", self._h)
         return curve
 
-    def mlp_build_table(self, nx: int = 1024, nv: int = 256):
-        check(load().bia_mlp_build_table(self._h, nx, nv), "bia_mlp_build_table", self._h)
+    def mlp_build_table(self, n: int = 64):
+        check(load().bia_mlp_build_table(self._h, n), "bia_mlp_build_table", self._h)
 
     def record_samples_(self, state, eps: float, dt: float, nsamples: int, seed: int, out):
         """out: float32 CUDA tensor [nsamples, 4]."""

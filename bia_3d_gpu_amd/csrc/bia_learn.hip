@@ -2,8 +2,10 @@
 // the solver records, with hand-written BF16 MFMA forward / backward kernels (FP32
 // accumulation), Adam, the sample recorder, and the tabulation the pair kernel reads.
 //
-// Network input (one hyperbolic pair): x1 = theta_gss / theta_0, v = e^(-1/4), branch flag
-// +1 attractive / -1 repulsive.  Output: t = (theta / theta_0 - x1) * e (see bia_pair.hpp).
+// Network input (one pair that is not weak over dt): the squashed dimensionless numbers
+// u_k = asinh(a_k / 1e-2) / asinh(1e4) of a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3,
+// a3 = beta dt^2 / r0^2, which fix the universal Kepler equation up to scale.  Output:
+// log(s / s0), the converged universal anomaly over s0 = dt / r0 (see kepler_pair in bia_pair.hpp).
 //
 // Parameters (fp32 master copy, 4481 floats): W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3.
 // Matrix products run on v_mfma_f32_32x32x16_bf16 with BF16 operands: one wave computes a
@@ -131,12 +133,12 @@ __global__ __launch_bounds__(64) void k_mlp(const float* __restrict__ w, Batch B
   for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
     const int i = tile * 64 + l;
     const bool ok = i < B.batch;
-    float x1 = 0, v = 0, br = 0, tt = 0;
+    float x1 = 0, v = 0, br = 0, tt = 0;  // the three features u1 u2 u3 and the target
     if (ok) {
       const float* s = B.samples + (size_t)batch_row(B, i) * 4;
       x1 = s[0];
       v = s[1];
-      br = 1.f - 2.f * s[2];
+      br = s[2];
       tt = s[3];
     }
     sXf[l * 4 + 0] = x1;
@@ -265,54 +267,55 @@ __global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ pa
   w[p] -= lr * (a / bc1) / (sqrtf(b / bc2) + 1e-8f);
 }
 
-// grid of the table: rows [x1, v, b, 0]
-__global__ void k_table_grid(int nx, int nv, float* __restrict__ rows) {
+// grid of the table: rows [u1, u2, u3, 0] on a regular n^3 grid of [-1, 1]^3, u1 fastest
+__global__ void k_table_grid(int n, float* __restrict__ rows) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 2 * nx * nv) return;
-  const int ix = q % nx, iv = (q / nx) % nv, b = q / (nx * nv);
+  if (q >= n * n * n) return;
+  const int i1 = q % n, i2 = (q / n) % n, i3 = q / (n * n);
   float* r = rows + (size_t)q * 4;
-  r[0] = -1.f + 2.f * ix / (nx - 1);
-  r[1] = (float)iv / (nv - 1);
-  r[2] = (float)b;
+  r[0] = -1.f + 2.f * i1 / (n - 1);
+  r[1] = -1.f + 2.f * i2 / (n - 1);
+  r[2] = -1.f + 2.f * i3 / (n - 1);
   r[3] = 0.f;
 }
 
-// Recorder: random pairs (i, j) of a system; hyperbolic pairs are solved to convergence from the
-// reference guess with the safeguarded Newton and written as [x1, v, b, t]; other pairs get
-// x1 = NaN.
+// Recorder: each thread draws random pairs (i, j) of a system until it finds one that is not
+// weak over dt (some |a_k| >= 1e-2; at most kDraws draws), solves it to convergence from the
+// series start and writes the row [u1, u2, u3, log(s / s0)]; NaN in u1 when none was found or
+// the solve failed.
+constexpr int kDraws = 4096;
+
 __global__ __launch_bounds__(64) void k_record(int n, int ld, double eps, double dt, const double* __restrict__ st,
                                                int nsamples, unsigned long long seed, float* __restrict__ out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsamples) return;
-  const unsigned long long h = mix64(seed ^ (0xd1b54a32d192ed03ULL * (unsigned long long)(s + 1)));
-  const int i = (int)(h % (unsigned long long)n);
-  int j = (int)((h >> 32) % (unsigned long long)(n - 1));
-  if (j >= i) j++;
-  double rel[6];
-  for (int c = 0; c < 6; c++) rel[c] = st[(size_t)c * ld + i] - st[(size_t)c * ld + j];
-  double exi[3], eeta[3], yz[4];
-  frame(rel, rel + 3, exi, eeta, yz);
   float* o = out + (size_t)s * 4;
-  const double W = sqrt(yz[0] * yz[0] + yz[1] * yz[1]) * (yz[2] * yz[2] + yz[3] * yz[3]) + 2 * eps;
   o[0] = __builtin_nanf("");
   o[1] = o[2] = o[3] = 0.f;
-  if (!(W > 0)) return;
-  HypState S;
-  unsigned fl = 0;
-  hyp_setup(eps, dt, yz, S, &fl);
-  int ev = 0;
-  bool okc = false;
-  const double th = hyp_solve_safe(S, S.th_gss, 40, &ev, &okc);
-  if (!okc) return;
-  double x1, v;
-  int b;
-  hyp_features(S, &x1, &v, &b);
-  const double t = (th / S.P.th0 - x1) / fabs(S.P.cs0);
-  if (!isfinite(t) || !isfinite(x1)) return;
-  o[0] = (float)x1;
-  o[1] = (float)v;
-  o[2] = (float)b;
-  o[3] = (float)t;
+  for (int draw = 0; draw < kDraws; draw++) {
+    const unsigned long long h =
+        mix64(seed ^ (0xd1b54a32d192ed03ULL * (unsigned long long)(s + 1)) ^ (0x9e3779b97f4a7c15ULL * (draw + 1)));
+    const int i = (int)(h % (unsigned long long)n);
+    int j = (int)((h >> 32) % (unsigned long long)(n - 1));
+    if (j >= i) j++;
+    double rel[6];
+    for (int c = 0; c < 6; c++) rel[c] = st[(size_t)c * ld + i] - st[(size_t)c * ld + j];
+    KepPair K;
+    kep_setup(eps, dt, rel, K);
+    double a[3];
+    kep_features(K, a);
+    if (!(fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))) >= kWeak) || !(K.tau > 0)) continue;
+    int ev = 0;
+    bool okc = false;
+    const double sv = kep_solve(K, kep_series_start(K), 200, &ev, &okc);
+    const double t = log(sv / (K.tau * K.ir0));
+    if (!okc || !isfinite(t)) return;
+    o[0] = (float)feat_squash(a[0]);
+    o[1] = (float)feat_squash(a[1]);
+    o[2] = (float)feat_squash(a[2]);
+    o[3] = (float)t;
+    return;
+  }
 }
 
 static int mlp_bufs(bia_ctx* ctx) {
@@ -441,19 +444,18 @@ int bia_mlp_train_dev(bia_ctx* ctx, const float* d_rows, int nrows, int steps, i
   return BIA_OK;
 }
 
-int bia_mlp_build_table(bia_ctx* ctx, int nx, int nv) {
+int bia_mlp_build_table(bia_ctx* ctx, int n) {
   BIA_LOCK(ctx);
-  if (!ctx->mlp_init || nx < 2 || nv < 2 || (size_t)nx * nv > (1u << 24)) return BIA_ERR_ARG;
-  const int rows = 2 * nx * nv;
+  if (!ctx->mlp_init || n < 2 || n > 256) return BIA_ERR_ARG;
+  const int rows = n * n * n;
   BIA_TRY(ctx, ctx->mlp_table.ensure((size_t)rows * sizeof(float) * 5));
   float* tab = ctx->mlp_table.as<float>();
   float* grid = tab + rows;  // rows x 4 scratch after the table
-  hipLaunchKernelGGL(k_table_grid, dim3((rows + 255) / 256), dim3(256), 0, ctx->stream, nx, nv, grid);
+  hipLaunchKernelGGL(k_table_grid, dim3((rows + 255) / 256), dim3(256), 0, ctx->stream, n, grid);
   BIA_HIP_TRY(ctx, hipGetLastError());
   BIA_TRY(ctx, mlp_forward_locked(ctx, grid, rows, tab));
   BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  ctx->table_nx = nx;
-  ctx->table_nv = nv;
+  ctx->table_n = n;
   ctx->mlp_ready = 1;
   return BIA_OK;
 }

@@ -11,7 +11,9 @@
 //   RKF45      adaptive Fehlberg 4(5) with step doubling on (xi,eta,g)    (bia/bia-rkf.cu:1-103)
 //
 // The reference iteration control (best iterate, exact-repeat exit, ITR cap, tolerance
-// 2.22e-16 on the dimensionless time residual) is kept in kPlain mode.  The hyperbola's
+// 2.22e-16 on the dimensionless time residual) is kept in kPlain mode.  The guess and learned
+// modes solve the same two-body problem in Sundman's universal anomaly instead (one formula
+// set for every conic, no frame, no trigonometry; see kepler_pair).  The hyperbola's
 // Newton residual is evaluated from one sincos of the half angle:
 //   sin((t0 +- t)/2) = sin(t0/2)cos(t/2) +- cos(t0/2)sin(t/2), cos t = (c-s)(c+s), sin t = 2sc
 // which is the reference's residual up to rounding, at a third of the transcendental cost.
@@ -55,15 +57,16 @@ enum Flag : unsigned {
 };
 
 // kPlain: the reference's guess selection and iteration control.
-// kLearn: learned guess (network tabulated over its input features) + safeguarded Newton.
-// kGuess: the reference's closed-form guess + the same safeguarded Newton (no network).
+// kPlain: the reference's algorithm and iteration control.
+// kGuess: universal-variable Kepler drift from its series start (no network).
+// kLearn: the same drift; pairs that are not weak over dt start from the learned table.
 enum Mode : int { kPlain = 0, kLearn = 1, kGuess = 2 };
 
 // Learned-initialiser table: the network's output t(x1, v) for the two hyperbolic branches on a
 // regular grid, x1 in [-1, 1] (nx points), v in [0, 1] (nv points), bilinear interpolation.
 struct LearnTable {
-  const float* t;  // [2][nv][nx]
-  int nx, nv;
+  const float* t;  // [n][n][n]: log(s / s0) over the squashed features (u1 fastest)
+  int n;
 };
 
 // Residual history used by the reference's exact-repeat exit.  On the device it lives in
@@ -241,76 +244,230 @@ BIA_HDF double hyp_solve_plain(const HypState& S, int itr, Hist hist, int* iters
 }
 
 // ------------------------------------------------------------------------------------
-// learned initial guess and safeguarded Newton (no reference counterpart)
+// Universal-variable Kepler drift (guess and learned modes; no reference counterpart)
 //
-// Features of a hyperbolic pair: x1 = theta_gss / theta_0 (the reference's closed-form guess
-// (:172), a function of the dimensionless mean anomaly TE and e), v = e^(-1/4) = |cs0|^(1/4),
-// branch b = 0 attractive (cs0 < 0) / 1 repulsive.  Target t = (theta/theta_0 - x1) * e, the
-// guess's scaled residual, O(1) and smooth; theta_L = theta_0 (x1 + t / e).
+// The same two-body problem r'' = eps r / |r|^3 over dt, solved in Sundman's universal
+// anomaly s (dt = r ds) with Stumpff functions, which covers ellipse, parabola and hyperbola,
+// attractive and repulsive, in one branch-free formula set and works in 3D directly (no
+// orbital-plane frame):
+//   mu = -eps, beta = 2 mu / r0 - v0^2 (= -W), eta = r0 . v0, zeta = mu - beta r0,
+//   G_k(s) = s^k c_k(beta s^2),
+//   t(s) = r0 s + eta G2 + zeta G3,  r(s) = dt/ds = r0 + eta G1 + zeta G2,
+//   Lagrange coefficients f - 1 = -mu G2 / r0, g - t = -mu G3, df = -mu G1 / (r0 r),
+//   dg - 1 = -mu G2 / r,
+// so the pair's departure from straight-line motion, (f - 1) r0 + (g - dt) v0, is formed from
+// small quantities without the cancellation of (r1 - r0) - v0 dt.  The anomaly solves
+// t(s) = dt by Halley steps kept inside a bracket (t increases with s); the error left by a
+// Halley step of size d is about |C^2 - t3 / (6 t1)| d^3 with C = t2 / (2 t1) (t1, t2, t3 the
+// derivatives of t), and the iteration stops when that estimate is below 1e-16 |s|.  Pairs that
+// are weak over dt (the dimensionless numbers a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3,
+// a3 = beta dt^2 / r0^2 all small) start from the second-order series s = dt/r0 (1 - a1/2) and
+// converge in one evaluation; the others start from it too (guess mode) or from the learned
+// table over (a1, a2, a3) (learned mode).
 // ------------------------------------------------------------------------------------
-BIA_HDF void hyp_features(const HypState& S, double* x1, double* v, int* b) {
-  *x1 = S.th_gss / S.P.th0;
-  *v = sqrt(sqrt(fabs(S.P.cs0)));
-  *b = (S.P.cs0 < 0) ? 0 : 1;
+struct Stumpff {
+  double c0, c1, c2, c3;
+};
+
+// c_k(y) = sum_j (-y)^j / (2j + k)!: series at y / 4^m with |y / 4^m| <= 0.1 (8 terms, truncation
+// below 1e-21), then m doublings c0(4y) = 2 c0^2 - 1, c1(4y) = c0 c1, c2(4y) = c1^2 / 2,
+// c3(4y) = (c2 + c0 c3) / 4.
+BIA_HDF Stumpff stumpff(double y) {
+  int m = 0;
+  while (fabs(y) > 0.1 && m < 60) {
+    y *= 0.25;
+    m++;
+  }
+  constexpr double i2 = 1.0 / 2, i3 = 1.0 / 6, i4 = 1.0 / 24, i5 = 1.0 / 120, i6 = 1.0 / 720, i7 = 1.0 / 5040,
+                   i8 = 1.0 / 40320, i9 = 1.0 / 362880, i10 = 1.0 / 3628800, i11 = 1.0 / 39916800,
+                   i12 = 1.0 / 479001600, i13 = 1.0 / 6227020800.0, i14 = 1.0 / 87178291200.0,
+                   i15 = 1.0 / 1307674368000.0, i16 = 1.0 / 20922789888000.0, i17 = 1.0 / 355687428096000.0;
+  const double z = -y;
+  double c2 = i16;
+  c2 = c2 * z + i14;
+  c2 = c2 * z + i12;
+  c2 = c2 * z + i10;
+  c2 = c2 * z + i8;
+  c2 = c2 * z + i6;
+  c2 = c2 * z + i4;
+  c2 = c2 * z + i2;
+  double c3 = i17;
+  c3 = c3 * z + i15;
+  c3 = c3 * z + i13;
+  c3 = c3 * z + i11;
+  c3 = c3 * z + i9;
+  c3 = c3 * z + i7;
+  c3 = c3 * z + i5;
+  c3 = c3 * z + i3;
+  double c1 = 1 - y * c3, c0 = 1 - y * c2;
+  for (; m > 0; m--) {
+    const double n3 = 0.25 * (c2 + c0 * c3);
+    const double n2 = 0.5 * c1 * c1;
+    const double n1 = c0 * c1;
+    const double n0 = 2 * c0 * c0 - 1;
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+    c3 = n3;
+  }
+  return {c0, c1, c2, c3};
 }
 
-BIA_HDF double table_lookup(const LearnTable& T, int b, double x1, double v) {
-  double fx = (x1 + 1) * 0.5 * (T.nx - 1);
-  double fv = v * (T.nv - 1);
-  fx = fmin(fmax(fx, 0.0), T.nx - 1.000001);
-  fv = fmin(fmax(fv, 0.0), T.nv - 1.000001);
-  const int ix = (int)fx, iv = (int)fv;
-  const double ax = fx - ix, av = fv - iv;
-  const float* p = T.t + ((size_t)b * T.nv + iv) * T.nx + ix;
-  const double t00 = p[0], t01 = p[1], t10 = p[T.nx], t11 = p[T.nx + 1];
-  return (1 - av) * ((1 - ax) * t00 + ax * t01) + av * ((1 - ax) * t10 + ax * t11);
+// pair quantities shared by the start, the iteration and the output
+struct KepPair {
+  double mu, r0, ir0, eta, beta, zeta, tau, dt;
+};
+
+BIA_HDF void kep_setup(double eps, double dt, const double rel[6], KepPair& K) {
+  const double r2 = rel[0] * rel[0] + rel[1] * rel[1] + rel[2] * rel[2];
+  const double v2 = rel[3] * rel[3] + rel[4] * rel[4] + rel[5] * rel[5];
+  K.mu = -eps;
+  K.r0 = sqrt(r2);
+  K.ir0 = 1 / K.r0;
+  K.eta = rel[0] * rel[3] + rel[1] * rel[4] + rel[2] * rel[5];
+  K.beta = 2 * K.mu * K.ir0 - v2;
+  K.zeta = K.mu - K.beta * K.r0;
+  K.dt = dt;
+  K.tau = dt;
+  if (K.beta > 0) {  // bound orbit: whole periods P = 2 pi mu / beta^1.5 return the pair to its start
+    const double P = 2 * kPi * K.mu / (K.beta * sqrt(K.beta));
+    if (fabs(dt) >= P) K.tau = dt - P * floor(dt / P);
+  }
 }
 
-// Newton on the reference's residual, kept inside a bracket of (-theta_0, theta_0): the residual
-// is monotone there, so each evaluation moves one end of the bracket, and a step leaving the
-// bracket (or a non-finite residual near a pole) is replaced by bisection.  Stops after a Newton
-// step |d theta| <= 1e-8 theta_0: quadratic convergence leaves ~1e-16 theta_0 after applying it.
-BIA_HDF double hyp_solve_safe(const HypState& S, double th, int kmax, int* evals, bool* ok) {
-  double lo = -S.P.th0, hi = S.P.th0;
-  if (!(th > lo && th < hi)) th = 0.5 * (lo + hi) + 0.5 * (S.th_gss - 0.5 * (lo + hi));
+// dimensionless strength over the step: a1 = eta tau / r0^2, a2 = zeta tau^2 / r0^3,
+// a3 = beta tau^2 / r0^2
+BIA_HDF void kep_features(const KepPair& K, double a[3]) {
+  const double q = K.tau * K.ir0;
+  a[0] = K.eta * q * K.ir0;
+  a[1] = K.zeta * q * q * K.ir0;
+  a[2] = K.beta * q * q;
+}
+
+// series start s = (tau / r0) (1 - a1 / 2)
+BIA_HDF double kep_series_start(const KepPair& K) {
+  const double q = K.tau * K.ir0;
+  return q - 0.5 * K.eta * q * q * K.ir0;
+}
+
+// Halley iteration on t(s) = tau inside a bracket; returns s, *evals = evaluations of t
+BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool* ok) {
+  constexpr double kInf = __builtin_huge_val();
+  double lo = -kInf, hi = kInf;
+  if (K.tau > 0) lo = 0;
+  else hi = 0;
+  if (!(s > lo && s < hi)) s = K.tau * K.ir0;
   *ok = false;
   int k = 0;
-  double prev = 0;  // size of the last Newton step taken (0 after a bisection)
-  for (; k < kmax; k++) {
-    double err, d;
-    double th_new = hyp_newton(th, S.TE, S.P, &err, &d);
-    if (err == 0) {
+  while (k < kmax) {
+    const Stumpff c = stumpff(K.beta * s * s);
+    const double G1 = s * c.c1, G2 = s * s * c.c2, G3 = s * s * s * c.c3;
+    const double F = K.r0 * s + K.eta * G2 + K.zeta * G3 - K.tau;
+    k++;
+    if (F == 0) {
       *ok = true;
-      *evals = k + 1;
-      return th;
+      break;
     }
-    const bool newton_ok = isfinite(err) && isfinite(d) && d != 0 && isfinite(th_new);
-    // converged when the error left after applying this step is below 1e-16 theta_0.  Newton's
-    // error contracts as e' = C e^2 and each step measures the error it removes, so after two
-    // Newton steps of sizes p then s the remaining error is about C s^2 = s^3 / p^2.  The first
-    // step has no such estimate and must itself be below 1e-10 theta_0.  (Tested before the
-    // bracket update, whose end may sit exactly at th when the step rounds to zero.)
-    const double step = fabs(th_new - th);
-    if (newton_ok && (k == 0 || prev == 0 ? step <= 1e-10 * S.P.th0
-                                          : step * step * step <= 1e-16 * S.P.th0 * prev * prev)) {
-      *ok = true;
-      *evals = k + 1;
-      return th_new;
+    if (F < 0) lo = s;
+    else hi = s;
+    const double F1 = K.r0 + K.eta * G1 + K.zeta * G2;
+    const double F2 = K.eta * c.c0 + K.zeta * G1;
+    const double F3 = K.zeta * c.c0 - K.beta * K.eta * G1;
+    const double iF1 = 1 / F1;
+    const double den = F1 - 0.5 * F * iF1 * F2;
+    const double d = -F / den;
+    const double sn = s + d;
+    if (isfinite(sn) && sn > lo && sn < hi && F1 > 0 && den > 0) {
+      const double C = 0.5 * F2 * iF1;
+      const double ad = fabs(d);
+      const double est = fabs(C * C - F3 * iF1 * (1.0 / 6)) * ad * ad * ad;
+      s = sn;
+      if (est <= 1e-16 * fabs(s)) {
+        *ok = true;
+        break;
+      }
+    } else if (lo > -kInf && hi < kInf) {
+      s = 0.5 * (lo + hi);  // bisection
+    } else {  // no upper (lower) end yet: move out
+      s = (hi == kInf) ? 2 * fmax(s, lo) + K.ir0 * fabs(K.tau) : 2 * fmin(s, hi) - K.ir0 * fabs(K.tau);
     }
-    if (!newton_ok) {
-      if (th > 0) hi = th;  // the only poles sit at the ends of the interval
-      else lo = th;
-    } else if ((err > 0) == (d > 0)) {
-      hi = th;  // residual increasing through zero above th
-    } else {
-      lo = th;
-    }
-    const bool take = newton_ok && th_new > lo && th_new < hi;
-    th = take ? th_new : 0.5 * (lo + hi);
-    prev = take ? step : 0;
   }
   *evals = k;
-  return th;
+  return s;
+}
+
+// half deltas of the pair (position departure from v0 dt, velocity change) at anomaly s
+BIA_HDF void kep_output(const KepPair& K, double s, const double rel[6], double d[6]) {
+  const Stumpff c = stumpff(K.beta * s * s);
+  const double G1 = s * c.c1, G2 = s * s * c.c2, G3 = s * s * s * c.c3;
+  const double rn = K.r0 + K.eta * G1 + K.zeta * G2;
+  const double irn = 1 / rn;
+  const double fm1 = -K.mu * G2 * K.ir0;
+  const double gmdt = (K.tau - K.dt) - K.mu * G3;
+  const double fd = -K.mu * G1 * K.ir0 * irn;
+  const double gdm1 = -K.mu * G2 * irn;
+  for (int q = 0; q < 3; q++) {
+    d[q] = 0.5 * (fm1 * rel[q] + gmdt * rel[3 + q]);
+    d[3 + q] = 0.5 * (fd * rel[q] + gdm1 * rel[3 + q]);
+  }
+}
+
+// learned start: trilinear lookup of log(s / s0) (s0 = tau / r0) in a table over the squashed
+// features u_k = asinh(a_k / kFeatScale) / asinh(kFeatMax / kFeatScale), clamped to [-1, 1]
+constexpr double kFeatScale = 1e-2;
+constexpr double kFeatMax = 1e2;
+constexpr double kWeak = 1e-2;  // all |a_k| below this: the series start (one evaluation)
+
+BIA_HDF double feat_squash(double a) {
+  const double u = asinh(a * (1 / kFeatScale)) * (1 / 9.903487555036127);  // asinh(1e4)
+  return fmin(fmax(u, -1.0), 1.0);
+}
+
+BIA_HDF double table_lookup3(const LearnTable& T, const double a[3]) {
+  const int n = T.n;
+  double f[3];
+  int i[3];
+  for (int k = 0; k < 3; k++) {
+    double x = (feat_squash(a[k]) + 1) * 0.5 * (n - 1);
+    x = fmin(fmax(x, 0.0), n - 1.000001);
+    i[k] = (int)x;
+    f[k] = x - i[k];
+  }
+  const float* p = T.t + ((size_t)i[2] * n + i[1]) * n + i[0];
+  const size_t s1 = n, s2 = (size_t)n * n;
+  const double c00 = (1 - f[0]) * p[0] + f[0] * p[1];
+  const double c01 = (1 - f[0]) * p[s1] + f[0] * p[s1 + 1];
+  const double c10 = (1 - f[0]) * p[s2] + f[0] * p[s2 + 1];
+  const double c11 = (1 - f[0]) * p[s2 + s1] + f[0] * p[s2 + s1 + 1];
+  const double c0 = (1 - f[1]) * c00 + f[1] * c01;
+  const double c1 = (1 - f[1]) * c10 + f[1] * c11;
+  return (1 - f[2]) * c0 + f[2] * c1;
+}
+
+template <int MODE>
+BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6], int* branch, int* iters,
+                         unsigned* fl, const LearnTable& T) {
+  KepPair K;
+  kep_setup(eps, dt, rel, K);
+  *branch = (K.beta < 0) ? kHyperbola : (K.beta > 0) ? kEllipse : kParabola;
+  double s = kep_series_start(K);
+  if (MODE == kLearn) {
+    double a[3];
+    kep_features(K, a);
+    if (fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))) >= kWeak) {
+      const double sl = K.tau * K.ir0 * exp(table_lookup3(T, a));
+      if (isfinite(sl) && sl > 0) {
+        s = sl;
+        *fl |= kLearned;
+      }
+    }
+  }
+  bool ok;
+  int ev = 0;
+  s = kep_solve(K, s, 60, &ev, &ok);
+  if (!ok) *fl |= kItrCap;
+  *iters = ev;
+  kep_output(K, s, rel, d);
 }
 
 // ------------------------------------------------------------------------------------
@@ -591,63 +748,41 @@ BIA_HD void rkf_pair(double eps, double dt, const double yz[4], double yn[4], in
 // one pair: frame, classifier (:503-506), 2D solve, half-split epilogue (:507-515)
 // rel = (r_i - r_j, g_i - g_j); d = the six half-deltas credited to i (j gets -d)
 // ------------------------------------------------------------------------------------
-template <int METHOD, int MODE>  // METHOD: 0 = RKF45, 1 = ALG (reference METHOD codes)
+template <int METHOD, int MODE>
 BIA_HDF void pair_eval(double eps, double dt, int itr, const double rel[6], double d[6], int* branch, int* iters,
                        unsigned* fl, Hist hist, const LearnTable& T) {
-  double exi[3], eeta[3], yz[4], yn[4];
-  frame(rel, rel + 3, exi, eeta, yz);
-  int it = 0;
-  if (METHOD == 0) {
-    *branch = kRkf;
-    rkf_pair(eps, dt, yz, yn, &it, fl);
+  if (METHOD == 1 && MODE != kPlain) {
+    kepler_pair<MODE>(eps, dt, rel, d, branch, iters, fl, T);
   } else {
-    double W = sqrt(yz[0] * yz[0] + yz[1] * yz[1]) * (yz[2] * yz[2] + yz[3] * yz[3]) + 2 * eps;
-    if (W > 0) {
-      *branch = kHyperbola;
-      HypState S;
-      hyp_setup(eps, dt, yz, S, fl);
-      double th;
-      if (MODE == kPlain) {
-        th = hyp_solve_plain(S, itr, hist, &it, fl);
-      } else {
-        double th0 = S.th_gss;
-        if (MODE == kLearn) {
-          double x1, v;
-          int b;
-          hyp_features(S, &x1, &v, &b);
-          const double tp = table_lookup(T, b, x1, v);
-          const double thl = S.P.th0 * (x1 + fabs(S.P.cs0) * tp);
-          // the reference guess judges the learned one: a non-finite or out-of-range proposal
-          // is replaced by it
-          if (isfinite(thl) && fabs(thl) < S.P.th0) {
-            th0 = thl;
-            *fl |= kLearned;
-          }
-        }
-        bool ok;
-        th = hyp_solve_safe(S, th0, 24, &it, &ok);
-        if (!ok) {  // fallback: the reference's own guess and iteration control
-          int it2 = 0;
-          th = hyp_solve_plain(S, itr, hist, &it2, fl);
-          it += it2;
-          *fl |= kFallback;
-        }
-      }
-      hyp_finish(S, th, yz, yn);
-    } else if (W == 0) {
-      *branch = kParabola;
-      parabola(eps, dt, yz, yn, &it);
+    double exi[3], eeta[3], yz[4], yn[4];
+    frame(rel, rel + 3, exi, eeta, yz);
+    int it = 0;
+    if (METHOD == 0) {
+      *branch = kRkf;
+      rkf_pair(eps, dt, yz, yn, &it, fl);
     } else {
-      *branch = kEllipse;
-      ellipse(eps, dt, itr, yz, yn, &it, fl);
+      const double W = sqrt(yz[0] * yz[0] + yz[1] * yz[1]) * (yz[2] * yz[2] + yz[3] * yz[3]) + 2 * eps;
+      if (W > 0) {
+        *branch = kHyperbola;
+        HypState S;
+        hyp_setup(eps, dt, yz, S, fl);
+        const double th = hyp_solve_plain(S, itr, hist, &it, fl);
+        hyp_finish(S, th, yz, yn);
+      } else if (W == 0) {
+        *branch = kParabola;
+        parabola(eps, dt, yz, yn, &it);
+      } else {
+        *branch = kEllipse;
+        ellipse(eps, dt, itr, yz, yn, &it, fl);
+      }
     }
-  }
-  *iters = it;
-  double drx = yn[0] - yz[0], dre = yn[1] - yz[1];
-  double dgx = yn[2] - yz[2], dge = yn[3] - yz[3];
-  for (int c = 0; c < 3; c++) {
-    d[c] = (drx * exi[c] + dre * eeta[c] - rel[3 + c] * dt) / 2;
-    d[3 + c] = (dgx * exi[c] + dge * eeta[c]) / 2;
+    *iters = it;
+    const double drx = yn[0] - yz[0], dre = yn[1] - yz[1];
+    const double dgx = yn[2] - yz[2], dge = yn[3] - yz[3];
+    for (int c = 0; c < 3; c++) {
+      d[c] = (drx * exi[c] + dre * eeta[c] - rel[3 + c] * dt) / 2;
+      d[3 + c] = (dgx * exi[c] + dge * eeta[c]) / 2;
+    }
   }
   bool fin = true;
   for (int c = 0; c < 6; c++) fin = fin && isfinite(d[c]);
@@ -657,7 +792,7 @@ BIA_HDF void pair_eval(double eps, double dt, int itr, const double rel[6], doub
 template <int METHOD>
 BIA_HDF void pair_plain(double eps, double dt, int itr, const double rel[6], double d[6], int* branch, int* iters,
                         unsigned* fl, Hist hist) {
-  const LearnTable none{nullptr, 0, 0};
+  const LearnTable none{nullptr, 0};
   pair_eval<METHOD, kPlain>(eps, dt, itr, rel, d, branch, iters, fl, hist, none);
 }
 

@@ -87,7 +87,7 @@ struct bia_ctx {
   bia::DevBuf mlp_w, mlp_ws, mlp_table;
   int mlp_init = 0;   // weights present
   int mlp_ready = 0;  // table built from the current weights
-  int table_nx = 0, table_nv = 0;
+  int table_n = 0;
   long long mlp_step = 0;
   // optional timing of the dominant (pair) kernel with HIP events on the context's stream
   int profiling = 0;

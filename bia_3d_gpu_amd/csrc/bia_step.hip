@@ -313,7 +313,7 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   const auto tuned = tuned_for(ctx, a);
   const int tuned_grid = tuned.first, tuned_occ = tuned.second;
   const int occ = occ_override > 0 ? occ_override : tuned_occ;
-  const LearnTable T{ctx->mlp_table.as<float>(), ctx->table_nx, ctx->table_nv};
+  const LearnTable T{ctx->mlp_table.as<float>(), ctx->table_n};
   int grid = grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : P.ntiles);
   grid = std::min(grid, P.ntiles);
   const int4* tiles = ctx->plan_tiles.as<int4>();

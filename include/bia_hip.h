@@ -39,10 +39,10 @@ extern "C" {
 #define BIA_METHOD_RKF 0  /* per-pair adaptive RKF45 */
 #define BIA_METHOD_ALG 1  /* analytic ellipse / parabola / hyperbola */
 
-/* initialiser modes for the analytic hyperbola branch */
-#define BIA_MODE_PLAIN 0  /* the reference's starting guess and iteration control */
-#define BIA_MODE_LEARN 1  /* learned starting guess + safeguarded Newton, reference guess as fallback */
-#define BIA_MODE_GUESS 2  /* the reference's closed-form guess + the same safeguarded Newton (control) */
+/* solver modes of the analytic method (BIA_METHOD_ALG) */
+#define BIA_MODE_PLAIN 0  /* the reference's conic solvers, starting guesses and iteration control */
+#define BIA_MODE_LEARN 1  /* universal-variable Kepler drift; pairs not weak over dt start from the learned table */
+#define BIA_MODE_GUESS 2  /* the same Kepler drift, every pair from its series start (no network) */
 
 /* pair couplings of the batched entry points */
 #define BIA_COUPLING_SCALAR 0  /* the reference's r'' = eps r/|r|^3, eps per system, equal-mass half split */
@@ -172,11 +172,12 @@ int bia_dim_rkf45_host(bia_ctx *ctx, int np, double eps, double *x0, double xe, 
 /* ---------------------------------------------------------------------------------------- */
 /* learned initialiser (no counterpart in the reference)                                     */
 /* ---------------------------------------------------------------------------------------- */
-/* A 3 -> 64 -> 64 -> 1 GELU network proposes the starting true anomaly of the hyperbola
- * Newton solve (BIA_MODE_LEARN).  Inputs of one hyperbolic pair: x1 = theta_gss / theta_0 (the
- * reference's closed-form guess), v = e^(-1/4), b = 0 attractive / 1 repulsive; output
- * t = (theta / theta_0 - x1) e.  Sample rows are float [x1, v, b, t].  Weights are 4481 floats:
- * W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3.  Products run on BF16 MFMA with FP32
+/* A 3 -> 64 -> 64 -> 1 GELU network proposes the starting universal anomaly of the Kepler drift
+ * (BIA_MODE_LEARN) for pairs that are not weak over dt.  Inputs: the squashed dimensionless
+ * numbers u_k = asinh(a_k / 1e-2) / asinh(1e4) of a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3,
+ * a3 = beta dt^2 / r0^2 (eta = r.v, beta = 2 mu / r0 - v^2, zeta = mu - beta r0, mu = -eps);
+ * output log(s / s0), s0 = dt / r0.  Sample rows are float [u1, u2, u3, target].  Weights are 4481
+ * floats: W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3.  Products run on BF16 MFMA with FP32
  * accumulation; the master weights are FP32. */
 int bia_mlp_nparam(void);
 int bia_mlp_init(bia_ctx *ctx, unsigned long long seed);
@@ -190,10 +191,11 @@ int bia_mlp_grad_dev(bia_ctx *ctx, const float *d_rows, int nrows, float *d_grad
  * loss_curve (host, may be NULL) receives the loss of each step */
 int bia_mlp_train_dev(bia_ctx *ctx, const float *d_rows, int nrows, int steps, int batch, float lr,
                       unsigned long long seed, float *loss_curve);
-/* tabulate the network on an nx x nv grid per branch for the pair kernel; enables BIA_MODE_LEARN */
-int bia_mlp_build_table(bia_ctx *ctx, int nx, int nv);
-/* record nsamples random pairs of a system: rows [x1, v, b, t] for hyperbolic pairs solved to
- * convergence, x1 = NaN for the others (device output) */
+/* tabulate the network on an n x n x n grid of (u1, u2, u3) in [-1, 1]^3 (2 <= n <= 256) for the
+ * pair kernel's trilinear lookup; enables BIA_MODE_LEARN */
+int bia_mlp_build_table(bia_ctx *ctx, int n);
+/* record nsamples pairs of a system that are not weak over dt (each of up to 4096 random draws):
+ * rows [u1, u2, u3, log(s / s0)] solved to convergence, u1 = NaN when none was found (device) */
 int bia_record_samples_dev(bia_ctx *ctx, int n, double eps, double dt, const double *d_state, int ld, int nsamples,
                            unsigned long long seed, float *d_out);
 

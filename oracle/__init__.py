@@ -231,9 +231,11 @@ def sensitivity(state_soa: np.ndarray, eps: float, dt: float, method: int = 1, i
     half-deltas under random 1-ulp perturbations of the pair's six relative-state inputs.
 
     Two correct evaluations of the reference formulas whose libm results differ by one ulp land
-    about this far apart.  A pair whose conic type is itself decided by rounding (classifier
-    value W below 64 rounding units of its terms, orc_conic_indeterminate) has no reproducible
-    answer; its two particles get an infinite bound (``indeterminate`` counts them).  It is large where the reference's formulation is ill-conditioned:
+    about this far apart.  Two kinds of pair have no answer to match and give their two
+    particles an infinite bound (``indeterminate`` counts such particles): a pair whose conic
+    type is decided by rounding (classifier value W below 64 rounding units of its terms,
+    orc_conic_indeterminate), and a pair on which the reference Newton stopped unconverged and
+    the independent RKF45 of the same pair disagrees with it by more than 1e-6.  It is large where the reference's formulation is ill-conditioned:
     acos(q) with q near 1 (pairs starting near periapsis, bia/find_dXYUV_kernel.cu:124) and
     near-parabolic hyperbolae (W = g^2 + 2 eps/r cancels, :98-107)."""
     st = np.ascontiguousarray(state_soa, dtype=np.float64).reshape(6, -1)
@@ -247,7 +249,8 @@ def sensitivity(state_soa: np.ndarray, eps: float, dt: float, method: int = 1, i
 
 
 def indeterminate(bounds) -> int:
-    """Number of particles excluded by sensitivity(): those with a pair of indeterminate type."""
+    """Number of particles excluded by sensitivity() (a pair of indeterminate type, or one the
+    reference algorithm gets wrong)."""
     return int(np.count_nonzero(~np.isfinite(bounds[0])))
 
 

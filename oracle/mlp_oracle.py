@@ -1,7 +1,7 @@
 """FP64 CPU restatement of the learned initialiser's network (test infrastructure only).
 
-Same function as the GPU kernels of bia_3d_gpu_amd/csrc/bia_learn.hip: inputs
-(x1, v, 1 - 2 b), two GELU (tanh form) layers of 64, linear output; loss = mean((y - t)^2).
+Same function as the GPU kernels of bia_3d_gpu_amd/csrc/bia_learn.hip: inputs the first three
+columns of a row, two GELU (tanh form) layers of 64, linear output; loss = mean((y - t)^2).
 Parameters: W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3 (4481 floats).  There is no reference
 counterpart; the gradient is pinned by central finite differences in tests/test_mlp.py.
 """
@@ -36,7 +36,7 @@ def gelu_d(z):
 
 def inputs(rows):
     rows = np.asarray(rows, dtype=np.float64)
-    return np.stack([rows[:, 0], rows[:, 1], 1 - 2 * rows[:, 2]], axis=1)
+    return np.ascontiguousarray(rows[:, :3])
 
 
 def forward(w, rows, cache=False):

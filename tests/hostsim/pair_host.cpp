@@ -4,9 +4,9 @@
 #include "../../bia_3d_gpu_amd/csrc/bia_pair.hpp"
 
 extern "C" void hs_pairs(int method, int64_t n, double eps, double dt, int itr, const double* rel, double* out,
-                         int* branch, int* iters, unsigned* flags, int mode, const float* table, int nx, int nv) {
+                         int* branch, int* iters, unsigned* flags, int mode, const float* table, int tn) {
   double hist[bia::kMaxItr];
-  const bia::LearnTable T{table, nx, nv};
+  const bia::LearnTable T{table, tn};
   for (int64_t k = 0; k < n; k++) {
     bia::Hist h{hist, 1};
     int br = 0, it = 0;

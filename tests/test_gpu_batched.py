@@ -12,7 +12,8 @@ def _cat(states):
     return np.ascontiguousarray(np.concatenate(states, axis=1))
 
 
-def test_batched_equals_single_system_calls(ctx):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_batched_equals_single_system_calls(ctx, mode):
     import torch
     from bia_3d_gpu_amd import ic
     sizes = [100, 333, 1000]
@@ -20,12 +21,12 @@ def test_batched_equals_single_system_calls(ctx):
     eps = [-2.0 / n for n in sizes]
     s = torch.from_numpy(_cat(states)).cuda()
     d = torch.empty_like(s)
-    ctx.step_batched_(s, d, sizes, eps=eps, dt=1e-3)
+    ctx.step_batched_(s, d, sizes, eps=eps, dt=1e-3, mode=mode)
     torch.cuda.synchronize()
     got = d.cpu().numpy()
     off = 0
     for st, n, e in zip(states, sizes, eps):
-        single = ctx.step_host(st, e, 1e-3)
+        single = ctx.step_host(st, e, 1e-3, mode=mode)
         assert np.array_equal(got[:, off:off + n], single)  # same tiles, same reduction order
         off += n
 
@@ -77,7 +78,8 @@ def test_coupled_step_vs_oracle(ctx, kind):
         assert sc[0] > 0 and sc[2] > 0  # attractive bound pairs and repulsive / unbound pairs
 
 
-def test_graph_driver_matches_eager_steps(ctx):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_graph_driver_matches_eager_steps(ctx, mode):
     import torch
     from bia_3d_gpu_amd import ic
     sizes = [300, 129]
@@ -87,27 +89,28 @@ def test_graph_driver_matches_eager_steps(ctx):
     b = a.clone()
     da, db = torch.empty_like(a), torch.empty_like(b)
     for k in range(4):
-        ctx.run_steps_(a, da, sizes, 1, eps=eps, dt=1e-3, use_graph=True)   # replay of the captured graph
-        ctx.step_batched_(b, db, sizes, eps=eps, dt=1e-3)                    # eager work in between
+        ctx.run_steps_(a, da, sizes, 1, eps=eps, dt=1e-3, mode=mode, use_graph=True)  # replay of the graph
+        ctx.step_batched_(b, db, sizes, eps=eps, dt=1e-3, mode=mode)                  # eager work in between
         b.add_(db)
         torch.cuda.synchronize()
         assert torch.equal(a, b), f"replay {k}"
-    ctx.run_steps_(a, da, sizes, 3, eps=eps, dt=1e-3, use_graph=True)
-    ctx.run_steps_(b, db, sizes, 3, eps=eps, dt=1e-3, use_graph=False)
+    ctx.run_steps_(a, da, sizes, 3, eps=eps, dt=1e-3, mode=mode, use_graph=True)
+    ctx.run_steps_(b, db, sizes, 3, eps=eps, dt=1e-3, mode=mode, use_graph=False)
     torch.cuda.synchronize()
     assert torch.equal(a, b)
 
 
-def test_autotuner_keeps_results(ctx):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_autotuner_keeps_results(ctx, mode):
     import torch
     from bia_3d_gpu_amd import ic
     n = 2048
     st = ic.plummer_system(n, seed=31)
     s = torch.from_numpy(st).cuda()
     d0, d1 = torch.empty_like(s), torch.empty_like(s)
-    ctx.step_batched_(s, d0, [n], eps=[-2.0 / n], dt=1e-3)
-    ms, grid, occ = ctx.autotune_(s, d1, [n], eps=[-2.0 / n], dt=1e-3)
-    ctx.step_batched_(s, d1, [n], eps=[-2.0 / n], dt=1e-3)
+    ctx.step_batched_(s, d0, [n], eps=[-2.0 / n], dt=1e-3, mode=mode)
+    ms, grid, occ = ctx.autotune_(s, d1, [n], eps=[-2.0 / n], dt=1e-3, mode=mode)
+    ctx.step_batched_(s, d1, [n], eps=[-2.0 / n], dt=1e-3, mode=mode)
     torch.cuda.synchronize()
     ntiles = (n // 64) * (n // 64 + 1) // 2
     print(f"autotuner: grid {grid} of {ntiles} tiles, occupancy variant {occ}, {ms:.3f} ms")

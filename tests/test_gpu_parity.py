@@ -87,6 +87,46 @@ def test_strong_interaction_all_branches(ctx):
     assert ok
 
 
+@pytest.mark.parametrize("case", ["fixture", "benchmark-like", "dense", "coulomb"])
+def test_kepler_mode_vs_oracle(ctx, fixture1332, case):
+    """BIA_MODE_GUESS (universal-variable Kepler drift) against the oracle's reference step
+    within the reference formulas' own rounding sensitivity; particles with a pair the
+    reference algorithm gets wrong or cannot determine are excluded and counted."""
+    from bia_3d_gpu_amd import ic, MODE_GUESS, COUPLING_COULOMB
+    import torch
+    if case == "coulomb":
+        st, q = ic.coulomb_box(512, seed=13)
+        s = torch.from_numpy(st).cuda()
+        d = torch.empty_like(s)
+        stats = torch.zeros(16, dtype=torch.int64, device="cuda")
+        ctx.step_batched_(s, d, [512], dt=5e-4, mode=MODE_GUESS, coupling=COUPLING_COULOMB, coup=1.0,
+                          charge=torch.from_numpy(q).cuda(), stats=stats)
+        torch.cuda.synchronize()
+        d, stats = d.cpu().numpy(), stats.cpu().numpy()
+        ref = O.bia_step_coupled(st, 2, 1.0, 5e-4, charge=q)
+        b = O.sensitivity(st, 1.0, 5e-4, coupling=2, charge=q)
+        eps = 1.0
+    else:
+        if case == "fixture":
+            st, eps, dt = fixture1332["state"], fixture1332["eps"], fixture1332["dt"]
+        elif case == "benchmark-like":
+            st, eps, dt = ic.plummer_with_binaries(2048, seed=5), -2.0 / 65536, 1e-3
+        else:
+            st, eps, dt = ic.uniform_ball_virial(1024, seed=6), -2.0 / 1024, 1e-2
+        d, stats = ctx.step_host(st, eps, dt, mode=MODE_GUESS, with_stats=True)
+        ref = O.bia_step(st, eps, dt)
+        b = O.sensitivity(st, eps, dt)
+    ok, rp, rv = O.within_condition_bound(st, d, ref, eps, bounds=b)
+    n = st.shape[1]
+    evals = stats[4:8].sum() / stats[0:4].sum()
+    print(f"{case}: error/allowed {rp:.3f} {rv:.3f}; excluded particles {O.indeterminate(b)}; "
+          f"evaluations per pair {evals:.4f}; branches {stats[:4]}")
+    assert ok
+    assert O.indeterminate(b) <= n // 100
+    assert stats[10] == 0 and stats[13] == 0  # no iteration cap, no non-finite output
+    assert evals < 1.1
+
+
 def test_invariants_vs_quad(ctx, fixture1332):
     st, eps = fixture1332["state"], fixture1332["eps"]
     g = ctx.invariants(st, eps)
