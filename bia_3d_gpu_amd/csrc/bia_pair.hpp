@@ -261,9 +261,9 @@ BIA_HDF double hyp_solve_plain(const HypState& S, int itr, Hist hist, int* iters
 // Halley step of size d is about |C^2 - t3 / (6 t1)| d^3 with C = t2 / (2 t1) (t1, t2, t3 the
 // derivatives of t), and the iteration stops when that estimate is below 1e-16 |s|.  Pairs that
 // are weak over dt (the dimensionless numbers a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3,
-// a3 = beta dt^2 / r0^2 all small) start from the second-order series s = dt/r0 (1 - a1/2) and
-// converge in one evaluation; the others start from it too (guess mode) or from the learned
-// table over (a1, a2, a3) (learned mode).
+// a3 = beta dt^2 / r0^2 all small) start from the rational series start (kep_series_start) and
+// converge in one evaluation; the others start from it too (guess mode) or from it corrected by
+// the learned table over (a1, a2, a3) (learned mode).
 // ------------------------------------------------------------------------------------
 struct Stumpff {
   double c0, c1, c2, c3;
@@ -344,10 +344,21 @@ BIA_HDF void kep_features(const KepPair& K, double a[3]) {
   a[2] = K.beta * q * q;
 }
 
-// series start s = (tau / r0) (1 - a1 / 2)
-BIA_HDF double kep_series_start(const KepPair& K) {
-  const double q = K.tau * K.ir0;
-  return q - 0.5 * K.eta * q * q * K.ir0;
+// start of the iteration: with sigma = s / s0, s0 = tau / r0, the time equation reads
+// sigma + (a1/2) sigma^2 + (a2/6) sigma^3 + O(a^2 sigma^4) = 1, whose one-step rational solution
+// sigma = 1 - (a1/2 + a2/6) / (1 + a1 + a2/2) is used while its denominator stays above 1/2
+// (else sigma = 1 - a1/2).  Its error is O(m^3), m = max |a_k|.
+BIA_HDF double kep_series_start(const KepPair& K, const double a[3]) {
+  const double s0 = K.tau * K.ir0;
+  const double den = 1 + a[0] + 0.5 * a[1];
+  const double sig = (den > 0.5) ? 1 - (0.5 * a[0] + a[1] * (1.0 / 6)) / den : 1 - 0.5 * a[0];
+  return s0 * sig;
+}
+
+// scale of the learned target: log(s / s_start) / min(m, 1)^3 is O(1) for every strength m
+BIA_HDF double kep_target_scale(const double a[3]) {
+  const double m = fmin(fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))), 1.0);
+  return m * m * m;
 }
 
 // Halley iteration on t(s) = tau inside a bracket; returns s, *evals = evaluations of t
@@ -370,6 +381,10 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
     }
     if (F < 0) lo = s;
     else hi = s;
+    if (hi - lo <= 4.5e-16 * fabs(s)) {  // bracket closed to rounding: s is the root
+      *ok = true;
+      break;
+    }
     const double F1 = K.r0 + K.eta * G1 + K.zeta * G2;
     const double F2 = K.eta * c.c0 + K.zeta * G1;
     const double F3 = K.zeta * c.c0 - K.beta * K.eta * G1;
@@ -377,15 +392,19 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
     const double den = F1 - 0.5 * F * iF1 * F2;
     const double d = -F / den;
     const double sn = s + d;
-    if (isfinite(sn) && sn > lo && sn < hi && F1 > 0 && den > 0) {
+    const bool valid = isfinite(sn) && F1 > 0 && den > 0;
+    if (valid) {
       const double C = 0.5 * F2 * iF1;
       const double ad = fabs(d);
       const double est = fabs(C * C - F3 * iF1 * (1.0 / 6)) * ad * ad * ad;
-      s = sn;
-      if (est <= 1e-16 * fabs(s)) {
+      if (est <= 1e-16 * fabs(sn)) {  // converged; a root at an end of the bracket may round just outside it
+        s = fmin(fmax(sn, lo), hi);
         *ok = true;
         break;
       }
+    }
+    if (valid && sn > lo && sn < hi) {
+      s = sn;
     } else if (lo > -kInf && hi < kInf) {
       s = 0.5 * (lo + hi);  // bisection
     } else {  // no upper (lower) end yet: move out
@@ -412,8 +431,9 @@ BIA_HDF void kep_output(const KepPair& K, double s, const double rel[6], double 
   }
 }
 
-// learned start: trilinear lookup of log(s / s0) (s0 = tau / r0) in a table over the squashed
-// features u_k = asinh(a_k / kFeatScale) / asinh(kFeatMax / kFeatScale), clamped to [-1, 1]
+// learned start: trilinear lookup of the target log(s / s_start) / min(m, 1)^3 in a table over the
+// squashed features u_k = asinh(a_k / kFeatScale) / asinh(kFeatMax / kFeatScale), clamped to
+// [-1, 1]
 constexpr double kFeatScale = 1e-2;
 constexpr double kFeatMax = 1e2;
 constexpr double kWeak = 1e-2;  // all |a_k| below this: the series start (one evaluation)
@@ -450,16 +470,14 @@ BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6]
   KepPair K;
   kep_setup(eps, dt, rel, K);
   *branch = (K.beta < 0) ? kHyperbola : (K.beta > 0) ? kEllipse : kParabola;
-  double s = kep_series_start(K);
-  if (MODE == kLearn) {
-    double a[3];
-    kep_features(K, a);
-    if (fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))) >= kWeak) {
-      const double sl = K.tau * K.ir0 * exp(table_lookup3(T, a));
-      if (isfinite(sl) && sl > 0) {
-        s = sl;
-        *fl |= kLearned;
-      }
+  double a[3];
+  kep_features(K, a);
+  double s = kep_series_start(K, a);
+  if (MODE == kLearn && fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))) >= kWeak) {
+    const double sl = s * exp(table_lookup3(T, a) * kep_target_scale(a));
+    if (isfinite(sl) && (sl > 0) == (s > 0)) {
+      s = sl;
+      *fl |= kLearned;
     }
   }
   bool ok;

@@ -1,7 +1,7 @@
 """Training driver of the learned initialiser (GPU kernels in csrc/bia_learn.hip).
 
-1. record: the solver records rows [u1, u2, u3, log(s / s0)] from pairs of given systems that
-   are not weak over their dt (the only pairs whose start the network sets);
+1. record: the solver records rows [u1, u2, u3, target] from pairs of given systems that are not
+   weak over their dt (the only pairs whose start the network sets);
 2. balance: rows are resampled to a flat distribution of the largest |u_k| (pair strength);
 3. train: Adam on mini-batches (BF16 MFMA forward/backward, FP32 master weights);
 4. tabulate: the network is evaluated on an n^3 grid that the pair kernel interpolates.
@@ -46,13 +46,16 @@ def balance(rows, count: int, seed: int, bins: int = 32):
 
 
 def training_systems(seed: int = 2024):
-    """Source runs of the recorder: a config-1 ball, a config-2 Coulomb box (its attractive and
-    repulsive couplings) and a Plummer sphere with hard binaries (another seed than the
-    benchmark's), each at step lengths where a good share of pairs is strong over dt."""
+    """Source runs of the recorder: config-1 balls (also at the reference fixture's coupling and
+    step, eps = -6e-7, dt = 1), a config-2 Coulomb box (its attractive and repulsive couplings)
+    and a Plummer sphere with hard binaries (another seed than the benchmark's), each at step
+    lengths where a good share of pairs is strong over dt."""
     s1 = ic.uniform_ball_virial(1332, seed=seed)
     s2, _ = ic.coulomb_box(4096, seed=seed + 1)
     s4 = ic.plummer_with_binaries(65536, seed=seed + 2)
+    s5 = ic.uniform_ball_virial(1332, seed=seed + 3)
     return [
+        (s5, -6e-7, 1.0),
         (s1, -2.0 / 1332, 1e-2),
         (s1, -2.0 / 1332, 3e-2),
         (s2, -2.0, 5e-3),

@@ -176,7 +176,8 @@ int bia_dim_rkf45_host(bia_ctx *ctx, int np, double eps, double *x0, double xe, 
  * (BIA_MODE_LEARN) for pairs that are not weak over dt.  Inputs: the squashed dimensionless
  * numbers u_k = asinh(a_k / 1e-2) / asinh(1e4) of a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3,
  * a3 = beta dt^2 / r0^2 (eta = r.v, beta = 2 mu / r0 - v^2, zeta = mu - beta r0, mu = -eps);
- * output log(s / s0), s0 = dt / r0.  Sample rows are float [u1, u2, u3, target].  Weights are 4481
+ * output log(s / s_start) / min(m, 1)^3, the converged anomaly relative to the rational series
+ * start, m = max |a_k|.  Sample rows are float [u1, u2, u3, target].  Weights are 4481
  * floats: W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3.  Products run on BF16 MFMA with FP32
  * accumulation; the master weights are FP32. */
 int bia_mlp_nparam(void);
@@ -195,7 +196,7 @@ int bia_mlp_train_dev(bia_ctx *ctx, const float *d_rows, int nrows, int steps, i
  * pair kernel's trilinear lookup; enables BIA_MODE_LEARN */
 int bia_mlp_build_table(bia_ctx *ctx, int n);
 /* record nsamples pairs of a system that are not weak over dt (each of up to 4096 random draws):
- * rows [u1, u2, u3, log(s / s0)] solved to convergence, u1 = NaN when none was found (device) */
+ * rows [u1, u2, u3, target] solved to convergence, u1 = NaN when none was found (device) */
 int bia_record_samples_dev(bia_ctx *ctx, int n, double eps, double dt, const double *d_state, int ld, int nsamples,
                            unsigned long long seed, float *d_out);
 

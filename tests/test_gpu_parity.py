@@ -122,9 +122,9 @@ def test_kepler_mode_vs_oracle(ctx, fixture1332, case):
     print(f"{case}: error/allowed {rp:.3f} {rv:.3f}; excluded particles {O.indeterminate(b)}; "
           f"evaluations per pair {evals:.4f}; branches {stats[:4]}")
     assert ok
-    assert O.indeterminate(b) <= n // 100
+    assert O.indeterminate(b) <= 3 * n // 100
     assert stats[10] == 0 and stats[13] == 0  # no iteration cap, no non-finite output
-    assert evals < 1.1
+    assert evals < 1.5
 
 
 def test_invariants_vs_quad(ctx, fixture1332):
