@@ -5,8 +5,8 @@
 // Network input (one pair that is not weak over dt): the squashed dimensionless numbers
 // u_k = asinh(a_k / 1e-2) / asinh(1e4) of a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3,
 // a3 = beta dt^2 / r0^2, which fix the universal Kepler equation up to scale.  Output:
-// log(s / s_start) / min(m, 1)^3, the converged universal anomaly relative to the rational
-// series start, scaled by the start's O(m^3) error, m = max |a_k| (kepler_pair in bia_pair.hpp).
+// log(s / s_start) / min(|a|, 1)^3, the converged universal anomaly relative to the rational
+// series start, scaled by the start's O(|a|^3) error (kepler_pair in bia_pair.hpp).
 //
 // Parameters (fp32 master copy, 4481 floats): W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3.
 // Matrix products run on v_mfma_f32_32x32x16_bf16 with BF16 operands: one wave computes a
@@ -282,9 +282,10 @@ __global__ void k_table_grid(int n, float* __restrict__ rows) {
 
 // Recorder: each thread draws random pairs (i, j) of a system until it finds one that is not
 // weak over dt (some |a_k| >= 1e-2; at most kDraws draws), solves it to convergence from the
-// series start and writes the row [u1, u2, u3, log(s / s_start) / min(m, 1)^3]; NaN in u1 when
-// none was found or the solve failed.
+// series start and writes the row [u1, u2, u3, log(s / s_start) / min(|a|, 1)^3]; NaN in u1 when
+// none was found, the solve failed or the target is an outlier (|target| > 4, ~2e-4 of pairs).
 constexpr int kDraws = 4096;
+constexpr double kMaxTarget = 4;  // rarer outliers (|target| beyond this) are left out of training
 
 __global__ __launch_bounds__(64) void k_record(int n, int ld, double eps, double dt, const double* __restrict__ st,
                                                int nsamples, unsigned long long seed, float* __restrict__ out) {
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(64) void k_record(int n, int ld, double eps, double
     const double s0 = kep_series_start(K, a);
     const double sv = kep_solve(K, s0, 200, &ev, &okc);
     const double t = log(sv / s0) / kep_target_scale(a);
-    if (!okc || !isfinite(t)) return;
+    if (!okc || !(fabs(t) <= kMaxTarget)) return;
     o[0] = (float)feat_squash(a[0]);
     o[1] = (float)feat_squash(a[1]);
     o[2] = (float)feat_squash(a[2]);

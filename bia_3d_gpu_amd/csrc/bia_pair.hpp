@@ -346,18 +346,18 @@ BIA_HDF void kep_features(const KepPair& K, double a[3]) {
 
 // start of the iteration: with sigma = s / s0, s0 = tau / r0, the time equation reads
 // sigma + (a1/2) sigma^2 + (a2/6) sigma^3 + O(a^2 sigma^4) = 1, whose one-step rational solution
-// sigma = 1 - (a1/2 + a2/6) / (1 + a1 + a2/2) is used while its denominator stays above 1/2
-// (else sigma = 1 - a1/2).  Its error is O(m^3), m = max |a_k|.
+// sigma = 1 - (a1/2 + a2/6) / (1 + a1 + a2/2) has an O(m^3) error, m = |a|.  The denominator is
+// held at 1/2 or more and sigma in [0.05, 2], so the start is a continuous function of a.
 BIA_HDF double kep_series_start(const KepPair& K, const double a[3]) {
   const double s0 = K.tau * K.ir0;
-  const double den = 1 + a[0] + 0.5 * a[1];
-  const double sig = (den > 0.5) ? 1 - (0.5 * a[0] + a[1] * (1.0 / 6)) / den : 1 - 0.5 * a[0];
+  const double den = fmax(1 + a[0] + 0.5 * a[1], 0.5);
+  const double sig = fmin(fmax(1 - (0.5 * a[0] + a[1] * (1.0 / 6)) / den, 0.05), 2.0);
   return s0 * sig;
 }
 
-// scale of the learned target: log(s / s_start) / min(m, 1)^3 is O(1) for every strength m
+// scale of the learned target: log(s / s_start) / min(|a|, 1)^3 is O(1) for every strength
 BIA_HDF double kep_target_scale(const double a[3]) {
-  const double m = fmin(fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))), 1.0);
+  const double m = fmin(sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]), 1.0);
   return m * m * m;
 }
 
@@ -431,7 +431,7 @@ BIA_HDF void kep_output(const KepPair& K, double s, const double rel[6], double 
   }
 }
 
-// learned start: trilinear lookup of the target log(s / s_start) / min(m, 1)^3 in a table over the
+// learned start: trilinear lookup of the target log(s / s_start) / min(|a|, 1)^3 in a table over the
 // squashed features u_k = asinh(a_k / kFeatScale) / asinh(kFeatMax / kFeatScale), clamped to
 // [-1, 1]
 constexpr double kFeatScale = 1e-2;
