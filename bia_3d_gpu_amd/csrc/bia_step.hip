@@ -3,9 +3,9 @@
 // per-particle entry point find_dxyuv_kernel_.
 //
 // Tiling.  The i<j triangle of each system is cut into 64x64 tiles (I <= J).  One wave owns one
-// tile: lane l holds row particle i = 64 I + l in registers and walks the 64 column particles j
-// of tile J in order (j is wave-uniform, so its state comes in through scalar loads).  Each lane
-// keeps its row sum in registers; the column sum of each j is a 64-lane butterfly reduction.
+// tile: lane l holds row particle i = 64 I + l in registers, the column tile sits in LDS, and
+// the lanes walk the columns in a skewed order so that every lane works on a different column at
+// every step and the column sums rotate with their columns (k_tiles).  Row sums stay in registers.
 // Results go to a per-system partial buffer part[K][c][i] in which every (tile K, particle i)
 // slot has exactly one writer:
 //   tile (I,J), I<J : rows of I  -> part[J][c][i],  columns of J -> part[I][c][j]
@@ -31,11 +31,6 @@ namespace bia {
 
 constexpr int kWave = 64;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
-}
 __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -72,8 +67,23 @@ struct PairStats {
 
 enum Coupling : int { kScalar = 0, kGravity = 1, kCoulomb = 2 };
 
+// lane l receives x from lane l + 1 (lane 63 from lane 0): DPP wave_rol:1 on both halves
+__device__ __forceinline__ double rot_down(double x) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(v & 0xffffffffLL), 0x134, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), 0x134, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // OCC: minimum waves per SIMD asked of the register allocator (1 = its own choice, 3 waves for the
-// ALG kernels; 4 = at most 128 VGPRs, a few spills); picked per shape by the autotuner
+// ALG kernels; 4 = at most 128 VGPRs, a few spills); picked per shape by the autotuner.
+//
+// One wave per tile.  Lane l holds row i = 64 I + l; the column tile's states are staged in LDS.
+// At step k lane l takes column j = (l + k) mod 64, so the 64 lanes work on 64 different
+// columns and no column needs a cross-lane reduction: the column sums travel with their column,
+// one lane down per step (rot_down), and after 64 steps lane l holds column l's sum.  A diagonal
+// tile (I = J) takes steps k = 1..32 (at k = 32 lanes 0..31 only), which covers each unordered
+// pair once; its column sums end in lane (l - 33) mod 64 and are fetched once per tile.
 template <int METHOD, int MODE, int COUP, bool STATS, int OCC>
 __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const int4* __restrict__ tiles,
                                                  const SysDesc* __restrict__ sys, const double* __restrict__ st,
@@ -81,6 +91,8 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
                                                  double coup, double dt, int itr, double* __restrict__ part,
                                                  unsigned long long* __restrict__ stats, LearnTable T) {
   __shared__ double hist_lds[kMaxItr * kWave];
+  __shared__ double cst[6][kWave];  // column tile state
+  __shared__ double cmq[2][kWave];  // column masses and charges (couplings only)
   const int lane = threadIdx.x;
   Hist hist{hist_lds + lane, kWave};
   PairStats ps;
@@ -100,29 +112,40 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
       mi = (row_ok && mass) ? mass[gi] : 1.0;
       if (COUP == kCoulomb) qi = row_ok ? charge[gi] : 0.0;
     }
-    double acc[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
     const int j0 = J * kWave;
     const int jn = min(kWave, S.n - j0);
-#pragma unroll 1
-    for (int jj = 0; jj < jn; ++jj) {
-      const int gj = S.off + j0 + jj;
-      double rj[6];
+    __syncthreads();  // the previous tile's readers are done with the LDS column tile
+    {
+      const bool ok = lane < jn;
+      const int gj = S.off + j0 + lane;
 #pragma unroll
-      for (int c = 0; c < 6; c++) rj[c] = st[(size_t)c * lds + gj];
+      for (int c = 0; c < 6; c++) cst[c][lane] = ok ? st[(size_t)c * lds + gj] : 0.0;
+      if (COUP != kScalar) {
+        cmq[0][lane] = (ok && mass) ? mass[gj] : 1.0;
+        cmq[1][lane] = (ok && COUP == kCoulomb) ? charge[gj] : 0.0;
+      }
+    }
+    __syncthreads();
+    double acc[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
+    const bool diag = (I == J);
+    const int k0 = diag ? 1 : 0, k1 = diag ? 33 : kWave;
+#pragma unroll 1
+    for (int k = k0; k < k1; ++k) {
+      const int jl = (lane + k) & (kWave - 1);
       double d[6] = {0, 0, 0, 0, 0, 0};
       double wi = 1, wj = 1;  // multiples of the half deltas credited to i and debited from j
-      if (row_ok && (I != J || lane < jj)) {
+      if (row_ok && jl < jn && (!diag || k < 32 || lane < 32)) {
         double eps = S.eps;
         if (COUP != kScalar) {
-          const double mj = mass ? mass[gj] : 1.0;
+          const double mj = cmq[0][jl];
           if (COUP == kGravity) eps = -coup * (mi + mj);
-          else eps = coup * qi * charge[gj] * (1 / mi + 1 / mj);
+          else eps = coup * qi * cmq[1][jl] * (1 / mi + 1 / mj);
           wi = 2 * mj / (mi + mj);
           wj = 2 * mi / (mi + mj);
         }
         double rel[6];
 #pragma unroll
-        for (int c = 0; c < 6; c++) rel[c] = ri[c] - rj[c];
+        for (int c = 0; c < 6; c++) rel[c] = ri[c] - cst[c][jl];
         int br = 0, it = 0;
         unsigned fl = 0;
         pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T);
@@ -132,20 +155,22 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
       for (int c = 0; c < 6; c++) {
         if (COUP == kScalar) {
           acc[c] += d[c];
-          const double s = wave_sum(d[c]);
-          if (lane == jj) col[c] = -s;
+          col[c] = rot_down(col[c] - d[c]);
         } else {
           acc[c] += wi * d[c];
-          const double s = wave_sum(wj * d[c]);
-          if (lane == jj) col[c] = -s;
+          col[c] = rot_down(col[c] - wj * d[c]);
         }
       }
     }
     const int ldp = S.nt * kWave;
     double* P = part + S.poff;
-    if (I == J) {
+    if (diag) {
+      // lane l holds the column sum of particle (l + 33) mod 64; particle l's is in lane l - 33
 #pragma unroll
-      for (int c = 0; c < 6; c++) P[((size_t)I * 6 + c) * ldp + il] = acc[c] + col[c];
+      for (int c = 0; c < 6; c++) {
+        const double cl = __shfl(col[c], (lane + 31) & (kWave - 1), kWave);
+        P[((size_t)I * 6 + c) * ldp + il] = acc[c] + cl;
+      }
     } else {
 #pragma unroll
       for (int c = 0; c < 6; c++) {
