@@ -132,6 +132,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learn", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--modes", default="plain,guess,learned",
+                    help="comma-separated subset of plain,guess,learned (profiling runs)")
+    ap.add_argument("--copy-mb", type=float, default=0.0,
+                    help="also run one device copy of this many MB (HBM counter calibration)")
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-steps", type=int, default=3000)
     args = ap.parse_args()
@@ -151,15 +155,26 @@ def main():
     st = make_state(n, args.seed)
     pairs = n * (n - 1) / 2
     ctx = Context(0)
+    modes = set(args.modes.split(","))
+    if args.copy_mb > 0:  # a plain copy of known size for scaling HBM byte counters
+        nb = int(args.copy_mb * 1e6) // 8
+        a = torch.empty(nb, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {nb * 8} bytes read, {nb * 8} bytes written")
+        del a, b
     res = {}
     for name, mode in (("plain", MODE_PLAIN), ("guess", MODE_GUESS)):
+        if name not in modes:
+            continue
         r = run_mode(ctx, st, eps, args.dt, args.itr, mode, args.steps, args.warmup, not args.no_autotune)
         r["pairs_per_s"] = pairs / (r["ms_per_step"] / 1e3)
         res[name] = r
         log(f"{name}: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms ({r['tflops']:.1f} TFLOP/s), "
             f"|dE/E0| {r['dE_rel']:.2e} over {r['dE_steps']} steps, tuned {r['tuned']}")
     train = None
-    if not args.no_learn:
+    if not args.no_learn and "learned" in modes:
         t0 = time.perf_counter()
         w, curve, rows = learn.train_default(ctx, nsamples=args.train_samples, steps=args.train_steps, log=log)
         torch.cuda.synchronize()
@@ -171,8 +186,8 @@ def main():
         res["learned"] = r
         log(f"learned: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms ({r['tflops']:.1f} TFLOP/s), "
             f"|dE/E0| {r['dE_rel']:.2e}, tuned {r['tuned']}")
-    plain = res["plain"]
-    head_name = "learned" if "learned" in res else "guess"
+    plain = res.get("plain")
+    head_name = "learned" if "learned" in res else "guess" if "guess" in res else "plain"
     head = res[head_name]
     # roofline of the dominant kernel (the pair kernel of the reported mode), its own algorithm's flops
     achieved = head["tflops"]
@@ -193,7 +208,8 @@ def main():
         "cpu_baseline": cpu,
         "modes": {k: {q: v[q] for q in ("pairs_per_s", "ms_per_step", "kernel_ms", "tflops", "dE_rel", "dE_steps",
                                          "stats", "tuned")} for k, v in res.items()},
-        "speedup_vs_plain": {k: v["pairs_per_s"] / plain["pairs_per_s"] for k, v in res.items()},
+        "speedup_vs_plain": None if plain is None else {k: v["pairs_per_s"] / plain["pairs_per_s"]
+                                                        for k, v in res.items()},
         "training": train,
     }
     print(json.dumps(out), flush=True)

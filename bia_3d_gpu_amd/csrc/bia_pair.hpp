@@ -270,35 +270,42 @@ struct Stumpff {
 };
 
 // c_k(y) = sum_j (-y)^j / (2j + k)!: series at y / 4^m with |y / 4^m| <= 0.1 (8 terms, truncation
-// below 1e-21), then m doublings c0(4y) = 2 c0^2 - 1, c1(4y) = c0 c1, c2(4y) = c1^2 / 2,
-// c3(4y) = (c2 + c0 c3) / 4.
+// below 1e-21; 4 terms when |y| <= 1e-4, truncation below 1e-22), then m doublings
+// c0(4y) = 2 c0^2 - 1, c1(4y) = c0 c1, c2(4y) = c1^2 / 2, c3(4y) = (c2 + c0 c3) / 4.
 BIA_HDF Stumpff stumpff(double y) {
-  int m = 0;
-  while (fabs(y) > 0.1 && m < 60) {
-    y *= 0.25;
-    m++;
-  }
   constexpr double i2 = 1.0 / 2, i3 = 1.0 / 6, i4 = 1.0 / 24, i5 = 1.0 / 120, i6 = 1.0 / 720, i7 = 1.0 / 5040,
                    i8 = 1.0 / 40320, i9 = 1.0 / 362880, i10 = 1.0 / 3628800, i11 = 1.0 / 39916800,
                    i12 = 1.0 / 479001600, i13 = 1.0 / 6227020800.0, i14 = 1.0 / 87178291200.0,
                    i15 = 1.0 / 1307674368000.0, i16 = 1.0 / 20922789888000.0, i17 = 1.0 / 355687428096000.0;
-  const double z = -y;
-  double c2 = i16;
-  c2 = c2 * z + i14;
-  c2 = c2 * z + i12;
-  c2 = c2 * z + i10;
-  c2 = c2 * z + i8;
-  c2 = c2 * z + i6;
-  c2 = c2 * z + i4;
-  c2 = c2 * z + i2;
-  double c3 = i17;
-  c3 = c3 * z + i15;
-  c3 = c3 * z + i13;
-  c3 = c3 * z + i11;
-  c3 = c3 * z + i9;
-  c3 = c3 * z + i7;
-  c3 = c3 * z + i5;
-  c3 = c3 * z + i3;
+  double c2, c3;
+  int m = 0;
+  if (fabs(y) <= 1e-4) {
+    const double z = -y;
+    c2 = ((i8 * z + i6) * z + i4) * z + i2;
+    c3 = ((i9 * z + i7) * z + i5) * z + i3;
+  } else {
+    while (fabs(y) > 0.1 && m < 60) {
+      y *= 0.25;
+      m++;
+    }
+    const double z = -y;
+    c2 = i16;
+    c2 = c2 * z + i14;
+    c2 = c2 * z + i12;
+    c2 = c2 * z + i10;
+    c2 = c2 * z + i8;
+    c2 = c2 * z + i6;
+    c2 = c2 * z + i4;
+    c2 = c2 * z + i2;
+    c3 = i17;
+    c3 = c3 * z + i15;
+    c3 = c3 * z + i13;
+    c3 = c3 * z + i11;
+    c3 = c3 * z + i9;
+    c3 = c3 * z + i7;
+    c3 = c3 * z + i5;
+    c3 = c3 * z + i3;
+  }
   double c1 = 1 - y * c3, c0 = 1 - y * c2;
   for (; m > 0; m--) {
     const double n3 = 0.25 * (c2 + c0 * c3);
@@ -313,6 +320,29 @@ BIA_HDF Stumpff stumpff(double y) {
   return {c0, c1, c2, c3};
 }
 
+// 1/x from the hardware reciprocal and two Newton steps (within an ulp; 1/0 and non-finite x
+// give a non-finite result); the host build divides
+BIA_HDF double fast_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+#else
+  return 1.0 / x;
+#endif
+}
+
+// G1..G3 at anomaly s
+struct KepG {
+  double s, c0, G1, G2, G3;
+};
+
+BIA_HDF KepG kep_g(double beta, double s) {
+  const Stumpff c = stumpff(beta * s * s);
+  const double s2 = s * s;
+  return {s, c.c0, s * c.c1, s2 * c.c2, s2 * s * c.c3};
+}
+
 // pair quantities shared by the start, the iteration and the output
 struct KepPair {
   double mu, r0, ir0, eta, beta, zeta, tau, dt;
@@ -323,7 +353,7 @@ BIA_HDF void kep_setup(double eps, double dt, const double rel[6], KepPair& K) {
   const double v2 = rel[3] * rel[3] + rel[4] * rel[4] + rel[5] * rel[5];
   K.mu = -eps;
   K.r0 = sqrt(r2);
-  K.ir0 = 1 / K.r0;
+  K.ir0 = fast_rcp(K.r0);
   K.eta = rel[0] * rel[3] + rel[1] * rel[4] + rel[2] * rel[5];
   K.beta = 2 * K.mu * K.ir0 - v2;
   K.zeta = K.mu - K.beta * K.r0;
@@ -361,8 +391,9 @@ BIA_HDF double kep_target_scale(const double a[3]) {
   return m * m * m;
 }
 
-// Halley iteration on t(s) = tau inside a bracket; returns s, *evals = evaluations of t
-BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool* ok) {
+// Halley iteration on t(s) = tau inside a bracket; returns s, *evals = evaluations of t, and in
+// *last the G functions at the last point where t was evaluated
+BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool* ok, KepG* last) {
   constexpr double kInf = __builtin_huge_val();
   double lo = -kInf, hi = kInf;
   if (K.tau > 0) lo = 0;
@@ -371,9 +402,9 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
   *ok = false;
   int k = 0;
   while (k < kmax) {
-    const Stumpff c = stumpff(K.beta * s * s);
-    const double G1 = s * c.c1, G2 = s * s * c.c2, G3 = s * s * s * c.c3;
-    const double F = K.r0 * s + K.eta * G2 + K.zeta * G3 - K.tau;
+    const KepG g = kep_g(K.beta, s);
+    *last = g;
+    const double F = K.r0 * s + K.eta * g.G2 + K.zeta * g.G3 - K.tau;
     k++;
     if (F == 0) {
       *ok = true;
@@ -385,12 +416,12 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
       *ok = true;
       break;
     }
-    const double F1 = K.r0 + K.eta * G1 + K.zeta * G2;
-    const double F2 = K.eta * c.c0 + K.zeta * G1;
-    const double F3 = K.zeta * c.c0 - K.beta * K.eta * G1;
-    const double iF1 = 1 / F1;
+    const double F1 = K.r0 + K.eta * g.G1 + K.zeta * g.G2;
+    const double F2 = K.eta * g.c0 + K.zeta * g.G1;
+    const double F3 = K.zeta * g.c0 - K.beta * K.eta * g.G1;
+    const double iF1 = fast_rcp(F1);
     const double den = F1 - 0.5 * F * iF1 * F2;
-    const double d = -F / den;
+    const double d = -F * fast_rcp(den);
     const double sn = s + d;
     const bool valid = isfinite(sn) && F1 > 0 && den > 0;
     if (valid) {
@@ -415,12 +446,26 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
   return s;
 }
 
-// half deltas of the pair (position departure from v0 dt, velocity change) at anomaly s
-BIA_HDF void kep_output(const KepPair& K, double s, const double rel[6], double d[6]) {
-  const Stumpff c = stumpff(K.beta * s * s);
-  const double G1 = s * c.c1, G2 = s * s * c.c2, G3 = s * s * s * c.c3;
+// half deltas of the pair (position departure from v0 dt, velocity change) at anomaly s.  When s
+// is within 1e-4 |s| of the last evaluated point, the G functions are carried there by their
+// Taylor series (G_k' = G_(k-1), G0' = -beta G1; terms to d^4, the rest below 1e-20 relative)
+// instead of a new Stumpff evaluation.
+BIA_HDF void kep_output(const KepPair& K, double s, const KepG& e, const double rel[6], double d[6]) {
+  double G1, G2, G3;
+  const double h = s - e.s;
+  if (fabs(h) <= 1e-4 * fabs(s)) {
+    const double h2 = 0.5 * h * h, h3 = h2 * h * (1.0 / 3), h4 = h3 * h * 0.25;
+    G3 = e.G3 + h * e.G2 + h2 * e.G1 + h3 * e.c0 - h4 * K.beta * e.G1;
+    G2 = e.G2 + h * e.G1 + h2 * e.c0 - K.beta * (h3 * e.G1 + h4 * e.c0);
+    G1 = e.G1 + h * e.c0 - K.beta * (h2 * e.G1 + h3 * e.c0) + h4 * K.beta * K.beta * e.G1;
+  } else {
+    const KepG g = kep_g(K.beta, s);
+    G1 = g.G1;
+    G2 = g.G2;
+    G3 = g.G3;
+  }
   const double rn = K.r0 + K.eta * G1 + K.zeta * G2;
-  const double irn = 1 / rn;
+  const double irn = fast_rcp(rn);
   const double fm1 = -K.mu * G2 * K.ir0;
   const double gmdt = (K.tau - K.dt) - K.mu * G3;
   const double fd = -K.mu * G1 * K.ir0 * irn;
@@ -482,10 +527,11 @@ BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6]
   }
   bool ok;
   int ev = 0;
-  s = kep_solve(K, s, 60, &ev, &ok);
+  KepG last;
+  s = kep_solve(K, s, 60, &ev, &ok, &last);
   if (!ok) *fl |= kItrCap;
   *iters = ev;
-  kep_output(K, s, rel, d);
+  kep_output(K, s, last, rel, d);
 }
 
 // ------------------------------------------------------------------------------------
