@@ -332,6 +332,19 @@ BIA_HDF double fast_rcp(double x) {
 #endif
 }
 
+// 1/sqrt(x) from the hardware estimate and two Newton steps (within an ulp); the host build
+// uses the library square root
+BIA_HDF double fast_rsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * fma(-hx * y, y, 1.5);
+  return y * fma(-hx * y, y, 1.5);
+#else
+  return 1.0 / sqrt(x);
+#endif
+}
+
 // G1..G3 at anomaly s
 struct KepG {
   double s, c0, G1, G2, G3;
@@ -352,8 +365,8 @@ BIA_HDF void kep_setup(double eps, double dt, const double rel[6], KepPair& K) {
   const double r2 = rel[0] * rel[0] + rel[1] * rel[1] + rel[2] * rel[2];
   const double v2 = rel[3] * rel[3] + rel[4] * rel[4] + rel[5] * rel[5];
   K.mu = -eps;
-  K.r0 = sqrt(r2);
-  K.ir0 = fast_rcp(K.r0);
+  K.ir0 = fast_rsqrt(r2);
+  K.r0 = r2 * K.ir0;
   K.eta = rel[0] * rel[3] + rel[1] * rel[4] + rel[2] * rel[5];
   K.beta = 2 * K.mu * K.ir0 - v2;
   K.zeta = K.mu - K.beta * K.r0;
@@ -381,7 +394,7 @@ BIA_HDF void kep_features(const KepPair& K, double a[3]) {
 BIA_HDF double kep_series_start(const KepPair& K, const double a[3]) {
   const double s0 = K.tau * K.ir0;
   const double den = fmax(1 + a[0] + 0.5 * a[1], 0.5);
-  const double sig = fmin(fmax(1 - (0.5 * a[0] + a[1] * (1.0 / 6)) / den, 0.05), 2.0);
+  const double sig = fmin(fmax(1 - (0.5 * a[0] + a[1] * (1.0 / 6)) * fast_rcp(den), 0.05), 2.0);
   return s0 * sig;
 }
 
@@ -448,12 +461,17 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
 
 // half deltas of the pair (position departure from v0 dt, velocity change) at anomaly s.  When s
 // is within 1e-4 |s| of the last evaluated point, the G functions are carried there by their
-// Taylor series (G_k' = G_(k-1), G0' = -beta G1; terms to d^4, the rest below 1e-20 relative)
-// instead of a new Stumpff evaluation.
+// Taylor series (G_k' = G_(k-1), G0' = -beta G1; terms to h^4, or to h^2 within 1e-6 |s|; the
+// rest below 1e-18 relative) instead of a new Stumpff evaluation.
 BIA_HDF void kep_output(const KepPair& K, double s, const KepG& e, const double rel[6], double d[6]) {
   double G1, G2, G3;
   const double h = s - e.s;
-  if (fabs(h) <= 1e-4 * fabs(s)) {
+  if (fabs(h) <= 1e-6 * fabs(s)) {  // terms to h^2; the h^3 terms are below 2e-19 relative
+    const double h2 = 0.5 * h * h;
+    G3 = e.G3 + h * e.G2 + h2 * e.G1;
+    G2 = e.G2 + h * e.G1 + h2 * e.c0;
+    G1 = e.G1 + h * e.c0 - h2 * K.beta * e.G1;
+  } else if (fabs(h) <= 1e-4 * fabs(s)) {
     const double h2 = 0.5 * h * h, h3 = h2 * h * (1.0 / 3), h4 = h3 * h * 0.25;
     G3 = e.G3 + h * e.G2 + h2 * e.G1 + h3 * e.c0 - h4 * K.beta * e.G1;
     G2 = e.G2 + h * e.G1 + h2 * e.c0 - K.beta * (h3 * e.G1 + h4 * e.c0);

@@ -75,8 +75,8 @@ __device__ __forceinline__ double rot_down(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// OCC: minimum waves per SIMD asked of the register allocator (1 = its own choice, 3 waves for the
-// ALG kernels; 4 = at most 128 VGPRs, a few spills); picked per shape by the autotuner.
+// OCC: minimum waves per SIMD asked of the register allocator (1 = its own choice; 4 = at most 128
+// VGPRs; 5 = at most 96, for the Kepler-drift modes); picked per shape by the autotuner.
 //
 // One wave per tile.  Lane l holds row i = 64 I + l; the column tile's states are staged in LDS.
 // At step k lane l takes column j = (l + k) mod 64, so the 64 lanes work on 64 different
@@ -349,10 +349,11 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
 #define L_OCC(M, MO, CP, S, O)                                                                                     \
   hipLaunchKernelGGL((k_tiles<M, MO, CP, S, O>), dim3(grid), dim3(kWave), 0, ctx->stream, P.ntiles, a.ld, tiles, sys, \
                      a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, a.d_stats, T)
-#define L_(M, MO, CP, S)                         \
-  do {                                           \
-    if (M == 1 && occ == 4) L_OCC(M, MO, CP, S, 4); \
-    else L_OCC(M, MO, CP, S, 1);                 \
+#define L_(M, MO, CP, S)                                    \
+  do {                                                      \
+    if (M == 1 && occ == 4) L_OCC(M, MO, CP, S, 4);         \
+    else if (M == 1 && MO != 0 && occ == 5) L_OCC(M, MO, CP, S, 5); \
+    else L_OCC(M, MO, CP, S, 1);                            \
   } while (0)
 #define L_COUP(M, MO, S)                                                  \
   do {                                                                    \
@@ -411,7 +412,9 @@ static int autotune_locked(bia_ctx* ctx, const StepArgs& a, float* best_ms, int*
   float best = 1e30f;
   int bestg = ntiles, besto = 1;
   int status = BIA_OK;
-  const std::vector<int> occs = (a.method == BIA_METHOD_ALG) ? std::vector<int>{1, 4} : std::vector<int>{1};
+  std::vector<int> occs = {1};
+  if (a.method == BIA_METHOD_ALG) occs.push_back(4);
+  if (a.method == BIA_METHOD_ALG && a.mode != BIA_MODE_PLAIN) occs.push_back(5);
   for (int o : occs)
     for (int g : cands) {
       if ((status = launch_step(ctx, b, g, o)) != BIA_OK) break;  // warm-up

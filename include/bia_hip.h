@@ -135,7 +135,7 @@ int bia_step_batched_dev(bia_ctx *ctx, int method, int mode, int coupling, int n
 
 /* Autotuner: times the pair kernel's persistent grid size and register-occupancy variant on this
  * input (one step per candidate, written to d_delta) and keeps the fastest for this list of sizes
- * in the context.  *best_grid receives grid * 10 + occupancy variant (1 default, 4). */
+ * in the context.  *best_grid receives grid * 10 + occupancy variant (1 default, 4 or 5 waves/SIMD). */
 int bia_autotune_dev(bia_ctx *ctx, int method, int mode, int coupling, int nsys, const int *sizes, const double *eps,
                      double coup, double dt, int itr, const double *d_state, int ld, const double *d_mass,
                      const double *d_charge, double *d_delta, float *best_ms, int *best_grid);

@@ -114,7 +114,7 @@ def test_autotuner_keeps_results(ctx, mode):
     torch.cuda.synchronize()
     ntiles = (n // 64) * (n // 64 + 1) // 2
     print(f"autotuner: grid {grid} of {ntiles} tiles, occupancy variant {occ}, {ms:.3f} ms")
-    assert 0 < grid <= ntiles and ms > 0 and occ in (1, 4)
+    assert 0 < grid <= ntiles and ms > 0 and occ in (1, 4, 5)
     assert torch.equal(d0, d1)  # grid size and register allocation change scheduling only
 
 
