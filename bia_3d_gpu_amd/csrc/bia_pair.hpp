@@ -527,17 +527,6 @@ BIA_HDF double table_lookup3(const LearnTable& T, const double a[3]) {
   return (1 - f[2]) * c0 + f[2] * c1;
 }
 
-// the start s corrected by the learned table (kept out of line: rare, and its registers would
-// otherwise be reserved in the hot loop)
-BIA_HDN double learned_start(const LearnTable& T, const double a[3], double s, unsigned* fl) {
-  const double sl = s * exp(table_lookup3(T, a) * kep_target_scale(a));
-  if (isfinite(sl) && (sl > 0) == (s > 0)) {
-    *fl |= kLearned;
-    return sl;
-  }
-  return s;
-}
-
 template <int MODE>
 BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6], int* branch, int* iters,
                          unsigned* fl, const LearnTable& T) {
@@ -547,7 +536,13 @@ BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6]
   double a[3];
   kep_features(K, a);
   double s = kep_series_start(K, a);
-  if (MODE == kLearn && fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))) >= kWeak) s = learned_start(T, a, s, fl);
+  if (MODE == kLearn && fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))) >= kWeak) {
+    const double sl = s * exp(table_lookup3(T, a) * kep_target_scale(a));
+    if (isfinite(sl) && (sl > 0) == (s > 0)) {
+      s = sl;
+      *fl |= kLearned;
+    }
+  }
   bool ok;
   int ev = 0;
   KepG last;

@@ -5,7 +5,7 @@
 // Tiling.  The i<j triangle of each system is cut into 64x64 tiles (I <= J).  One wave owns one
 // tile: lane l holds row particle i = 64 I + l in registers, the column tile sits in LDS, and
 // the lanes walk the columns in a skewed order so that every lane works on a different column at
-// every step and adds into that column's LDS sum (k_tiles).  Row sums stay in registers.
+// every step and the column sums rotate with their columns (k_tiles).  Row sums stay in registers.
 // Results go to a per-system partial buffer part[K][c][i] in which every (tile K, particle i)
 // slot has exactly one writer:
 //   tile (I,J), I<J : rows of I  -> part[J][c][i],  columns of J -> part[I][c][j]
@@ -67,16 +67,23 @@ struct PairStats {
 
 enum Coupling : int { kScalar = 0, kGravity = 1, kCoulomb = 2 };
 
+// lane l receives x from lane l + 1 (lane 63 from lane 0): DPP wave_rol:1 on both halves
+__device__ __forceinline__ double rot_down(double x) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(v & 0xffffffffLL), 0x134, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), 0x134, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // OCC: minimum waves per SIMD asked of the register allocator (1 = its own choice; 4 = at most 128
 // VGPRs; 5 = at most 96, for the Kepler-drift modes); picked per shape by the autotuner.
 //
 // One wave per tile.  Lane l holds row i = 64 I + l; the column tile's states are staged in LDS.
 // At step k lane l takes column j = (l + k) mod 64, so the 64 lanes work on 64 different
-// columns and no column needs a cross-lane reduction: each lane adds its pair's share into the
-// column's LDS sum (ds_add_f64; the 64 addresses of a step are distinct, and the steps of the one
-// wave run in program order, so every column is summed in a fixed order).  A diagonal tile
-// (I = J) takes steps k = 1..32 (at k = 32 lanes 0..31 only), which covers each unordered pair
-// once.
+// columns and no column needs a cross-lane reduction: the column sums travel with their column,
+// one lane down per step (rot_down), and after 64 steps lane l holds column l's sum.  A diagonal
+// tile (I = J) takes steps k = 1..32 (at k = 32 lanes 0..31 only), which covers each unordered
+// pair once; its column sums end in lane (l - 33) mod 64 and are fetched once per tile.
 template <int METHOD, int MODE, int COUP, bool STATS, int OCC>
 __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const int4* __restrict__ tiles,
                                                  const SysDesc* __restrict__ sys, const double* __restrict__ st,
@@ -85,7 +92,6 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
                                                  unsigned long long* __restrict__ stats, LearnTable T) {
   __shared__ double hist_lds[kMaxItr * kWave];
   __shared__ double cst[6][kWave];  // column tile state
-  __shared__ double ccol[6][kWave]; // column sums of the tile
   __shared__ double cmq[2][kWave];  // column masses and charges (couplings only)
   const int lane = threadIdx.x;
   Hist hist{hist_lds + lane, kWave};
@@ -113,17 +119,14 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
       const bool ok = lane < jn;
       const int gj = S.off + j0 + lane;
 #pragma unroll
-      for (int c = 0; c < 6; c++) {
-        cst[c][lane] = ok ? st[(size_t)c * lds + gj] : 0.0;
-        ccol[c][lane] = 0.0;
-      }
+      for (int c = 0; c < 6; c++) cst[c][lane] = ok ? st[(size_t)c * lds + gj] : 0.0;
       if (COUP != kScalar) {
         cmq[0][lane] = (ok && mass) ? mass[gj] : 1.0;
         cmq[1][lane] = (ok && COUP == kCoulomb) ? charge[gj] : 0.0;
       }
     }
     __syncthreads();
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+    double acc[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
     const bool diag = (I == J);
     const int k0 = diag ? 1 : 0, k1 = diag ? 33 : kWave;
 #pragma unroll 1
@@ -152,23 +155,27 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
       for (int c = 0; c < 6; c++) {
         if (COUP == kScalar) {
           acc[c] += d[c];
-          atomicAdd(&ccol[c][jl], -d[c]);
+          col[c] = rot_down(col[c] - d[c]);
         } else {
           acc[c] += wi * d[c];
-          atomicAdd(&ccol[c][jl], -wj * d[c]);
+          col[c] = rot_down(col[c] - wj * d[c]);
         }
       }
     }
     const int ldp = S.nt * kWave;
     double* P = part + S.poff;
     if (diag) {
+      // lane l holds the column sum of particle (l + 33) mod 64; particle l's is in lane l - 33
 #pragma unroll
-      for (int c = 0; c < 6; c++) P[((size_t)I * 6 + c) * ldp + il] = acc[c] + ccol[c][lane];
+      for (int c = 0; c < 6; c++) {
+        const double cl = __shfl(col[c], (lane + 31) & (kWave - 1), kWave);
+        P[((size_t)I * 6 + c) * ldp + il] = acc[c] + cl;
+      }
     } else {
 #pragma unroll
       for (int c = 0; c < 6; c++) {
         P[((size_t)J * 6 + c) * ldp + il] = acc[c];
-        P[((size_t)I * 6 + c) * ldp + j0 + lane] = ccol[c][lane];
+        P[((size_t)I * 6 + c) * ldp + j0 + lane] = col[c];
       }
     }
   }
