@@ -95,6 +95,19 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
                 tflops=flops / (kms / 1e3) / 1e12, dE_rel=abs(e_1 - e_0) / abs(e_0), dE_steps=steps)
 
 
+def measured_traffic(mode: str, n: int):
+    """HBM bytes per pair-kernel launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes (profiles/hbm_traffic_plummer65536.txt), for the N=65536 workload only."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_plummer65536.txt")
+    if n != 65536 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and not line.startswith("#") and f[0] == mode:
+            return int(f[1]) + int(f[2])
+    return None
+
+
 def cpu_baseline(st_host, eps, dt, itr, budget_s: float):
     """Reference Fortran CPU pair loop (bia.f90:142-196 with the reference's find_* routines),
     single thread, rows [0, r) of the same system until ~budget_s seconds have elapsed."""
@@ -192,7 +205,7 @@ def main():
     # roofline of the dominant kernel (the pair kernel of the reported mode), its own algorithm's flops
     achieved = head["tflops"]
     roof = {"bound": "mfma", "pipe": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(head_name, n),
             "flops_per_launch": head["flops"], "kernel_ms": head["kernel_ms"], "kernel": f"pair kernel, {head_name} mode"}
     cpu = None if args.no_cpu else cpu_baseline(st, eps, args.dt, args.itr, args.cpu_budget)
     out = {
