@@ -373,7 +373,8 @@ BIA_HDF void kep_setup(double eps, double dt, const double rel[6], KepPair& K) {
   K.dt = dt;
   K.tau = dt;
   if (K.beta > 0) {  // bound orbit: whole periods P = 2 pi mu / beta^1.5 return the pair to its start
-    const double P = 2 * kPi * K.mu / (K.beta * sqrt(K.beta));
+    const double rb = fast_rsqrt(K.beta);
+    const double P = 2 * kPi * K.mu * rb * rb * rb;
     if (fabs(dt) >= P) K.tau = dt - P * floor(dt / P);
   }
 }
@@ -404,8 +405,29 @@ BIA_HDF double kep_target_scale(const double a[3]) {
   return m * m * m;
 }
 
+// One Halley step on t(s) = tau at the point g: returns the step d; *est = the error left after
+// it, |C^2 - t3 / (6 t1)| d^3 with C = t2 / (2 t1) (t1, t2, t3 the derivatives of t); *valid when
+// the step is usable (finite, t1 > 0, positive Halley denominator); *F = t(s) - tau.
+BIA_HDF double kep_halley(const KepPair& K, const KepG& g, double* F, double* est, bool* valid) {
+  const double s = g.s;
+  const double f = K.r0 * s + K.eta * g.G2 + K.zeta * g.G3 - K.tau;
+  const double F1 = K.r0 + K.eta * g.G1 + K.zeta * g.G2;
+  const double F2 = K.eta * g.c0 + K.zeta * g.G1;
+  const double F3 = K.zeta * g.c0 - K.beta * K.eta * g.G1;
+  const double iF1 = fast_rcp(F1);
+  const double den = F1 - 0.5 * f * iF1 * F2;
+  const double d = -f * fast_rcp(den);
+  const double C = 0.5 * F2 * iF1;
+  const double ad = fabs(d);
+  *F = f;
+  *est = fabs(C * C - F3 * iF1 * (1.0 / 6)) * ad * ad * ad;
+  *valid = isfinite(s + d) && F1 > 0 && den > 0;
+  return d;
+}
+
 // Halley iteration on t(s) = tau inside a bracket; returns s, *evals = evaluations of t, and in
-// *last the G functions at the last point where t was evaluated
+// *last the G functions at the last point where t was evaluated.  The first step is taken apart
+// from the loop: nearly every pair converges in it, and it needs no bracket bookkeeping.
 BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool* ok, KepG* last) {
   constexpr double kInf = __builtin_huge_val();
   double lo = -kInf, hi = kInf;
@@ -413,11 +435,30 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
   else hi = 0;
   if (!(s > lo && s < hi)) s = K.tau * K.ir0;
   *ok = false;
-  int k = 0;
+  {
+    const KepG g = kep_g(K.beta, s);
+    *last = g;
+    double F, est;
+    bool valid;
+    const double d = kep_halley(K, g, &F, &est, &valid);
+    const double sn = s + d;
+    if ((valid && est <= 1e-16 * fabs(sn) && sn >= lo && sn <= hi) || F == 0) {
+      *ok = true;
+      *evals = 1;
+      return F == 0 ? s : sn;
+    }
+    if (F < 0) lo = s;
+    else hi = s;
+    s = (valid && sn > lo && sn < hi) ? sn : (hi == kInf) ? 2 * s + K.ir0 * fabs(K.tau)
+                                         : (lo == -kInf) ? 2 * s - K.ir0 * fabs(K.tau) : 0.5 * (lo + hi);
+  }
+  int k = 1;
   while (k < kmax) {
     const KepG g = kep_g(K.beta, s);
     *last = g;
-    const double F = K.r0 * s + K.eta * g.G2 + K.zeta * g.G3 - K.tau;
+    double F, est;
+    bool valid;
+    const double d = kep_halley(K, g, &F, &est, &valid);
     k++;
     if (F == 0) {
       *ok = true;
@@ -429,23 +470,11 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
       *ok = true;
       break;
     }
-    const double F1 = K.r0 + K.eta * g.G1 + K.zeta * g.G2;
-    const double F2 = K.eta * g.c0 + K.zeta * g.G1;
-    const double F3 = K.zeta * g.c0 - K.beta * K.eta * g.G1;
-    const double iF1 = fast_rcp(F1);
-    const double den = F1 - 0.5 * F * iF1 * F2;
-    const double d = -F * fast_rcp(den);
     const double sn = s + d;
-    const bool valid = isfinite(sn) && F1 > 0 && den > 0;
-    if (valid) {
-      const double C = 0.5 * F2 * iF1;
-      const double ad = fabs(d);
-      const double est = fabs(C * C - F3 * iF1 * (1.0 / 6)) * ad * ad * ad;
-      if (est <= 1e-16 * fabs(sn)) {  // converged; a root at an end of the bracket may round just outside it
-        s = fmin(fmax(sn, lo), hi);
-        *ok = true;
-        break;
-      }
+    if (valid && est <= 1e-16 * fabs(sn)) {  // converged; a root at an end of the bracket may round just outside it
+      s = fmin(fmax(sn, lo), hi);
+      *ok = true;
+      break;
     }
     if (valid && sn > lo && sn < hi) {
       s = sn;
@@ -484,13 +513,14 @@ BIA_HDF void kep_output(const KepPair& K, double s, const KepG& e, const double 
   }
   const double rn = K.r0 + K.eta * G1 + K.zeta * G2;
   const double irn = fast_rcp(rn);
-  const double fm1 = -K.mu * G2 * K.ir0;
-  const double gmdt = (K.tau - K.dt) - K.mu * G3;
-  const double fd = -K.mu * G1 * K.ir0 * irn;
-  const double gdm1 = -K.mu * G2 * irn;
+  const double hmu = -0.5 * K.mu;  // the half split folded into the Lagrange coefficients
+  const double fm1 = hmu * G2 * K.ir0;
+  const double gmdt = 0.5 * (K.tau - K.dt) + hmu * G3;
+  const double fd = hmu * G1 * K.ir0 * irn;
+  const double gdm1 = hmu * G2 * irn;
   for (int q = 0; q < 3; q++) {
-    d[q] = 0.5 * (fm1 * rel[q] + gmdt * rel[3 + q]);
-    d[3 + q] = 0.5 * (fd * rel[q] + gdm1 * rel[3 + q]);
+    d[q] = fm1 * rel[q] + gmdt * rel[3 + q];
+    d[3 + q] = fd * rel[q] + gdm1 * rel[3 + q];
   }
 }
 

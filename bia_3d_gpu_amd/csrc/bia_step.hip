@@ -31,6 +31,11 @@ namespace bia {
 
 constexpr int kWave = 64;
 
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
 __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -78,8 +83,9 @@ __device__ __forceinline__ double rot_down(double x) {
 // OCC: minimum waves per SIMD asked of the register allocator (1 = its own choice; 4 = at most 128
 // VGPRs; 5 = at most 96, for the Kepler-drift modes); picked per shape by the autotuner.
 //
-// One wave per tile.  Lane l holds row i = 64 I + l; the column tile's states are staged in LDS.
-// At step k lane l takes column j = (l + k) mod 64, so the 64 lanes work on 64 different
+// One wave per tile.  Lane l holds row i = 64 I + l.  Plain mode walks the columns in order
+// (below).  The Kepler modes stage the column tile's states in LDS and at step k lane l takes
+// column j = (l + k) mod 64, so the 64 lanes work on 64 different
 // columns and no column needs a cross-lane reduction: the column sums travel with their column,
 // one lane down per step (rot_down), and after 64 steps lane l holds column l's sum.  A diagonal
 // tile (I = J) takes steps k = 1..32 (at k = 32 lanes 0..31 only), which covers each unordered
@@ -114,6 +120,56 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
     }
     const int j0 = J * kWave;
     const int jn = min(kWave, S.n - j0);
+    const int ldp = S.nt * kWave;
+    double* P = part + S.poff;
+    if constexpr (MODE == kPlain) {
+      // reference solvers: divergent and register-heavy, so the columns are walked in order (j
+      // wave-uniform, its state through scalar loads) and each column sum is a wave reduction
+      double acc[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+      for (int jj = 0; jj < jn; ++jj) {
+        const int gj = S.off + j0 + jj;
+        double rj[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) rj[c] = st[(size_t)c * lds + gj];
+        double d[6] = {0, 0, 0, 0, 0, 0};
+        double wi = 1, wj = 1;
+        if (row_ok && (I != J || lane < jj)) {
+          double eps = S.eps;
+          if (COUP != kScalar) {
+            const double mj = mass ? mass[gj] : 1.0;
+            if (COUP == kGravity) eps = -coup * (mi + mj);
+            else eps = coup * qi * charge[gj] * (1 / mi + 1 / mj);
+            wi = 2 * mj / (mi + mj);
+            wj = 2 * mi / (mi + mj);
+          }
+          double rel[6];
+#pragma unroll
+          for (int c = 0; c < 6; c++) rel[c] = ri[c] - rj[c];
+          int br = 0, it = 0;
+          unsigned fl = 0;
+          pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T);
+          if (STATS) ps.add(br, it, fl);
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+          acc[c] += wi * d[c];
+          const double sum = wave_sum(wj * d[c]);
+          if (lane == jj) col[c] = -sum;
+        }
+      }
+      if (I == J) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) P[((size_t)I * 6 + c) * ldp + il] = acc[c] + col[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+          P[((size_t)J * 6 + c) * ldp + il] = acc[c];
+          P[((size_t)I * 6 + c) * ldp + j0 + lane] = col[c];
+        }
+      }
+      continue;
+    }
     __syncthreads();  // the previous tile's readers are done with the LDS column tile
     {
       const bool ok = lane < jn;
@@ -162,8 +218,6 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
         }
       }
     }
-    const int ldp = S.nt * kWave;
-    double* P = part + S.poff;
     if (diag) {
       // lane l holds the column sum of particle (l + 33) mod 64; particle l's is in lane l - 33
 #pragma unroll
