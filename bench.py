@@ -95,6 +95,32 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
                 tflops=flops / (kms / 1e3) / 1e12, dE_rel=abs(e_1 - e_0) / abs(e_0), dE_steps=steps)
 
 
+def run_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 512):
+    """SURVEY config 3 as a side measurement: 512 independent Plummer systems (a = 1, M = 1, G = 1,
+    m = 1/N_s, each from its own seed), N_s log-uniform in [64, 4096], dt = 1e-3, in one
+    persistent launch per step; the steps are replayed from a captured hipGraph."""
+    import torch
+
+    from bia_3d_gpu_amd import ic
+
+    sizes = ic.ensemble_sizes(512, 64, 4096, seed).astype(int)
+    st = np.ascontiguousarray(np.concatenate([ic.plummer_system(int(n), seed + 1 + k) for k, n in enumerate(sizes)],
+                                             axis=1))
+    eps = [-2.0 / n for n in sizes]
+    s = torch.from_numpy(st).to("cuda:0")
+    d = torch.empty_like(s)
+    ctx.autotune_(s, d, sizes, eps=eps, dt=1e-3, mode=mode)
+    ctx.run_steps_(s, d, sizes, max(warmup, 1), eps=eps, dt=1e-3, mode=mode, use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.run_steps_(s, d, sizes, steps, eps=eps, dt=1e-3, mode=mode, use_graph=True)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    pairs = float(sum(int(n) * (int(n) - 1) // 2 for n in sizes))
+    return {"systems": 512, "particles": int(sizes.sum()), "pairs_per_step": pairs, "ms_per_step": sec * 1e3,
+            "pairs_per_s": pairs / sec, "steps": steps, "graph": True}
+
+
 def measured_traffic(mode: str, n: int):
     """HBM bytes per pair-kernel launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     passes (profiles/hbm_traffic_plummer65536.txt), for the N=65536 workload only."""
@@ -145,6 +171,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learn", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 512-system ensemble side measurement")
     ap.add_argument("--modes", default="plain,guess,learned",
                     help="comma-separated subset of plain,guess,learned (profiling runs)")
     ap.add_argument("--copy-mb", type=float, default=0.0,
@@ -199,6 +226,13 @@ def main():
         res["learned"] = r
         log(f"learned: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms ({r['tflops']:.1f} TFLOP/s), "
             f"|dE/E0| {r['dE_rel']:.2e}, tuned {r['tuned']}")
+    ensemble = None
+    if not args.no_ensemble:
+        emode = MODE_LEARN if "learned" in res else MODE_GUESS
+        ensemble = run_ensemble(ctx, emode, max(args.steps, 3), args.warmup)
+        ensemble["mode"] = "learned" if emode == MODE_LEARN else "guess"
+        log(f"ensemble (512 systems, {ensemble['particles']} particles): {ensemble['ms_per_step']:.2f} ms/step, "
+            f"{ensemble['pairs_per_s']:.3e} pairs/s")
     plain = res.get("plain")
     head_name = "learned" if "learned" in res else "guess" if "guess" in res else "plain"
     head = res[head_name]
@@ -224,6 +258,7 @@ def main():
         "speedup_vs_plain": None if plain is None else {k: v["pairs_per_s"] / plain["pairs_per_s"]
                                                         for k, v in res.items()},
         "training": train,
+        "ensemble": ensemble,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

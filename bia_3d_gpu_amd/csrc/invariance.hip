@@ -5,7 +5,8 @@
 //
 // The O(N^2) potential term is computed over 64x64 tiles of the i<j triangle (one wave per
 // tile, lane = row, column particle wave-uniform).  Every lane accumulates in double-double
-// (TwoSum), tile partials are written to a buffer and summed in a fixed order, so the result
+// (TwoSum), tile partials are written to a buffer and summed in a fixed order (256 blocks over
+// contiguous chunks, then one wave), so the result
 // is deterministic and close to the quad-precision sum of Invariance_QPU.f90 rather than to
 // the plain double sum.  The ten O(N) sums use the same double-double scheme.
 #include "bia_runtime.hpp"
@@ -89,6 +90,25 @@ __global__ __launch_bounds__(256) void k_inv_linear(int n, int ld, const double*
   }
 }
 
+// first stage of the potential's reduction: block b sums tile partials [b*chunk, (b+1)*chunk) in a
+// fixed order (thread stride, then the block's waves in order) into out[b]
+__global__ __launch_bounds__(256) void k_inv_pe_reduce(int ntile, int chunk, const DD* __restrict__ pe,
+                                                       DD* __restrict__ out) {
+  __shared__ DD red[4];
+  DD a{0, 0};
+  const int b0 = blockIdx.x * chunk, b1 = min(ntile, b0 + chunk);
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) dd_add(a, pe[b]);
+  a = dd_wave_sum(a);
+  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) red[wv] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    DD r = red[0];
+    for (int w2 = 1; w2 < 4; w2++) dd_add(r, red[w2]);
+    out[blockIdx.x] = r;
+  }
+}
+
 // single-wave final reduction in a fixed order; out = KE PE Lx Ly Lz Ua Va Wa Xa Ya Za
 __global__ __launch_bounds__(64) void k_inv_final(int nlin, const DD* __restrict__ lin, int ntile,
                                                   const DD* __restrict__ pe, double eps, double* __restrict__ out) {
@@ -120,17 +140,21 @@ static int invariants_locked(bia_ctx* ctx, int n, double eps, const double* d_st
   if (n < 1 || ld < n || !d_state || !d_out) return BIA_ERR_ARG;
   const int nt = (n + 63) / 64;
   const size_t ntiles = (size_t)nt * (nt + 1) / 2;
-  const int nlin = 256;
+  const int nlin = 256, npe = 256;
   BIA_TRY(ctx, ensure_tiles(ctx, nt));
-  BIA_TRY(ctx, ctx->inv_part.ensure((ntiles + (size_t)nlin * 10) * sizeof(DD)));
+  BIA_TRY(ctx, ctx->inv_part.ensure((ntiles + (size_t)nlin * 10 + npe) * sizeof(DD)));
   DD* pe = ctx->inv_part.as<DD>();
   DD* lin = pe + ntiles;
+  DD* pe2 = lin + (size_t)nlin * 10;
   hipLaunchKernelGGL(k_inv_pe_tiles, dim3((unsigned)ntiles), dim3(64), 0, ctx->stream, n, ld, d_state,
                      ctx->tiles.as<int2>(), pe);
   BIA_HIP_TRY(ctx, hipGetLastError());
+  const int chunk = (int)((ntiles + npe - 1) / npe);
+  hipLaunchKernelGGL(k_inv_pe_reduce, dim3(npe), dim3(256), 0, ctx->stream, (int)ntiles, chunk, pe, pe2);
+  BIA_HIP_TRY(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_inv_linear, dim3(nlin), dim3(256), 0, ctx->stream, n, ld, d_state, lin);
   BIA_HIP_TRY(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_inv_final, dim3(1), dim3(64), 0, ctx->stream, nlin, lin, (int)ntiles, pe, eps, d_out);
+  hipLaunchKernelGGL(k_inv_final, dim3(1), dim3(64), 0, ctx->stream, nlin, lin, npe, pe2, eps, d_out);
   BIA_HIP_TRY(ctx, hipGetLastError());
   return BIA_OK;
 }
