@@ -442,7 +442,7 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
     bool valid;
     const double d = kep_halley(K, g, &F, &est, &valid);
     const double sn = s + d;
-    if ((valid && est <= 1e-16 * fabs(sn) && sn >= lo && sn <= hi) || F == 0) {
+    if ((valid && est <= 1e-16 * fabs(sn) && sn * K.tau >= 0) || F == 0) {  // root on the side of s = 0 that tau is
       *ok = true;
       *evals = 1;
       return F == 0 ? s : sn;

@@ -190,7 +190,8 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
       const int jl = (lane + k) & (kWave - 1);
       double d[6] = {0, 0, 0, 0, 0, 0};
       double wi = 1, wj = 1;  // multiples of the half deltas credited to i and debited from j
-      if (row_ok && jl < jn && (!diag || k < 32 || lane < 32)) {
+      const bool active = row_ok & (jl < jn) & (!diag | (k < 32) | (lane < 32));
+      if (active) {
         double eps = S.eps;
         if (COUP != kScalar) {
           const double mj = cmq[0][jl];
