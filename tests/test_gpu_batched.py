@@ -44,8 +44,9 @@ def test_gravity_coupling_equal_masses_is_the_reference_step(ctx):
     assert np.array_equal(d.cpu().numpy(), ctx.step_host(st, -2.0 / n, 1e-3))
 
 
+@pytest.mark.parametrize("mode", [0, 2])
 @pytest.mark.parametrize("kind", ["gravity", "coulomb"])
-def test_coupled_step_vs_oracle(ctx, kind):
+def test_coupled_step_vs_oracle(ctx, kind, mode):
     import torch
     from bia_3d_gpu_amd import ic, COUPLING_GRAVITY, COUPLING_COULOMB
     n = 512
@@ -62,7 +63,7 @@ def test_coupled_step_vs_oracle(ctx, kind):
     tm = None if m is None else torch.from_numpy(m).cuda()
     tq = None if q is None else torch.from_numpy(q).cuda()
     stats = torch.zeros(16, dtype=torch.int64, device="cuda")
-    ctx.step_batched_(s, d, [n], dt=dt, coupling=coupling, coup=coup, mass=tm, charge=tq, stats=stats)
+    ctx.step_batched_(s, d, [n], dt=dt, coupling=coupling, coup=coup, mass=tm, charge=tq, stats=stats, mode=mode)
     torch.cuda.synchronize()
     got = d.cpu().numpy()
     ref = O.bia_step_coupled(st, coupling, coup, dt, mass=m, charge=q)
@@ -70,7 +71,7 @@ def test_coupled_step_vs_oracle(ctx, kind):
     ok, rp, rv = O.within_condition_bound(st, got, ref, coup, bounds=b)
     sc = stats.cpu().numpy()
     skipped = O.indeterminate(b)
-    print(kind, "branches", sc[:4], "flags", sc[8:14], f"error/allowed {rp:.2f} {rv:.2f}",
+    print(kind, "mode", mode, "branches", sc[:4], "flags", sc[8:14], f"error/allowed {rp:.2f} {rv:.2f}",
           f"particles excluded (a pair of indeterminate type or one the reference gets wrong): {skipped}")
     assert ok
     assert skipped <= 3 * n // 100  # pairs the reference gets wrong or cannot determine are rare
@@ -133,4 +134,36 @@ def test_batched_rkf(ctx):
     for st, n, e in zip(states, sizes, eps):
         ref = O.bia_step(st, e, 1e-2, method=0)
         assert O.within_condition_bound(st, got[:, off:off + n], ref, e, 1e-2, method=0)[0]
+        off += n
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_mass_spectrum_ensemble_vs_oracle(ctx, mode):
+    """Config-5 style ensemble at test size: several Plummer systems with log-uniform masses in
+    [0.1, 10] (normalised to total mass 1 per system), gravity coupling eps_ij = -(m_i + m_j),
+    centre-of-mass split, one batched launch; every system against the oracle's coupled step."""
+    import torch
+    from bia_3d_gpu_amd import ic, COUPLING_GRAVITY
+    sizes = [int(n) for n in ic.ensemble_sizes(5, 64, 700, seed=64)]
+    rng = np.random.default_rng(65)
+    states, masses = [], []
+    for k, n in enumerate(sizes):
+        states.append(ic.plummer_system(n, seed=70 + k))
+        m = np.exp(rng.uniform(np.log(0.1), np.log(10), n))
+        masses.append(m / m.sum())
+    st = _cat(states)
+    m_all = np.concatenate(masses)
+    s = torch.from_numpy(st).cuda()
+    d = torch.empty_like(s)
+    ctx.step_batched_(s, d, sizes, dt=5e-4, coupling=COUPLING_GRAVITY, coup=1.0,
+                      mass=torch.from_numpy(m_all).cuda(), mode=mode)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    off = 0
+    for st_k, m_k, n in zip(states, masses, sizes):
+        ref = O.bia_step_coupled(st_k, COUPLING_GRAVITY, 1.0, 5e-4, mass=m_k)
+        b = O.sensitivity(st_k, 1.0, 5e-4, coupling=COUPLING_GRAVITY, mass=m_k)
+        ok, rp, rv = O.within_condition_bound(st_k, got[:, off:off + n], ref, 1.0, bounds=b)
+        print(f"system n={n}: error/allowed {rp:.3f} {rv:.3f}; excluded {O.indeterminate(b)}")
+        assert ok and O.indeterminate(b) <= 3 * n // 100
         off += n
