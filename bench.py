@@ -121,6 +121,38 @@ def run_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 512):
             "pairs_per_s": pairs / sec, "steps": steps, "graph": True}
 
 
+def run_fixture(ctx, steps: int):
+    """Side measurement on the reference's own N = 1332 input (tests/golden/0001332_10^0.bbb), the
+    only case with published reference timings (one BIA step, dt = 1, eps = -6e-7: 0.193 s of
+    GPU solver time on a TITAN V, 4.59e6 pair-interactions/s; the DIM RKF45 run to t = 1: 2.946 s
+    total).  Times the plain mode (the reference's algorithm), the guess mode, and the DIM run."""
+    import torch
+
+    from bia_3d_gpu_amd import MODE_GUESS, MODE_PLAIN, io
+
+    st = io.read_bbb(os.path.join(ROOT, "tests", "golden", "0001332_10^0.bbb"))[0]
+    n = st.shape[1]
+    pairs = n * (n - 1) / 2
+    out = {"n": n, "pairs_per_step": pairs, "reference_published": {"bia_alg_gpu_solver_s": 0.193,
+                                                                     "dim_total_s": 2.946}}
+    s = torch.from_numpy(np.ascontiguousarray(st)).to("cuda:0")
+    d = torch.empty_like(s)
+    for name, mode in (("plain", MODE_PLAIN), ("guess", MODE_GUESS)):
+        ctx.step_(s, d, -6e-7, 1.0, mode=mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.step_(s, d, -6e-7, 1.0, mode=mode)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        out[name] = {"s_per_step": sec, "pairs_per_s": pairs / sec}
+    y = np.ascontiguousarray(st.T).reshape(-1)
+    t0 = time.perf_counter()
+    r = ctx.dim_rkf45(y, -3e-7, 0.0, 1.0)
+    out["dim"] = {"s": time.perf_counter() - t0, "accepted_steps": int(r["naccept"]), "ier": int(r["ier"])}
+    return out
+
+
 def measured_traffic(mode: str, n: int):
     """HBM bytes per pair-kernel launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     passes (profiles/hbm_traffic_plummer65536.txt), for the N=65536 workload only."""
@@ -172,6 +204,7 @@ def main():
     ap.add_argument("--no-learn", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 512-system ensemble side measurement")
+    ap.add_argument("--no-fixture", action="store_true", help="skip the N=1332 reference-fixture side measurement")
     ap.add_argument("--modes", default="plain,guess,learned",
                     help="comma-separated subset of plain,guess,learned (profiling runs)")
     ap.add_argument("--copy-mb", type=float, default=0.0,
@@ -233,6 +266,11 @@ def main():
         ensemble["mode"] = "learned" if emode == MODE_LEARN else "guess"
         log(f"ensemble (512 systems, {ensemble['particles']} particles): {ensemble['ms_per_step']:.2f} ms/step, "
             f"{ensemble['pairs_per_s']:.3e} pairs/s")
+    fixture = None if args.no_fixture else run_fixture(ctx, 20)
+    if fixture:
+        log(f"reference fixture N=1332: plain {fixture['plain']['s_per_step'] * 1e3:.3f} ms/step, guess "
+            f"{fixture['guess']['s_per_step'] * 1e3:.3f} ms/step, DIM {fixture['dim']['s']:.3f} s "
+            f"({fixture['dim']['accepted_steps']} steps)")
     plain = res.get("plain")
     head_name = "learned" if "learned" in res else "guess" if "guess" in res else "plain"
     head = res[head_name]
@@ -259,6 +297,7 @@ def main():
                                                         for k, v in res.items()},
         "training": train,
         "ensemble": ensemble,
+        "reference_fixture": fixture,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

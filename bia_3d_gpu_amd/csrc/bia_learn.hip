@@ -86,7 +86,7 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
   return x;
 }
 
-// sample rows: [x1, v, b (0 attractive / 1 repulsive), t]
+// sample rows: [u1, u2, u3, target]
 struct Batch {
   const float* samples;
   int nsamples;
@@ -134,18 +134,18 @@ __global__ __launch_bounds__(64) void k_mlp(const float* __restrict__ w, Batch B
   for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
     const int i = tile * 64 + l;
     const bool ok = i < B.batch;
-    float x1 = 0, v = 0, br = 0, tt = 0;  // the three features u1 u2 u3 and the target
+    float u1 = 0, u2 = 0, u3 = 0, tt = 0;  // the three features and the target
     if (ok) {
       const float* s = B.samples + (size_t)batch_row(B, i) * 4;
-      x1 = s[0];
-      v = s[1];
-      br = s[2];
+      u1 = s[0];
+      u2 = s[1];
+      u3 = s[2];
       tt = s[3];
     }
-    sXf[l * 4 + 0] = x1;
-    sXf[l * 4 + 1] = v;
-    sXf[l * 4 + 2] = br;
-    for (int k = 0; k < 16; k++) sX[l * 16 + k] = (bf16)(k == 0 ? x1 : k == 1 ? v : k == 2 ? br : 0.f);
+    sXf[l * 4 + 0] = u1;
+    sXf[l * 4 + 1] = u2;
+    sXf[l * 4 + 2] = u3;
+    for (int k = 0; k < 16; k++) sX[l * 16 + k] = (bf16)(k == 0 ? u1 : k == 1 ? u2 : k == 2 ? u3 : 0.f);
     __syncthreads();
     // layer 1: Z1 = X W1^T + b1
     Acc acc;

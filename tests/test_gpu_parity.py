@@ -127,6 +127,35 @@ def test_kepler_mode_vs_oracle(ctx, fixture1332, case):
     assert evals < 1.5
 
 
+@pytest.mark.parametrize("mode", [0, 2])
+def test_tiny_and_ragged_systems(ctx, mode):
+    """N = 1 (no pair: the step is the drift alone), 2, 3 and sizes around the 64-particle tile,
+    alone and batched together in one launch."""
+    import torch
+    from bia_3d_gpu_amd import ic
+    sizes = [1, 2, 3, 63, 64, 65, 127, 129]
+    states = [ic.uniform_ball_virial(max(n, 2), seed=90 + n)[:, :n].copy() for n in sizes]
+    dt = 1e-3
+    for st, n in zip(states, sizes):
+        eps = -2.0 / max(n, 2)
+        d = ctx.step_host(st, eps, dt, mode=mode)
+        ref = O.bia_step(st, eps, dt)
+        if n == 1:
+            assert np.array_equal(d[:3], st[3:] * dt) and np.all(d[3:] == 0)
+            continue
+        ok, rp, rv = O.within_condition_bound(st, d, ref, eps, dt)
+        assert ok, (n, rp, rv)
+    s_ = torch.from_numpy(np.ascontiguousarray(np.concatenate(states, axis=1))).cuda()
+    d_ = torch.empty_like(s_)
+    ctx.step_batched_(s_, d_, sizes, eps=[-2.0 / max(n, 2) for n in sizes], dt=dt, mode=mode)
+    torch.cuda.synchronize()
+    got = d_.cpu().numpy()
+    off = 0
+    for st, n in zip(states, sizes):
+        assert np.array_equal(got[:, off:off + n], ctx.step_host(st, -2.0 / max(n, 2), dt, mode=mode))
+        off += n
+
+
 def test_invariants_vs_quad(ctx, fixture1332):
     st, eps = fixture1332["state"], fixture1332["eps"]
     g = ctx.invariants(st, eps)
