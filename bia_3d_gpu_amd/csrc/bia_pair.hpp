@@ -585,7 +585,7 @@ BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6]
 // ------------------------------------------------------------------------------------
 // ellipse: find_dXYUV_kernel.cu:242-364
 // ------------------------------------------------------------------------------------
-BIA_HDN void ellipse(double eps, double dt, int itr, const double yz[4], double yn[4], int* iters,
+BIA_HDF void ellipse(double eps, double dt, int itr, const double yz[4], double yn[4], int* iters,
                     unsigned* fl) {
   double X = yz[0], Y = yz[1], U = yz[2], V = yz[3];
   double g2 = U * U + V * V;
@@ -691,7 +691,7 @@ BIA_HDF double par_dtdth(double th, double rmg) {
   return rmg * t;
 }
 
-BIA_HDN void parabola(double eps, double dt, const double yz[4], double yn[4], int* iters) {
+BIA_HDF void parabola(double eps, double dt, const double yz[4], double yn[4], int* iters) {
   double r_ini = sqrt(yz[0] * yz[0] + yz[1] * yz[1]);
   double r_dot_g = yz[0] * yz[2] + yz[1] * yz[3];
   double r_crs_g = yz[0] * yz[3] - yz[1] * yz[2];
@@ -860,6 +860,32 @@ BIA_HD void rkf_pair(double eps, double dt, const double yz[4], double yn[4], in
 // one pair: frame, classifier (:503-506), 2D solve, half-split epilogue (:507-515)
 // rel = (r_i - r_j, g_i - g_j); d = the six half-deltas credited to i (j gets -d)
 // ------------------------------------------------------------------------------------
+// The rare conics are kept out of line, with their inputs and outputs passed by value: an array
+// whose address crosses a call stays in scratch memory on every path of the caller.
+struct Conic4 {
+  double y[4];
+  int iters;
+  unsigned fl;
+};
+
+BIA_HDN Conic4 ellipse_call(double eps, double dt, int itr, double y0, double y1, double y2, double y3) {
+  const double yz[4] = {y0, y1, y2, y3};
+  Conic4 o;
+  o.iters = 0;
+  o.fl = 0;
+  ellipse(eps, dt, itr, yz, o.y, &o.iters, &o.fl);
+  return o;
+}
+
+BIA_HDN Conic4 parabola_call(double eps, double dt, double y0, double y1, double y2, double y3) {
+  const double yz[4] = {y0, y1, y2, y3};
+  Conic4 o;
+  o.iters = 0;
+  o.fl = 0;
+  parabola(eps, dt, yz, o.y, &o.iters);
+  return o;
+}
+
 template <int METHOD, int MODE>
 BIA_HDF void pair_eval(double eps, double dt, int itr, const double rel[6], double d[6], int* branch, int* iters,
                        unsigned* fl, Hist hist, const LearnTable& T) {
@@ -880,12 +906,13 @@ BIA_HDF void pair_eval(double eps, double dt, int itr, const double rel[6], doub
         hyp_setup(eps, dt, yz, S, fl);
         const double th = hyp_solve_plain(S, itr, hist, &it, fl);
         hyp_finish(S, th, yz, yn);
-      } else if (W == 0) {
-        *branch = kParabola;
-        parabola(eps, dt, yz, yn, &it);
       } else {
-        *branch = kEllipse;
-        ellipse(eps, dt, itr, yz, yn, &it, fl);
+        *branch = (W == 0) ? kParabola : kEllipse;
+        const Conic4 o = (W == 0) ? parabola_call(eps, dt, yz[0], yz[1], yz[2], yz[3])
+                                  : ellipse_call(eps, dt, itr, yz[0], yz[1], yz[2], yz[3]);
+        for (int c = 0; c < 4; c++) yn[c] = o.y[c];
+        it = o.iters;
+        *fl |= o.fl;
       }
     }
     *iters = it;
