@@ -52,20 +52,19 @@ enum Flag : unsigned {
   kAcosClamp = 8u,     // |cos(theta_ini)| > 1 clamp taken (hyperbola)
   kRkfFail = 16u,      // per-pair RKF ended before reaching dt
   kNonFinite = 32u,    // non-finite output
-  kLearned = 64u,      // learned initial guess accepted
-  kFallback = 128u     // learned path fell back to the reference guess
+  kLearned = 64u       // learned start used (learned mode)
 };
 
-// kPlain: the reference's guess selection and iteration control.
 // kPlain: the reference's algorithm and iteration control.
 // kGuess: universal-variable Kepler drift from its series start (no network).
 // kLearn: the same drift; pairs that are not weak over dt start from the learned table.
 enum Mode : int { kPlain = 0, kLearn = 1, kGuess = 2 };
 
-// Learned-initialiser table: the network's output t(x1, v) for the two hyperbolic branches on a
-// regular grid, x1 in [-1, 1] (nx points), v in [0, 1] (nv points), bilinear interpolation.
+// Learned-initialiser table: the network's output (the scaled correction of the Kepler drift's
+// start, see kepler_pair) on a regular n^3 grid of the squashed features (u1, u2, u3) in
+// [-1, 1]^3, u1 fastest; trilinear interpolation.
 struct LearnTable {
-  const float* t;  // [n][n][n]: log(s / s0) over the squashed features (u1 fastest)
+  const float* t;
   int n;
 };
 

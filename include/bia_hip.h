@@ -51,11 +51,13 @@ extern "C" {
 
 /* per-step statistics written by the step entry points (uint64 counters, BIA_STATS_LEN long) */
 #define BIA_STATS_LEN 16
-/* [0..3]  pairs per branch (hyperbola, parabola, ellipse, rkf)
- * [4..7]  summed solver iterations per branch (Newton evaluations / accepted RKF steps)
+/* [0..3]  pairs per branch (hyperbola, parabola, ellipse, rkf); in the Kepler-drift modes the
+ *         branch is the sign of the energy (W > 0, = 0, < 0)
+ * [4..7]  summed solver iterations per branch (Newton evaluations / accepted RKF steps /
+ *         evaluations of the universal time equation)
  * [8..15] pairs with status bit b set: 1 best-iterate never updated, 2 exact-repeat exit,
- *         4 iteration cap, 8 acos clamp, 16 RKF failure, 32 non-finite, 64 learned guess used,
- *         128 learned path fell back to the reference guess */
+ *         4 iteration cap, 8 acos clamp, 16 RKF failure, 32 non-finite, 64 learned start used,
+ *         128 unused */
 
 typedef struct bia_ctx bia_ctx;
 
