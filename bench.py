@@ -17,8 +17,8 @@ pair kernel of that mode) with the algorithmic FP64 flop count of its own algori
 counters of an untimed step on the same input; bia_3d_gpu_amd.flops_per_step) over its average
 duration, measured with HIP events recorded on the kernel's own stream.  ``energy`` gives
 |dE/E0| over the timed steps of each mode from the library's double-double invariants.
-``cpu_baseline`` times the reference's own Fortran CPU pair loop (oracle/_ref, single thread)
-on a bounded row sample.
+``cpu_baseline`` times the reference's own Fortran CPU pair loop (oracle/_ref) on bounded row
+samples, on one core and on one process per host core (at most 16).
 """
 from __future__ import annotations
 
@@ -166,28 +166,50 @@ def measured_traffic(mode: str, n: int):
     return None
 
 
-def cpu_baseline(st_host, eps, dt, itr, budget_s: float):
-    """Reference Fortran CPU pair loop (bia.f90:142-196 with the reference's find_* routines),
-    single thread, rows [0, r) of the same system until ~budget_s seconds have elapsed."""
+def _cpu_rows(args):
+    """One CPU worker: the reference's Fortran pair loop on the given row ranges, until budget_s."""
+    st_host, eps, dt, itr, ranges, budget_s = args
     import oracle as O
 
     n = st_host.shape[1]
     kind = "reference" if O.ref_available() else "port"
-    rows, pairs, t = 0, 0, 0.0
-    chunk = max(1, int(2e6 / n))
-    while t < budget_s and rows < n:
-        r1 = min(n, rows + chunk)
+    pairs, t = 0, 0.0
+    for r0, r1 in ranges:
+        if t >= budget_s:
+            break
         t0 = time.perf_counter()
         if kind == "reference":
-            O.ref_cpu_rows(st_host, eps, dt, 1, itr, rows=(rows, r1))
+            O.ref_cpu_rows(st_host, eps, dt, 1, itr, rows=(r0, r1))
         else:
-            O.bia_step(st_host, eps, dt, 1, itr, rows=(rows, r1))
+            O.bia_step(st_host, eps, dt, 1, itr, rows=(r0, r1))
         t += time.perf_counter() - t0
-        pairs += sum(n - 1 - i for i in range(rows, r1))
-        rows = r1
-    return dict(value=pairs / t, unit="pair-interactions/s", cores=1, kind=kind,
-                sample=f"rows 0..{rows - 1} of the same N={n} system ({pairs} pairs, {t:.1f} s), "
-                       f"one step, {'amdflang -O3 build of the reference Fortran CPU routines' if kind == 'reference' else 'C oracle'}")
+        pairs += sum(n - 1 - i for i in range(r0, r1))
+    return pairs, t, kind
+
+
+def cpu_baseline(st_host, eps, dt, itr, budget_s: float):
+    """Reference Fortran CPU pair loop (bia.f90:142-196 with the reference's find_* routines) on
+    row chunks of the same system: one thread for budget_s / 2 seconds, then one process per host
+    core (at most 16) on interleaved chunks for another budget_s / 2.  Run before the GPU is
+    initialised (the workers are forked)."""
+    import multiprocessing as mp
+
+    n = st_host.shape[1]
+    chunk = max(1, int(2e6 / n))
+    ranges = [(r, min(n, r + chunk)) for r in range(0, n, chunk)]
+    p1, t1, kind = _cpu_rows((st_host, eps, dt, itr, ranges, budget_s / 2))
+    cores = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)))
+    work = [(st_host, eps, dt, itr, ranges[k::cores], budget_s / 2) for k in range(cores)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        out = pool.map(_cpu_rows, work)
+    wall = time.perf_counter() - t0
+    pairs = sum(o[0] for o in out)
+    what = "amdflang -O3 build of the reference Fortran CPU routines" if kind == "reference" else "C oracle"
+    return dict(value=pairs / wall, unit="pair-interactions/s", cores=cores, kind=kind,
+                single_core_value=p1 / t1,
+                sample=f"row chunks of the same N={n} system, one step, {what}: {p1} pairs on one core in "
+                       f"{t1:.1f} s; {pairs} pairs on {cores} processes in {wall:.1f} s wall")
 
 
 def main():
@@ -199,7 +221,7 @@ def main():
     ap.add_argument("--seed", type=int, default=65536)
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--itr", type=int, default=18)
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learn", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -221,12 +243,17 @@ def main():
 
     from bia_3d_gpu_amd import Context, MODE_GUESS, MODE_LEARN, MODE_PLAIN, learn
 
-    assert torch.cuda.is_available()
     n = args.n
     eps = -2.0 / n
     log(f"building state N={n} seed={args.seed}")
     st = make_state(n, args.seed)
     pairs = n * (n - 1) / 2
+    # CPU baseline first: its worker processes are forked before this process touches the GPU
+    cpu = None if args.no_cpu else cpu_baseline(st, eps, args.dt, args.itr, args.cpu_budget)
+    if cpu:
+        log(f"cpu baseline: {cpu['value']:.3e} pairs/s on {cpu['cores']} cores "
+            f"({cpu['single_core_value']:.3e} on one)")
+    assert torch.cuda.is_available()
     ctx = Context(0)
     modes = set(args.modes.split(","))
     if args.copy_mb > 0:  # a plain copy of known size for scaling HBM byte counters
@@ -279,7 +306,6 @@ def main():
     roof = {"bound": "mfma", "pipe": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(head_name, n),
             "flops_per_launch": head["flops"], "kernel_ms": head["kernel_ms"], "kernel": f"pair kernel, {head_name} mode"}
-    cpu = None if args.no_cpu else cpu_baseline(st, eps, args.dt, args.itr, args.cpu_budget)
     out = {
         "metric": METRIC, "value": head["pairs_per_s"], "unit": "pair-interactions/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
