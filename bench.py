@@ -123,9 +123,10 @@ def run_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 512):
 
 def run_fixture(ctx, steps: int):
     """Side measurement on the reference's own N = 1332 input (tests/golden/0001332_10^0.bbb), the
-    only case with published reference timings (one BIA step, dt = 1, eps = -6e-7: 0.193 s of
-    GPU solver time on a TITAN V, 4.59e6 pair-interactions/s; the DIM RKF45 run to t = 1: 2.946 s
-    total).  Times the plain mode (the reference's algorithm), the guess mode, and the DIM run."""
+    only case with published reference timings (one BIA step, dt = 1, eps = -6e-7, TITAN V: ALG
+    0.193 s and per-pair RKF 0.293 s of GPU solver time; the DIM RKF45 run to t = 1: 2.946 s
+    total).  Times the plain ALG mode (the reference's algorithm), the guess mode, the per-pair
+    RKF method and the DIM run."""
     import torch
 
     from bia_3d_gpu_amd import MODE_GUESS, MODE_PLAIN, io
@@ -134,17 +135,19 @@ def run_fixture(ctx, steps: int):
     n = st.shape[1]
     pairs = n * (n - 1) / 2
     out = {"n": n, "pairs_per_step": pairs, "reference_published": {"bia_alg_gpu_solver_s": 0.193,
+                                                                     "bia_rkf_gpu_solver_s": 0.293,
                                                                      "dim_total_s": 2.946}}
     s = torch.from_numpy(np.ascontiguousarray(st)).to("cuda:0")
     d = torch.empty_like(s)
-    for name, mode in (("plain", MODE_PLAIN), ("guess", MODE_GUESS)):
-        ctx.step_(s, d, -6e-7, 1.0, mode=mode)
+    for name, method, mode, k in (("plain", 1, MODE_PLAIN, steps), ("guess", 1, MODE_GUESS, steps),
+                                  ("rkf", 0, MODE_PLAIN, 2)):
+        ctx.step_(s, d, -6e-7, 1.0, method=method, mode=mode)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(steps):
-            ctx.step_(s, d, -6e-7, 1.0, mode=mode)
+        for _ in range(k):
+            ctx.step_(s, d, -6e-7, 1.0, method=method, mode=mode)
         torch.cuda.synchronize()
-        sec = (time.perf_counter() - t0) / steps
+        sec = (time.perf_counter() - t0) / k
         out[name] = {"s_per_step": sec, "pairs_per_s": pairs / sec}
     y = np.ascontiguousarray(st.T).reshape(-1)
     t0 = time.perf_counter()
@@ -296,8 +299,8 @@ def main():
     fixture = None if args.no_fixture else run_fixture(ctx, 20)
     if fixture:
         log(f"reference fixture N=1332: plain {fixture['plain']['s_per_step'] * 1e3:.3f} ms/step, guess "
-            f"{fixture['guess']['s_per_step'] * 1e3:.3f} ms/step, DIM {fixture['dim']['s']:.3f} s "
-            f"({fixture['dim']['accepted_steps']} steps)")
+            f"{fixture['guess']['s_per_step'] * 1e3:.3f} ms/step, RKF {fixture['rkf']['s_per_step'] * 1e3:.1f} "
+            f"ms/step, DIM {fixture['dim']['s']:.3f} s ({fixture['dim']['accepted_steps']} steps)")
     plain = res.get("plain")
     head_name = "learned" if "learned" in res else "guess" if "guess" in res else "plain"
     head = res[head_name]

@@ -2,12 +2,13 @@
 // (reference dim/rkf_kernel.cu:29-53 force, :75-145 Fehlberg stages, :194-495 stepper;
 // driven by dim/dim_3D.f90:30).
 //
-// Device layout: SoA rows x,y,z,u,v,w of stride ld.  One Fehlberg stage is one kernel:
-// thread i evaluates the direct-sum acceleration at the stage input w_{s-1} (j staged through
-// LDS 256 at a time, summed in ascending j like the reference), stores k_s[i] and in the
-// same pass forms particle i's next stage input w_s[i] = y0[i] + h sum_l b_sl k_l[i].  The
-// last stage forms y_n and, for the error-controlled step, the per-block maximum of
-// |err_i| / (relerr (|y0|+|yn|)/2 + abserr), so only nblocks doubles cross PCIe per trial step.
+// Device layout: SoA rows x,y,z,u,v,w of stride ld.  One Fehlberg stage is one kernel: a
+// workgroup of 16 waves evaluates the direct-sum acceleration of 64 particles at the stage input
+// w_{s-1} (each wave one contiguous range of j, the 16 partial sums added in wave order), stores
+// k_s[i] and in the same pass forms the particles' next stage input
+// w_s[i] = y0[i] + h sum_l b_sl k_l[i].  The last stage forms y_n and, for the error-controlled
+// step, the per-block maximum of |err_i| / (relerr (|y0|+|yn|)/2 + abserr), so only nblocks
+// doubles cross PCIe per trial step.
 #include <stdlib.h>
 #include <string.h>
 
@@ -18,7 +19,10 @@
 
 namespace bia {
 
-constexpr int kDimBlock = 256;
+constexpr int kDimRows = 64;   // particles per workgroup (one per lane)
+constexpr int kDimSplit = 16;  // waves per workgroup, each summing one contiguous range of j
+constexpr int kDimBlock = kDimRows * kDimSplit;
+constexpr int kDimPad = 256;   // row stride ld is a multiple of this
 
 struct DimBufs {
   double* y[4];   // yl0, yln, yh0, yhn (swapped by pointer)
@@ -28,36 +32,42 @@ struct DimBufs {
   double* bmax;   // per-block error maxima (+1 flag slot per block)
 };
 
+// direct-sum acceleration of the workgroup's 64 particles at the stage input w (rkf_kernel.cu:29-53:
+// a_i = sum_j eps (r_i - r_j) / (|r_i - r_j|^2 + 1e-99)^(3/2)).  Wave q sums its contiguous range of
+// j in ascending order (j is wave-uniform: scalar loads); the 16 partial sums are then added in
+// wave order, so the result is deterministic.  Valid in wave 0 only.
 __device__ __forceinline__ void dim_accel(int np, int ld, double eps, const double* __restrict__ w, int i,
                                           double& ax, double& ay, double& az) {
-  __shared__ double sx[kDimBlock], sy[kDimBlock], sz[kDimBlock];
+  __shared__ double part[kDimSplit][3][kDimRows];
+  const int lane = threadIdx.x % kDimRows, q = threadIdx.x / kDimRows;
   const bool ok = i < np;
   const double xi = ok ? w[i] : 0, yi = ok ? w[ld + i] : 0, zi = ok ? w[2 * (size_t)ld + i] : 0;
+  const int jc = (np + kDimSplit - 1) / kDimSplit;
+  const int j0 = q * jc, j1 = min(np, j0 + jc);
+  double sx = 0, sy = 0, sz = 0;
+  for (int j = j0; j < j1; ++j) {
+    const double xij = xi - w[j], yij = yi - w[ld + j], zij = zi - w[2 * (size_t)ld + j];
+    const double rsq = xij * xij + yij * yij + zij * zij + 1e-99;  // rkf_kernel.cu:44
+    const double e = eps / (rsq * sqrt(rsq));
+    sx += xij * e;
+    sy += yij * e;
+    sz += zij * e;
+  }
+  part[q][0][lane] = sx;
+  part[q][1][lane] = sy;
+  part[q][2][lane] = sz;
+  __syncthreads();
   ax = ay = az = 0;
-  for (int j0 = 0; j0 < np; j0 += kDimBlock) {
-    const int jl = j0 + threadIdx.x;
-    __syncthreads();
-    if (jl < np) {
-      sx[threadIdx.x] = w[jl];
-      sy[threadIdx.x] = w[ld + jl];
-      sz[threadIdx.x] = w[2 * (size_t)ld + jl];
-    }
-    __syncthreads();
-    const int jn = min(kDimBlock, np - j0);
-    if (ok) {
-      for (int jj = 0; jj < jn; ++jj) {
-        double xij = xi - sx[jj], yij = yi - sy[jj], zij = zi - sz[jj];
-        double rsq = xij * xij + yij * yij + zij * zij + 1e-99;  // rkf_kernel.cu:44
-        double e = eps / (rsq * sqrt(rsq));
-        ax += xij * e;
-        ay += yij * e;
-        az += zij * e;
-      }
+  if (q == 0) {
+    for (int p = 0; p < kDimSplit; p++) {
+      ax += part[p][0][lane];
+      ay += part[p][1][lane];
+      az += part[p][2][lane];
     }
   }
 }
 
-// stage S (1..6) of one Fehlberg step of size h from y0:
+// stage S (1..6) of one Fehlberg step of size h from y0 (wave 0 of each workgroup writes):
 //   k_S = f(win);  S < 6: wout = y0 + h sum_{l<=S} b_{S+1,l} k_l
 //                  S = 6: yn = y0 + h sum ga_l k_l; ERR: er, block max of |er|/et
 template <int S, bool ERR>
@@ -68,9 +78,10 @@ __global__ __launch_bounds__(kDimBlock) void k_dim_stage(int np, int ld, double 
                                                          double* __restrict__ er, double* __restrict__ bmax,
                                                          double relerr, double abserr) {
   using namespace fehl;
-  const int i = blockIdx.x * kDimBlock + threadIdx.x;
+  const int i = blockIdx.x * kDimRows + threadIdx.x % kDimRows;
   double ax, ay, az;
   dim_accel(np, ld, eps, win, i, ax, ay, az);
+  if (threadIdx.x >= kDimRows) return;  // waves 1.. only helped with the sums
   double emax = 0, bad = 0;
   if (i < np) {
     const size_t sl = (size_t)6 * ld;  // stride between stages
@@ -107,25 +118,14 @@ __global__ __launch_bounds__(kDimBlock) void k_dim_stage(int np, int ld, double 
     }
   }
   if (S == 6 && ERR) {
-    __shared__ double red[kDimBlock / 64][2];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       emax = fmax(emax, __shfl_xor(emax, off, 64));
       bad = fmax(bad, __shfl_xor(bad, off, 64));
     }
-    if ((threadIdx.x & 63) == 0) {
-      red[threadIdx.x / 64][0] = emax;
-      red[threadIdx.x / 64][1] = bad;
-    }
-    __syncthreads();
     if (threadIdx.x == 0) {
-      double m = red[0][0], b = red[0][1];
-      for (int q = 1; q < kDimBlock / 64; q++) {
-        m = fmax(m, red[q][0]);
-        b = fmax(b, red[q][1]);
-      }
-      bmax[2 * blockIdx.x] = m;
-      bmax[2 * blockIdx.x + 1] = b;
+      bmax[2 * blockIdx.x] = emax;
+      bmax[2 * blockIdx.x + 1] = bad;
     }
   }
 }
@@ -133,10 +133,10 @@ __global__ __launch_bounds__(kDimBlock) void k_dim_stage(int np, int ld, double 
 // derivative only (initial step-size estimate): f = (v, a) at y
 __global__ __launch_bounds__(kDimBlock) void k_dim_func(int np, int ld, double eps, const double* __restrict__ y,
                                                         double* __restrict__ f) {
-  const int i = blockIdx.x * kDimBlock + threadIdx.x;
+  const int i = blockIdx.x * kDimRows + threadIdx.x % kDimRows;
   double ax, ay, az;
   dim_accel(np, ld, eps, y, i, ax, ay, az);
-  if (i < np) {
+  if (threadIdx.x < kDimRows && i < np) {
     f[i] = y[3 * (size_t)ld + i];
     f[ld + i] = y[4 * (size_t)ld + i];
     f[2 * (size_t)ld + i] = y[5 * (size_t)ld + i];
@@ -178,8 +178,8 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
   const double EPSMIN = 1e-15;  // rkf_kernel.cu:201-202
   const int ITEMAX = 10000;
   if (np < 1 || !x0 || !y || !hio || !ier) return BIA_ERR_ARG;
-  const int ld = ((np + kDimBlock - 1) / kDimBlock) * kDimBlock;
-  const int nb = ld / kDimBlock;
+  const int ld = ((np + kDimPad - 1) / kDimPad) * kDimPad;
+  const int nb = (np + kDimRows - 1) / kDimRows;
   const size_t row = (size_t)6 * ld;
   // layout: yl0 yln yh0 yhn | k (6 stages) | w0 w1 | er | bmax
   const size_t ndbl = 4 * row + 6 * row + 2 * row + row + 2 * (size_t)nb;
