@@ -93,8 +93,9 @@ int bia_destroy(bia_ctx* c) {
   if (!c) return BIA_ERR_ARG;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  bia::DevBuf* bufs[] = {&c->part, &c->tiles, &c->state, &c->delta, &c->stats, &c->leg_in, &c->leg_out,
-                         &c->inv_state, &c->inv_part, &c->inv_out, &c->dim_work, &c->dim_ctl,
+  c->leg_in.release();
+  c->leg_out.release();
+  bia::DevBuf* bufs[] = {&c->part, &c->tiles, &c->state, &c->delta, &c->stats, &c->inv_state, &c->inv_part, &c->inv_out, &c->dim_work, &c->dim_ctl,
                          &c->mlp_w, &c->mlp_ws, &c->mlp_table};
   for (auto* b : bufs) b->release();
   c->plan_tiles.release();

@@ -59,6 +59,45 @@ struct DevBuf {
   }
 };
 
+// Pinned host memory (hipHostMalloc) that kernels read and write directly through its device
+// address: the per-particle reference entry point moves its small rows this way instead of with
+// two copies per call.
+struct HostBuf {
+  void* p = nullptr;
+  void* dev = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need) {
+    if (need <= bytes) return BIA_OK;
+    release();
+    if (need == 0) return BIA_OK;
+    if (hipHostMalloc(&p, need, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return BIA_ERR_OOM;
+    }
+    if (hipHostGetDevicePointer(&dev, p, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      release();
+      return BIA_ERR_HIP;
+    }
+    bytes = need;
+    return BIA_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = dev = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* host() const {
+    return static_cast<T*>(p);
+  }
+  template <class T>
+  T* device() const {
+    return static_cast<T*>(dev);
+  }
+};
+
 }  // namespace bia
 
 struct bia_ctx {
@@ -78,7 +117,7 @@ struct bia_ctx {
   hipGraphExec_t graph_exec = nullptr;
   std::vector<long long> graph_key;
   // legacy per-row buffers
-  bia::DevBuf leg_in, leg_out;
+  bia::HostBuf leg_in, leg_out;  // rows of the per-particle reference entry point
   // invariance workspace
   bia::DevBuf inv_state, inv_part, inv_out;
   // DIM workspace

@@ -656,24 +656,24 @@ void find_dxyuv_kernel_(int METHOD, int N, int I, double EPS, double DT, int ITR
   if ((s = ctx->leg_in.ensure(bytes)) != BIA_OK) legacy_fail("find_dxyuv_kernel_ allocation", s, ctx);
   if ((s = ctx->leg_out.ensure(bytes)) != BIA_OK) legacy_fail("find_dxyuv_kernel_ allocation", s, ctx);
   hipError_t e = hipSetDevice(ctx->device);
-  if (e == hipSuccess) e = hipMemcpyAsync(ctx->leg_in.p, RXYZUVW, bytes, hipMemcpyHostToDevice, ctx->stream);
+  // only rows j > i are read and produced (the reference copies both whole arrays each way and
+  // leaves the other rows of dXYZUVW unspecified): they pass through pinned host buffers that the
+  // kernel reads and writes directly, one launch and one synchronisation per call
   const int cnt = N - (i + 1);
+  const size_t off = (size_t)6 * (i + 1), rb = (size_t)6 * cnt * sizeof(double);
   if (e == hipSuccess && cnt > 0) {
+    memcpy(ctx->leg_in.host<double>() + off, RXYZUVW + off, rb);
     dim3 g((cnt + kWave - 1) / kWave), b(kWave);
     if (METHOD == 1)
-      hipLaunchKernelGGL(k_legacy_row<1>, g, b, 0, ctx->stream, N, i, EPS, DT, ITR, ctx->leg_in.as<double>(),
-                         ctx->leg_out.as<double>());
+      hipLaunchKernelGGL(k_legacy_row<1>, g, b, 0, ctx->stream, N, i, EPS, DT, ITR, ctx->leg_in.device<double>(),
+                         ctx->leg_out.device<double>());
     else
-      hipLaunchKernelGGL(k_legacy_row<0>, g, b, 0, ctx->stream, N, i, EPS, DT, ITR, ctx->leg_in.as<double>(),
-                         ctx->leg_out.as<double>());
+      hipLaunchKernelGGL(k_legacy_row<0>, g, b, 0, ctx->stream, N, i, EPS, DT, ITR, ctx->leg_in.device<double>(),
+                         ctx->leg_out.device<double>());
     e = hipGetLastError();
-    // only rows j > i are produced (the reference copies back the whole array, leaving the
-    // rest unspecified); copy back exactly those
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(dXYZUVW + (size_t)6 * (i + 1), ctx->leg_out.as<double>() + (size_t)6 * (i + 1),
-                         (size_t)6 * cnt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) memcpy(dXYZUVW + off, ctx->leg_out.host<double>() + off, rb);
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     snprintf(ctx->last_error, sizeof(ctx->last_error), "%s", hipGetErrorString(e));
     legacy_fail("find_dxyuv_kernel_", BIA_ERR_HIP, ctx);

@@ -32,6 +32,10 @@ def test_reference_bia_f90_on_library(tmp_path):
     p = subprocess.run([os.path.join(REF_BIN, "bia_gpu_alg")], cwd=run, capture_output=True, text=True, timeout=600)
     print(p.stdout[-3000:])
     assert p.returncode == 0, p.stderr[-2000:]
+    # the program's own solver timing (bia/bia.f90:212; 0.193 s on the reference's TITAN V)
+    timing = [ln for ln in p.stdout.splitlines() if "for BIA-ALG" in ln]
+    print("solver timing of the unmodified host program on libbia_hip:", timing)
+    assert timing
     (dt, got), = io.read_timed(str(out / "0001332_10^0.BIA"))
     (_, ref), = io.read_timed(os.path.join(GOLDEN, "0001332_10^0.BIA"))
     dp, dv = np.abs(got[:3] - ref[:3]).max(), np.abs(got[3:] - ref[3:]).max()
