@@ -96,12 +96,13 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
 
 
 def run_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 512):
-    """SURVEY config 3 as a side measurement: 512 independent Plummer systems (a = 1, M = 1, G = 1,
-    m = 1/N_s, each from its own seed), N_s log-uniform in [64, 4096], dt = 1e-3, in one
-    persistent launch per step; the steps are replayed from a captured hipGraph."""
+    """Side measurement: 512 independent Plummer systems (a = 1, M = 1, G = 1, m = 1/N_s, each
+    from its own seed), N_s log-uniform in [64, 4096], dt = 1e-3, in one persistent launch per
+    step; the steps are replayed from a captured hipGraph.  Then the same number of steps through
+    the monitored driver (statistics every step, invariants of all 512 systems four times)."""
     import torch
 
-    from bia_3d_gpu_amd import ic
+    from bia_3d_gpu_amd import driver, ic
 
     sizes = ic.ensemble_sizes(512, 64, 4096, seed).astype(int)
     st = np.ascontiguousarray(np.concatenate([ic.plummer_system(int(n), seed + 1 + k) for k, n in enumerate(sizes)],
@@ -117,8 +118,14 @@ def run_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 512):
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / steps
     pairs = float(sum(int(n) * (int(n) - 1) // 2 for n in sizes))
+    every = max(1, steps // 4)
+    rep = driver.run(ctx, s, sizes, steps, eps=eps, dt=1e-3, mode=mode, monitor_every=every)
+    h = rep.histogram()
     return {"systems": 512, "particles": int(sizes.sum()), "pairs_per_step": pairs, "ms_per_step": sec * 1e3,
-            "pairs_per_s": pairs / sec, "steps": steps, "graph": True}
+            "pairs_per_s": pairs / sec, "steps": steps, "graph": True,
+            "monitored": {"ms_per_step": rep.seconds * 1e3 / steps, "monitor_every": every,
+                          "max_abs_energy_error": float(np.abs(rep.energy_error()).max()),
+                          "evaluations_per_pair": h["evaluations_per_pair"], "iteration_cap": h["iteration cap"]}}
 
 
 def run_fixture(ctx, steps: int):
@@ -295,7 +302,8 @@ def main():
         ensemble = run_ensemble(ctx, emode, max(args.steps, 3), args.warmup)
         ensemble["mode"] = "learned" if emode == MODE_LEARN else "guess"
         log(f"ensemble (512 systems, {ensemble['particles']} particles): {ensemble['ms_per_step']:.2f} ms/step, "
-            f"{ensemble['pairs_per_s']:.3e} pairs/s")
+            f"{ensemble['pairs_per_s']:.3e} pairs/s; monitored driver {ensemble['monitored']['ms_per_step']:.2f} "
+            f"ms/step, max |dE/E0| {ensemble['monitored']['max_abs_energy_error']:.2e}")
     fixture = None if args.no_fixture else run_fixture(ctx, 20)
     if fixture:
         log(f"reference fixture N=1332: plain {fixture['plain']['s_per_step'] * 1e3:.3f} ms/step, guess "

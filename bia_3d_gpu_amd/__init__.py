@@ -4,7 +4,8 @@ Interaction Approximation N-body method.
 The compute lives in ``libbia_hip.so`` (hand-written HIP kernels, C ABI in
 ``include/bia_hip.h``).  This package is the Python host side: ctypes bindings that mirror
 the reference's Fortran call interface (:mod:`.api`), the reference's file formats
-(:mod:`.io`) and seeded synthetic initial conditions (:mod:`.ic`).
+(:mod:`.io`), seeded synthetic initial conditions (:mod:`.ic`), the monitored multi-step
+driver (:mod:`.driver`) and the learned-initialiser training (:mod:`.learn`).
 """
 from . import io, ic  # noqa: F401
 from .api import (  # noqa: F401

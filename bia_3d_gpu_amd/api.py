@@ -242,13 +242,35 @@ class Context:
 
     def run_steps_(self, state, delta, sizes, nsteps: int, eps=None, dt: float = 1e-3, method: int = METHOD_ALG,
                    mode: int = MODE_PLAIN, coupling: int = 0, coup: float = 1.0, mass=None, charge=None,
-                   itr: int = 18, use_graph: bool = True):
+                   itr: int = 18, use_graph: bool = True, stats=None):
+        """nsteps of (step, state += delta) on the device; ``stats`` (int64 CUDA tensor of
+        STATS_LEN, optional) accumulates the per-pair statistics of every step."""
         sz, e = self._batched_args(sizes, eps, coupling)
         ptr = lambda t: None if t is None else t.data_ptr()
         check(load().bia_run_steps_dev(self._h, method, mode, coupling, len(sz), sz, e, coup, dt, itr,
                                        state.data_ptr(), state.shape[1], ptr(mass), ptr(charge), delta.data_ptr(),
-                                       nsteps, int(use_graph)), "bia_run_steps_dev", self._h)
+                                       nsteps, int(use_graph), ptr(stats)), "bia_run_steps_dev", self._h)
         return state
+
+    def invariants_at_(self, state, offset: int, n: int, eps: float, out):
+        """Device invariants of the n particles at column ``offset`` of a [6, ld] state (one system
+        of an ensemble); ``out`` is an 11-element float64 CUDA tensor (any offset into a larger one)."""
+        ld = state.shape[1]
+        ptr = state.data_ptr() + 8 * int(offset)
+        check(load().bia_invariants_dev(self._h, int(n), float(eps), ptr, ld, out.data_ptr()),
+              "bia_invariants_dev", self._h)
+        return out
+
+    def invariants_batched_(self, state, sizes, eps, out):
+        """Device invariants of every system of a [6, ld] state (systems of ``sizes`` side by
+        side, ``eps`` per system or one for all) into ``out``, a float64 CUDA tensor of
+        ``len(sizes) * 11`` elements (e.g. [nsys, 11]); one launch per kernel for the list."""
+        sz, e = self._batched_args(sizes, eps, 0)
+        if out.numel() < 11 * len(sz) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous tensor of at least 11 * len(sizes) float64")
+        check(load().bia_invariants_batched_dev(self._h, len(sz), sz, e, state.data_ptr(), state.shape[1],
+                                                out.data_ptr()), "bia_invariants_batched_dev", self._h)
+        return out
 
     # -- learned initialiser ---------------------------------------------------------------------
     def mlp_init(self, seed: int = 1):

@@ -39,21 +39,6 @@ int prof_end(bia_ctx* ctx) {
   return BIA_OK;
 }
 
-int ensure_tiles(bia_ctx* ctx, int nt) {
-  if (ctx->tiles_nt == nt) return BIA_OK;
-  const size_t ntiles = (size_t)nt * (nt + 1) / 2;
-  BIA_TRY(ctx, ctx->tiles.ensure(ntiles * sizeof(int2)));
-  int2* h = (int2*)malloc(ntiles * sizeof(int2));
-  if (!h) return BIA_ERR_OOM;
-  size_t k = 0;
-  for (int I = 0; I < nt; I++)
-    for (int J = I; J < nt; J++) h[k++] = make_int2(I, J);
-  hipError_t e = hipMemcpy(ctx->tiles.p, h, ntiles * sizeof(int2), hipMemcpyHostToDevice);
-  free(h);
-  BIA_HIP_TRY(ctx, e);
-  ctx->tiles_nt = nt;
-  return BIA_OK;
-}
 
 }  // namespace bia
 
@@ -95,7 +80,8 @@ int bia_destroy(bia_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   c->leg_in.release();
   c->leg_out.release();
-  bia::DevBuf* bufs[] = {&c->part, &c->tiles, &c->state, &c->delta, &c->stats, &c->inv_state, &c->inv_part, &c->inv_out, &c->dim_work, &c->dim_ctl,
+  bia::DevBuf* bufs[] = {&c->part, &c->state, &c->delta, &c->stats, &c->inv_state, &c->inv_part, &c->inv_out,
+                         &c->inv_tiles, &c->inv_sys, &c->dim_work, &c->dim_ctl,
                          &c->mlp_w, &c->mlp_ws, &c->mlp_table};
   for (auto* b : bufs) b->release();
   c->plan_tiles.release();

@@ -106,8 +106,7 @@ struct bia_ctx {
   hipStream_t cap_stream = nullptr;  // private blocking stream for graph capture when stream is null
   std::mutex mu;                 // one call at a time per context
   // step workspace
-  bia::DevBuf part, tiles, state, delta, stats;
-  int tiles_nt = -1;
+  bia::DevBuf part, state, delta, stats;
   // batched plans: device tile list (int4 I, J, sys, 0), system descriptors, reduce chunks
   bia::Plan plan;
   bia::DevBuf plan_tiles, plan_sys, plan_chunks;
@@ -118,8 +117,11 @@ struct bia_ctx {
   std::vector<long long> graph_key;
   // legacy per-row buffers
   bia::HostBuf leg_in, leg_out;  // rows of the per-particle reference entry point
-  // invariance workspace
-  bia::DevBuf inv_state, inv_part, inv_out;
+  // invariance workspace: staging, partial sums, and the plan of the last list of systems
+  // (device tile list int4 {sys, I, J, 0} and per-system descriptors, keyed by sizes and eps)
+  bia::DevBuf inv_state, inv_part, inv_out, inv_tiles, inv_sys;
+  std::vector<long long> inv_key;
+  int inv_ntiles = 0, inv_npe = 0, inv_nlin = 0, inv_maxnpe = 0, inv_maxnlin = 0;
   // DIM workspace
   bia::DevBuf dim_work, dim_ctl;
   // learned initialiser: fp32 master weights + Adam moments, training workspace, tabulated output
@@ -164,6 +166,4 @@ bia_ctx* default_ctx();
 // record the profiling events around the pair kernel (no-ops unless profiling is on)
 int prof_begin(bia_ctx* ctx);
 int prof_end(bia_ctx* ctx);
-// upper-triangle list of 64-wide tile pairs (I <= J) for nt tiles, cached on the context
-int ensure_tiles(bia_ctx* ctx, int nt);
 }  // namespace bia

@@ -533,7 +533,7 @@ static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, 
                                 (long long)(size_t)a.d_delta, (long long)(size_t)a.d_mass,
                                 (long long)(size_t)a.d_charge, a.ld, (long long)(size_t)ctx->stream, tuned.first, tuned.second,
                                 ctx->mlp_ready, (long long)(size_t)ctx->part.p,
-                                (long long)(size_t)ctx->mlp_table.p};
+                                (long long)(size_t)ctx->mlp_table.p, (long long)(size_t)a.d_stats};
   long long dtb, cb;
   memcpy(&dtb, &a.dt, 8);
   memcpy(&cb, &a.coup, 8);
@@ -606,10 +606,10 @@ int bia_autotune_dev(bia_ctx* ctx, int method, int mode, int coupling, int nsys,
 
 int bia_run_steps_dev(bia_ctx* ctx, int method, int mode, int coupling, int nsys, const int* sizes, const double* eps,
                       double coup, double dt, int itr, double* d_state, int ld, const double* d_mass,
-                      const double* d_charge, double* d_delta, int nsteps, int use_graph) {
+                      const double* d_charge, double* d_delta, int nsteps, int use_graph, unsigned long long* d_stats) {
   BIA_LOCKED(ctx);
   if (nsteps < 0) return BIA_ERR_ARG;
-  StepArgs a{method, mode, coupling, nsys, sizes, eps, coup, dt, itr, d_state, ld, d_mass, d_charge, d_delta, nullptr};
+  StepArgs a{method, mode, coupling, nsys, sizes, eps, coup, dt, itr, d_state, ld, d_mass, d_charge, d_delta, d_stats};
   return run_steps_locked(ctx, a, d_state, nsteps, use_graph);
 }
 

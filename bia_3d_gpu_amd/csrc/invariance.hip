@@ -4,11 +4,16 @@
 //   KE = sum |v|^2, PE = eps * sum_{i<j} 1/|r_i - r_j|, L = sum r x v, sums of x..w
 //
 // The O(N^2) potential term is computed over 64x64 tiles of the i<j triangle (one wave per
-// tile, lane = row, column particle wave-uniform).  Every lane accumulates in double-double
+// tile, lane = row, column particle wave-uniform); a list of systems side by side in one state
+// (an ensemble) is one launch of each kernel, tiles of all systems in one grid.  Every lane accumulates in double-double
 // (TwoSum), tile partials are written to a buffer and summed in a fixed order (256 blocks over
 // contiguous chunks, then one wave), so the result
 // is deterministic and close to the quad-precision sum of Invariance_QPU.f90 rather than to
 // the plain double sum.  The ten O(N) sums use the same double-double scheme.
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
 #include "bia_runtime.hpp"
 
 namespace bia {
@@ -36,12 +41,48 @@ __device__ __forceinline__ DD dd_wave_sum(DD a) {
   return a;
 }
 
-// potential tiles: partial[t] = sum over the tile's pairs of 1/r (double-double)
-__global__ __launch_bounds__(64) void k_inv_pe_tiles(int n, int ld, const double* __restrict__ st,
-                                                     const int2* __restrict__ tiles, DD* __restrict__ part) {
+// One system of a list: its columns in the state and where its partial sums go.
+struct InvSys {
+  long long col;        // first column of the system in the [6, ld] state
+  int n;                // particles
+  long long tile0;      // first of its potential tiles in the global tile numbering
+  int nt;               // 64-wide particle tiles (ntiles = nt (nt + 1) / 2)
+  int chunk, npe, pe0;  // first reduction stage: npe blocks of chunk tiles, results at pe0
+  int nlin, lin0;       // linear sums: nlin blocks, results at lin0 (10 each)
+  double eps;
+};
+
+// (I, J) of tile t of the row-major upper triangle of nt tiles (row I starts at I nt - I (I-1)/2)
+__device__ __forceinline__ int2 tri_tile(long long t, int nt) {
+  const double b = 2.0 * nt + 1;
+  int I = (int)((b - sqrt(b * b - 8.0 * (double)t)) * 0.5);
+  I = max(0, min(I, nt - 1));
+  auto start = [nt](long long r) { return r * nt - r * (r - 1) / 2; };
+  while (I > 0 && start(I) > t) --I;
+  while (I + 1 < nt && start(I + 1) <= t) ++I;
+  return make_int2(I, I + (int)(t - start(I)));
+}
+
+// the system owning global tile t: the last s with sys[s].tile0 <= t
+__device__ __forceinline__ int sys_of_tile(const InvSys* __restrict__ sys, int nsys, long long t) {
+  int lo = 0, hi = nsys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sys[mid].tile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// potential tiles: part[t] = sum over the tile's pairs of 1/r (double-double)
+__global__ __launch_bounds__(64) void k_inv_pe_tiles(int nsys, const InvSys* __restrict__ sys, int ld,
+                                                     const double* __restrict__ state, DD* __restrict__ part) {
   const int lane = threadIdx.x;
-  const int2 t = tiles[blockIdx.x];
-  const int I = t.x, J = t.y;
+  const long long t = blockIdx.x;
+  const InvSys S = sys[sys_of_tile(sys, nsys, t)];
+  const int2 IJ = tri_tile(t - S.tile0, S.nt);
+  const int I = IJ.x, J = IJ.y, n = S.n;
+  const double* st = state + S.col;
   const int i = I * 64 + lane;
   const bool ok = i < n;
   const double xi = ok ? st[i] : 0, yi = ok ? st[ld + i] : 0, zi = ok ? st[2 * (size_t)ld + i] : 0;
@@ -56,17 +97,25 @@ __global__ __launch_bounds__(64) void k_inv_pe_tiles(int n, int ld, const double
     }
   }
   s = dd_wave_sum(s);
-  if (lane == 0) part[blockIdx.x] = s;
+  if (lane == 0) part[t] = s;
 }
 
-// ten linear sums per block: Xa Ya Za Ua Va Wa Lx Ly Lz KE
-__global__ __launch_bounds__(256) void k_inv_linear(int n, int ld, const double* __restrict__ st,
-                                                    DD* __restrict__ part) {
+// ten linear sums per block (Xa Ya Za Ua Va Wa Lx Ly Lz KE); block b of a system takes particles
+// b*256 + thread, stepping by kLinStride, so the partition does not depend on how many blocks
+// of the grid are empty
+constexpr int kLinBlocks = 256;
+constexpr int kLinStride = kLinBlocks * 256;
+__global__ __launch_bounds__(256) void k_inv_linear(const InvSys* __restrict__ sys, int ld,
+                                                    const double* __restrict__ state, DD* __restrict__ part) {
+  const InvSys S = sys[blockIdx.y];
+  if ((int)blockIdx.x >= S.nlin) return;
+  const double* st = state + S.col;
+  const int n = S.n;
   __shared__ DD red[4][10];
   DD a[10];
 #pragma unroll
   for (int k = 0; k < 10; k++) a[k] = DD{0, 0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kLinStride) {
     double x = st[i], y = st[ld + i], z = st[2 * (size_t)ld + i];
     double u = st[3 * (size_t)ld + i], v = st[4 * (size_t)ld + i], w = st[5 * (size_t)ld + i];
     dd_add(a[0], x); dd_add(a[1], y); dd_add(a[2], z);
@@ -86,18 +135,21 @@ __global__ __launch_bounds__(256) void k_inv_linear(int n, int ld, const double*
   if (threadIdx.x < 10) {
     DD r = red[0][threadIdx.x];
     for (int w2 = 1; w2 < 4; w2++) dd_add(r, red[w2][threadIdx.x]);
-    part[(size_t)blockIdx.x * 10 + threadIdx.x] = r;
+    part[((size_t)S.lin0 + blockIdx.x) * 10 + threadIdx.x] = r;
   }
 }
 
-// first stage of the potential's reduction: block b sums tile partials [b*chunk, (b+1)*chunk) in a
-// fixed order (thread stride, then the block's waves in order) into out[b]
-__global__ __launch_bounds__(256) void k_inv_pe_reduce(int ntile, int chunk, const DD* __restrict__ pe,
+// first stage of the potential's reduction: block b of a system sums its tile partials
+// [b*chunk, (b+1)*chunk) in a fixed order (thread stride, then the block's waves in order)
+__global__ __launch_bounds__(256) void k_inv_pe_reduce(const InvSys* __restrict__ sys, const DD* __restrict__ pe,
                                                        DD* __restrict__ out) {
+  const InvSys S = sys[blockIdx.y];
+  if ((int)blockIdx.x >= S.npe) return;
   __shared__ DD red[4];
   DD a{0, 0};
-  const int b0 = blockIdx.x * chunk, b1 = min(ntile, b0 + chunk);
-  for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) dd_add(a, pe[b]);
+  const long long ntiles = (long long)S.nt * (S.nt + 1) / 2;
+  const long long b0 = (long long)blockIdx.x * S.chunk, b1 = min(ntiles, b0 + S.chunk);
+  for (long long b = b0 + threadIdx.x; b < b1; b += blockDim.x) dd_add(a, pe[S.tile0 + b]);
   a = dd_wave_sum(a);
   const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
   if (lane == 0) red[wv] = a;
@@ -105,27 +157,30 @@ __global__ __launch_bounds__(256) void k_inv_pe_reduce(int ntile, int chunk, con
   if (threadIdx.x == 0) {
     DD r = red[0];
     for (int w2 = 1; w2 < 4; w2++) dd_add(r, red[w2]);
-    out[blockIdx.x] = r;
+    out[S.pe0 + blockIdx.x] = r;
   }
 }
 
-// single-wave final reduction in a fixed order; out = KE PE Lx Ly Lz Ua Va Wa Xa Ya Za
-__global__ __launch_bounds__(64) void k_inv_final(int nlin, const DD* __restrict__ lin, int ntile,
-                                                  const DD* __restrict__ pe, double eps, double* __restrict__ out) {
+// one wave per system, final reduction in a fixed order; out[11 s ..] = KE PE Lx Ly Lz Ua Va Wa Xa Ya Za
+__global__ __launch_bounds__(64) void k_inv_final(const InvSys* __restrict__ sys, const DD* __restrict__ lin,
+                                                  const DD* __restrict__ pe, double* __restrict__ out_all) {
+  const InvSys S = sys[blockIdx.x];
+  double* out = out_all + 11 * (size_t)blockIdx.x;
   const int lane = threadIdx.x;
   DD a[11];
 #pragma unroll
   for (int k = 0; k < 11; k++) a[k] = DD{0, 0};
-  for (int b = lane; b < nlin; b += 64)
+  for (int b = lane; b < S.nlin; b += 64)
 #pragma unroll
-    for (int k = 0; k < 10; k++) dd_add(a[k], lin[(size_t)b * 10 + k]);
-  for (int b = lane; b < ntile; b += 64) dd_add(a[10], pe[b]);
+    for (int k = 0; k < 10; k++) dd_add(a[k], lin[((size_t)S.lin0 + b) * 10 + k]);
+  for (int b = lane; b < S.npe; b += 64) dd_add(a[10], pe[S.pe0 + b]);
 #pragma unroll
   for (int k = 0; k < 11; k++) a[k] = dd_wave_sum(a[k]);
   if (lane == 0) {
     double v[11];
     for (int k = 0; k < 11; k++) v[k] = a[k].hi + a[k].lo;
     // eps * PE in double-double: product error term via fma
+    const double eps = S.eps;
     double p = eps * a[10].hi;
     double pe_ = p + (fma(eps, a[10].hi, -p) + eps * a[10].lo);
     out[0] = v[9];
@@ -136,25 +191,78 @@ __global__ __launch_bounds__(64) void k_inv_final(int nlin, const DD* __restrict
   }
 }
 
-static int invariants_locked(bia_ctx* ctx, int n, double eps, const double* d_state, int ld, double* d_out) {
-  if (n < 1 || ld < n || !d_state || !d_out) return BIA_ERR_ARG;
-  const int nt = (n + 63) / 64;
-  const size_t ntiles = (size_t)nt * (nt + 1) / 2;
-  const int nlin = 256, npe = 256;
-  BIA_TRY(ctx, ensure_tiles(ctx, nt));
-  BIA_TRY(ctx, ctx->inv_part.ensure((ntiles + (size_t)nlin * 10 + npe) * sizeof(DD)));
+// Descriptors of the systems (uploaded when the sizes or eps change).  The first reduction stage
+// of a system uses ceil(ntiles / chunk) blocks with chunk = ceil(ntiles / 256), and the linear sums
+// min(256, ceil(n / 256)) blocks: the non-empty blocks of a fixed 256-block partition, so a
+// system's result does not depend on the other systems of the list.
+static int ensure_inv_plan(bia_ctx* ctx, int nsys, const int* sizes, const double* eps) {
+  std::vector<long long> key(2 * (size_t)nsys);
+  for (int s = 0; s < nsys; s++) {
+    key[2 * s] = sizes[s];
+    memcpy(&key[2 * s + 1], &eps[s], 8);
+  }
+  if (key == ctx->inv_key) return BIA_OK;
+  std::vector<InvSys> h(nsys);
+  long long col = 0, tile0 = 0;
+  int pe0 = 0, lin0 = 0, maxnpe = 0, maxnlin = 0;
+  for (int s = 0; s < nsys; s++) {
+    InvSys& S = h[s];
+    S.col = col;
+    S.n = sizes[s];
+    S.nt = (S.n + 63) / 64;
+    const long long ntiles = (long long)S.nt * (S.nt + 1) / 2;
+    S.tile0 = tile0;
+    S.chunk = (int)((ntiles + kLinBlocks - 1) / kLinBlocks);
+    S.npe = (int)((ntiles + S.chunk - 1) / S.chunk);
+    S.pe0 = pe0;
+    S.nlin = std::min(kLinBlocks, (S.n + 255) / 256);
+    S.lin0 = lin0;
+    S.eps = eps[s];
+    col += S.n;
+    tile0 += ntiles;
+    pe0 += S.npe;
+    lin0 += S.nlin;
+    maxnpe = std::max(maxnpe, S.npe);
+    maxnlin = std::max(maxnlin, S.nlin);
+  }
+  if (tile0 > INT_MAX) return BIA_ERR_ARG;  // one potential tile per workgroup of a 1-D grid
+  ctx->inv_key.clear();  // invalid until the upload below succeeds
+  BIA_TRY(ctx, ctx->inv_sys.ensure(h.size() * sizeof(InvSys)));
+  BIA_HIP_TRY(ctx, hipMemcpy(ctx->inv_sys.p, h.data(), h.size() * sizeof(InvSys), hipMemcpyHostToDevice));
+  ctx->inv_ntiles = (int)tile0;
+  ctx->inv_npe = pe0;
+  ctx->inv_nlin = lin0;
+  ctx->inv_maxnpe = maxnpe;
+  ctx->inv_maxnlin = maxnlin;
+  ctx->inv_key = std::move(key);
+  return BIA_OK;
+}
+
+// invariants of nsys systems side by side in a [6, ld] state (columns in the order of sizes)
+static int invariants_locked(bia_ctx* ctx, int nsys, const int* sizes, const double* eps, const double* d_state,
+                             int ld, double* d_out) {
+  if (nsys < 1 || !sizes || !eps || !d_state || !d_out) return BIA_ERR_ARG;
+  long long total = 0;
+  for (int s = 0; s < nsys; s++) {
+    if (sizes[s] < 1) return BIA_ERR_ARG;
+    total += sizes[s];
+  }
+  if (total > ld || nsys > 65535) return BIA_ERR_ARG;  // nsys is the grid's y extent
+  BIA_TRY(ctx, ensure_inv_plan(ctx, nsys, sizes, eps));
+  const InvSys* sys = ctx->inv_sys.as<InvSys>();
+  BIA_TRY(ctx, ctx->inv_part.ensure(((size_t)ctx->inv_ntiles + (size_t)ctx->inv_nlin * 10 + ctx->inv_npe) *
+                                    sizeof(DD)));
   DD* pe = ctx->inv_part.as<DD>();
-  DD* lin = pe + ntiles;
-  DD* pe2 = lin + (size_t)nlin * 10;
-  hipLaunchKernelGGL(k_inv_pe_tiles, dim3((unsigned)ntiles), dim3(64), 0, ctx->stream, n, ld, d_state,
-                     ctx->tiles.as<int2>(), pe);
+  DD* lin = pe + ctx->inv_ntiles;
+  DD* pe2 = lin + (size_t)ctx->inv_nlin * 10;
+  hipLaunchKernelGGL(k_inv_pe_tiles, dim3((unsigned)ctx->inv_ntiles), dim3(64), 0, ctx->stream, nsys, sys, ld,
+                     d_state, pe);
   BIA_HIP_TRY(ctx, hipGetLastError());
-  const int chunk = (int)((ntiles + npe - 1) / npe);
-  hipLaunchKernelGGL(k_inv_pe_reduce, dim3(npe), dim3(256), 0, ctx->stream, (int)ntiles, chunk, pe, pe2);
+  hipLaunchKernelGGL(k_inv_pe_reduce, dim3(ctx->inv_maxnpe, nsys), dim3(256), 0, ctx->stream, sys, pe, pe2);
   BIA_HIP_TRY(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_inv_linear, dim3(nlin), dim3(256), 0, ctx->stream, n, ld, d_state, lin);
+  hipLaunchKernelGGL(k_inv_linear, dim3(ctx->inv_maxnlin, nsys), dim3(256), 0, ctx->stream, sys, ld, d_state, lin);
   BIA_HIP_TRY(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_inv_final, dim3(1), dim3(64), 0, ctx->stream, nlin, lin, npe, pe2, eps, d_out);
+  hipLaunchKernelGGL(k_inv_final, dim3(nsys), dim3(64), 0, ctx->stream, sys, lin, pe2, d_out);
   BIA_HIP_TRY(ctx, hipGetLastError());
   return BIA_OK;
 }
@@ -166,7 +274,7 @@ static int invariants_host_locked(bia_ctx* ctx, int n, double eps, const double*
   for (int c = 0; c < 6; c++)
     BIA_HIP_TRY(ctx, hipMemcpyAsync(d + (size_t)c * n, in[c], (size_t)n * sizeof(double), hipMemcpyHostToDevice,
                                     ctx->stream));
-  BIA_TRY(ctx, invariants_locked(ctx, n, eps, d, n, dout));
+  BIA_TRY(ctx, invariants_locked(ctx, 1, &n, &eps, d, n, dout));
   BIA_HIP_TRY(ctx, hipMemcpyAsync(out, dout, 11 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return BIA_OK;
@@ -183,7 +291,16 @@ int bia_invariants_dev(bia_ctx* ctx, int n, double eps, const double* d_state, i
   std::lock_guard<std::mutex> lk(ctx->mu);
   ctx->last_error[0] = 0;
   BIA_HIP_TRY(ctx, hipSetDevice(ctx->device));
-  return invariants_locked(ctx, n, eps, d_state, ld, d_out);
+  return invariants_locked(ctx, 1, &n, &eps, d_state, ld, d_out);
+}
+
+int bia_invariants_batched_dev(bia_ctx* ctx, int nsys, const int* sizes, const double* eps, const double* d_state,
+                               int ld, double* d_out) {
+  if (!ctx) return BIA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->last_error[0] = 0;
+  BIA_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return invariants_locked(ctx, nsys, sizes, eps, d_state, ld, d_out);
 }
 
 int bia_invariants_host(bia_ctx* ctx, int n, double eps, const double* X, const double* Y, const double* Z,

@@ -144,10 +144,12 @@ int bia_autotune_dev(bia_ctx *ctx, int method, int mode, int coupling, int nsys,
 
 /* Device-resident multi-step driver: nsteps of (step, state += delta) enqueued on the context's
  * stream with no host synchronisation; with use_graph the step is captured once as a hipGraph and
- * replayed.  d_state is updated in place; d_delta holds the last step's deltas. */
+ * replayed.  d_state is updated in place; d_delta holds the last step's deltas.  d_stats (device,
+ * BIA_STATS_LEN, may be NULL) accumulates the per-pair statistics of every step. */
 int bia_run_steps_dev(bia_ctx *ctx, int method, int mode, int coupling, int nsys, const int *sizes, const double *eps,
                       double coup, double dt, int itr, double *d_state, int ld, const double *d_mass,
-                      const double *d_charge, double *d_delta, int nsteps, int use_graph);
+                      const double *d_charge, double *d_delta, int nsteps, int use_graph,
+                      unsigned long long *d_stats);
 
 /* Host-array variant of bia_step_dev for Fortran callers (bind(C) interface in
  * bia_3d_gpu_amd/fortran/bia_hip.f90).  dX..dW are overwritten; stats (may be NULL) is
@@ -162,6 +164,13 @@ int bia_step_host(bia_ctx *ctx, int method, int mode, int n, double eps, double 
 int bia_invariants_dev(bia_ctx *ctx, int n, double eps, const double *d_state, int ld, double *d_out);
 int bia_invariants_host(bia_ctx *ctx, int n, double eps, const double *X, const double *Y, const double *Z,
                         const double *U, const double *V, const double *W, double *out);
+/* The same for nsys systems side by side in one [6, ld] device state (columns in the order of
+ * sizes, eps per system; both host arrays): d_out[11 s + k] is invariant k of system s.  One
+ * launch of each kernel for the whole list; each system's values are bitwise those of
+ * bia_invariants_dev on its columns alone.  No counterpart in the reference (its invariance is per
+ * realisation, bia/bia.f90:236-241): for the monitored multi-step driver over ensembles. */
+int bia_invariants_batched_dev(bia_ctx *ctx, int nsys, const int *sizes, const double *eps, const double *d_state,
+                               int ld, double *d_out);
 
 /* DIM direct-sum RKF45 over [*x0, xe] on an AoS state y[6 np] (host, in/out), with the
  * reference's step controller (dim/rkf_kernel.cu:237-491).  *h is the step (in/out; set

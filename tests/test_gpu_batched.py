@@ -167,3 +167,59 @@ def test_mass_spectrum_ensemble_vs_oracle(ctx, mode):
         print(f"system n={n}: error/allowed {rp:.3f} {rv:.3f}; excluded {O.indeterminate(b)}")
         assert ok and O.indeterminate(b) <= 3 * n // 100
         off += n
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_monitored_driver_matches_plain_steps(ctx, mode, tmp_path):
+    """The monitored multi-step driver (graph replays in chunks, invariants and statistics
+    recorded on the device, .bbb checkpoints written from pinned memory) against an eager loop
+    of single steps with the same monitoring done by hand."""
+    import torch
+    from bia_3d_gpu_amd import STATS_LEN, driver, ic, io
+    sizes = [200, 333, 64]
+    st = _cat([ic.plummer_system(n, seed=70 + k) for k, n in enumerate(sizes)])
+    eps = np.array([-2.0 / n for n in sizes])
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    nsteps, dt = 7, 1e-3
+    a = torch.from_numpy(st).cuda()
+    lines = []
+    rep = driver.run(ctx, a, sizes, nsteps, eps=eps, dt=dt, mode=mode, monitor_every=2, checkpoint_every=3,
+                     checkpoint_path=str(tmp_path / "ck_{step:04d}.bbb"), log=lines.append)
+    assert rep.steps == [0, 2, 4, 6, 7] and len(lines) == 5
+    assert rep.checkpoints == [str(tmp_path / "ck_0003.bbb"), str(tmp_path / "ck_0006.bbb")]
+
+    b = torch.from_numpy(st).cuda()
+    d = torch.empty_like(b)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    acc = np.zeros(STATS_LEN, dtype=np.int64)
+    out = torch.empty(11, dtype=torch.float64, device="cuda")
+    for k in range(nsteps + 1):
+        if k in rep.steps:
+            i = rep.steps.index(k)
+            assert np.array_equal(rep.stats[i], acc), k
+            for s, n in enumerate(sizes):
+                ctx.invariants_at_(b, int(offs[s]), n, float(eps[s]), out)
+                assert np.array_equal(rep.invariants[i, s], out.cpu().numpy()), (k, s)
+        if k in (3, 6):
+            ck = io.read_bbb(str(tmp_path / f"ck_{k:04d}.bbb"))
+            assert [c.shape[1] for c in ck] == sizes
+            assert np.array_equal(np.concatenate(ck, axis=1), b.cpu().numpy()), k
+        if k == nsteps:
+            break
+        s_k = torch.zeros(STATS_LEN, dtype=torch.int64, device="cuda")
+        ctx.step_batched_(b, d, sizes, eps=eps, dt=dt, mode=mode, stats=s_k)
+        b.add_(d)
+        acc += s_k.cpu().numpy()
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    # resuming from the step-6 checkpoint gives the same step 7
+    c, csz = driver.load_checkpoint(str(tmp_path / "ck_0006.bbb"))
+    assert csz == sizes
+    driver.run(ctx, c, csz, 1, eps=eps, dt=dt, mode=mode)
+    torch.cuda.synchronize()
+    assert torch.equal(a, c)
+    h = rep.histogram()
+    npairs = sum(n * (n - 1) // 2 for n in sizes)
+    assert h["pairs"] == nsteps * npairs and h["evaluations_per_pair"] > 0
+    assert np.isfinite(rep.energy_error()).all() and np.abs(rep.energy_error()).max() < 1e-3
+    print(lines[-1])

@@ -166,6 +166,44 @@ def test_invariants_vs_quad(ctx, fixture1332):
         assert abs(g[name] - q[k]) <= tol + 2e-16 * abs(q[k]), (name, g[name], q[k])
 
 
+def test_batched_invariants_equal_single_system_calls(ctx):
+    """Invariants of a list of systems side by side (one launch per kernel for the list): each
+    system's values bitwise those of the single-system call on its columns, and within the
+    double-double accuracy of the quad-precision oracle."""
+    import torch
+    from bia_3d_gpu_amd import ic
+    sizes = [1, 2, 63, 64, 65, 300, 1000, 4100]
+    states = [ic.plummer_system(n, seed=90 + k) if n > 2 else np.random.default_rng(k).standard_normal((6, n))
+              for k, n in enumerate(sizes)]
+    eps = np.array([-1.0 / n for n in sizes])
+    st = np.ascontiguousarray(np.concatenate(states, axis=1))
+    ld = st.shape[1] + 5  # spare columns after the systems
+    s = torch.zeros((6, ld), dtype=torch.float64, device="cuda")
+    s[:, :st.shape[1]] = torch.from_numpy(st).cuda()
+    out = torch.full((len(sizes), 11), float("nan"), dtype=torch.float64, device="cuda")
+    one = torch.empty(11, dtype=torch.float64, device="cuda")
+    for rep in range(2):  # the second call reuses the cached plan
+        ctx.invariants_batched_(s, sizes, eps, out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        off = 0
+        for k, n in enumerate(sizes):
+            ctx.invariants_at_(s, off, n, float(eps[k]), one)
+            torch.cuda.synchronize()
+            assert np.array_equal(got[k], one.cpu().numpy()), (rep, k, n)
+            off += n
+    for k, n in enumerate(sizes):
+        q = O.invariance(states[k], float(eps[k]), "quad")
+        scale = np.abs(states[k]).max() ** 2 * n
+        np.testing.assert_allclose(got[k], q, rtol=1e-15, atol=1e-15 * scale)
+        if n == 1:
+            assert got[k][1] == 0.0  # no pairs
+    # a different eps list with the same sizes rebuilds the plan
+    ctx.invariants_batched_(s, sizes, 2 * eps, out)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(out.cpu().numpy()[:, 1], 2 * got[:, 1], rtol=1e-15)
+
+
 def test_invariance_legacy_symbol(fixture1332):
     from bia_3d_gpu_amd import invariance_gpu
     st, eps = fixture1332["state"], fixture1332["eps"]
