@@ -110,13 +110,16 @@ def flops_per_step(stats, mode: int = MODE_PLAIN) -> float:
     parabola 472 + 76 k, RKF 80 + 790 s.
 
     Guess and learned modes (universal-variable Kepler drift, csrc/bia_pair.hpp kepler_pair),
-    counted from that source: 126 per pair (relative state 6, set-up 23, series start 6, output
-    with its Stumpff evaluation 79, accumulation 12) + 75 per evaluation of the time equation,
-    + 120 per learned start (features, three asinh, trilinear lookup, exp).
+    counted from that source with a reciprocal or reciprocal square root as one division (and
+    one square root), comparisons and min/max not counted: 109 per pair (relative state 6,
+    kep_setup 23, features 7, series start 9, kep_output 52 with its Taylor carry of the G
+    functions, accumulation 12) + 58 per evaluation of the time equation (kep_g 22, kep_halley
+    32, step and convergence test 4), + 120 per learned start (three asinh, trilinear lookup,
+    exp, scale).  The bound-orbit period reduction, taken by few pairs, is not counted.
     """
     s = np.asarray(stats, dtype=np.float64)
     if mode != MODE_PLAIN:
-        return 126 * s[0:4].sum() + 75 * s[4:8].sum() + 120 * s[14]
+        return 109 * s[0:4].sum() + 58 * s[4:8].sum() + 120 * s[14]
     n_h, n_p, n_e, n_r = s[0:4]
     it_h, it_p, it_e, it_r = s[4:8]
     k_h = it_h - 2 * n_h

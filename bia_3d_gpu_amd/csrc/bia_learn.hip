@@ -3,10 +3,11 @@
 // accumulation), Adam, the sample recorder, and the tabulation the pair kernel reads.
 //
 // Network input (one pair that is not weak over dt): the squashed dimensionless numbers
-// u_k = asinh(a_k / 1e-2) / asinh(1e4) of a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3,
-// a3 = beta dt^2 / r0^2, which fix the universal Kepler equation up to scale.  Output:
-// log(s / s_start) / min(|a|, 1)^3, the converged universal anomaly relative to the rational
-// series start, scaled by the start's O(|a|^3) error (kepler_pair in bia_pair.hpp).
+// u_k = sign(a_k) log2(1 + |a_k| / 1e-2) / log2(1 + 1e4) (feat_squash in bia_pair.hpp, single
+// precision) of a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3, a3 = beta dt^2 / r0^2,
+// which fix the universal Kepler equation up to scale.  Output: log(s / s_start) /
+// min(|a|, 1)^3, the converged universal anomaly relative to the rational series start,
+// scaled by the start's O(|a|^3) error (kepler_pair in bia_pair.hpp).
 //
 // Parameters (fp32 master copy, 4481 floats): W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3.
 // Matrix products run on v_mfma_f32_32x32x16_bf16 with BF16 operands: one wave computes a

@@ -276,19 +276,21 @@ BIA_HDF Stumpff stumpff(double y) {
                    i8 = 1.0 / 40320, i9 = 1.0 / 362880, i10 = 1.0 / 3628800, i11 = 1.0 / 39916800,
                    i12 = 1.0 / 479001600, i13 = 1.0 / 6227020800.0, i14 = 1.0 / 87178291200.0,
                    i15 = 1.0 / 1307674368000.0, i16 = 1.0 / 20922789888000.0, i17 = 1.0 / 355687428096000.0;
-  double c2, c3;
-  int m = 0;
+  Stumpff c;
   if (fabs(y) <= 1e-4) {
     const double z = -y;
-    c2 = ((i8 * z + i6) * z + i4) * z + i2;
-    c3 = ((i9 * z + i7) * z + i5) * z + i3;
+    c.c2 = ((i8 * z + i6) * z + i4) * z + i2;
+    c.c3 = ((i9 * z + i7) * z + i5) * z + i3;
+    c.c1 = 1 - y * c.c3;
+    c.c0 = 1 - y * c.c2;
   } else {
+    int m = 0;
     while (fabs(y) > 0.1 && m < 60) {
       y *= 0.25;
       m++;
     }
     const double z = -y;
-    c2 = i16;
+    double c2 = i16, c3 = i17;
     c2 = c2 * z + i14;
     c2 = c2 * z + i12;
     c2 = c2 * z + i10;
@@ -296,7 +298,6 @@ BIA_HDF Stumpff stumpff(double y) {
     c2 = c2 * z + i6;
     c2 = c2 * z + i4;
     c2 = c2 * z + i2;
-    c3 = i17;
     c3 = c3 * z + i15;
     c3 = c3 * z + i13;
     c3 = c3 * z + i11;
@@ -304,41 +305,55 @@ BIA_HDF Stumpff stumpff(double y) {
     c3 = c3 * z + i7;
     c3 = c3 * z + i5;
     c3 = c3 * z + i3;
+    double c1 = 1 - y * c3, c0 = 1 - y * c2;
+    for (; m > 0; m--) {
+      const double n3 = 0.25 * (c2 + c0 * c3);
+      const double n2 = 0.5 * c1 * c1;
+      const double n1 = c0 * c1;
+      const double n0 = 2 * c0 * c0 - 1;
+      c0 = n0;
+      c1 = n1;
+      c2 = n2;
+      c3 = n3;
+    }
+    c.c0 = c0;
+    c.c1 = c1;
+    c.c2 = c2;
+    c.c3 = c3;
   }
-  double c1 = 1 - y * c3, c0 = 1 - y * c2;
-  for (; m > 0; m--) {
-    const double n3 = 0.25 * (c2 + c0 * c3);
-    const double n2 = 0.5 * c1 * c1;
-    const double n1 = c0 * c1;
-    const double n0 = 2 * c0 * c0 - 1;
-    c0 = n0;
-    c1 = n1;
-    c2 = n2;
-    c3 = n3;
-  }
-  return {c0, c1, c2, c3};
+  return c;
 }
 
-// 1/x from the hardware reciprocal and two Newton steps (within an ulp; 1/0 and non-finite x
-// give a non-finite result); the host build divides
+// 1/x from the hardware reciprocal estimate r (relative error e = 1 - x r near 2^-24) and one
+// third-order correction r (1 + e + e^2), which leaves e^3 (within an ulp); 1/0 and non-finite x
+// give a non-finite result.  The host build divides.
 BIA_HDF double fast_rcp(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  double r = __builtin_amdgcn_rcp(x);
-  r = fma(fma(-x, r, 1.0), r, r);
-  return fma(fma(-x, r, 1.0), r, r);
+  const double r = __builtin_amdgcn_rcp(x);
+  const double e = fma(-x, r, 1.0);
+  return fma(r, fma(e, e, e), r);
 #else
   return 1.0 / x;
 #endif
 }
 
-// 1/sqrt(x) from the hardware estimate and two Newton steps (within an ulp); the host build
-// uses the library square root
+// the hardware reciprocal estimate alone (relative error near 2^-24), for error estimates
+BIA_HDF double rcp_estimate(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcp(x);
+#else
+  return 1.0 / x;
+#endif
+}
+
+// 1/sqrt(x) from the hardware estimate y (e = 1 - x y^2 near 2^-24) and one third-order
+// correction y (1 + e/2 + 3e^2/8), which leaves 5e^3/16 (within an ulp); the host build uses the
+// library square root
 BIA_HDF double fast_rsqrt(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  double y = __builtin_amdgcn_rsq(x);
-  const double hx = 0.5 * x;
-  y = y * fma(-hx * y, y, 1.5);
-  return y * fma(-hx * y, y, 1.5);
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-(x * y), y, 1.0);
+  return fma(y, e * fma(e, 0.375, 0.5), y);
 #else
   return 1.0 / sqrt(x);
 #endif
@@ -350,8 +365,8 @@ struct KepG {
 };
 
 BIA_HDF KepG kep_g(double beta, double s) {
-  const Stumpff c = stumpff(beta * s * s);
   const double s2 = s * s;
+  const Stumpff c = stumpff(beta * s2);
   return {s, c.c0, s * c.c1, s2 * c.c2, s2 * s * c.c3};
 }
 
@@ -371,81 +386,91 @@ BIA_HDF void kep_setup(double eps, double dt, const double rel[6], KepPair& K) {
   K.zeta = K.mu - K.beta * K.r0;
   K.dt = dt;
   K.tau = dt;
-  if (K.beta > 0) {  // bound orbit: whole periods P = 2 pi mu / beta^1.5 return the pair to its start
+  // bound orbit: whole periods P = 2 pi mu / beta^1.5 return the pair to its start; |dt| >= P is
+  // tested as dt^2 beta^3 >= (2 pi mu)^2, so the period is formed only when it is needed
+  if (K.beta > 0 && dt * dt * (K.beta * K.beta * K.beta) >= (2 * kPi * K.mu) * (2 * kPi * K.mu)) {
     const double rb = fast_rsqrt(K.beta);
     const double P = 2 * kPi * K.mu * rb * rb * rb;
-    if (fabs(dt) >= P) K.tau = dt - P * floor(dt / P);
+    K.tau = dt - P * floor(dt / P);
   }
 }
 
 // dimensionless strength over the step: a1 = eta tau / r0^2, a2 = zeta tau^2 / r0^3,
 // a3 = beta tau^2 / r0^2
 BIA_HDF void kep_features(const KepPair& K, double a[3]) {
-  const double q = K.tau * K.ir0;
+  const double q = K.tau * K.ir0, q2 = q * q;
   a[0] = K.eta * q * K.ir0;
-  a[1] = K.zeta * q * q * K.ir0;
-  a[2] = K.beta * q * q;
+  a[1] = K.zeta * q2 * K.ir0;
+  a[2] = K.beta * q2;
 }
 
 // start of the iteration: with sigma = s / s0, s0 = tau / r0, the time equation reads
 // sigma + (a1/2) sigma^2 + (a2/6) sigma^3 + O(a^2 sigma^4) = 1, whose one-step rational solution
 // sigma = 1 - (a1/2 + a2/6) / (1 + a1 + a2/2) has an O(m^3) error, m = |a|.  The denominator is
-// held at 1/2 or more and sigma in [0.05, 2], so the start is a continuous function of a.
+// held at 1/2 or more and sigma in [0.05, 2], so the start is a continuous function of a.  The
+// quotient takes the hardware reciprocal estimate: its relative error (near 2^-24) times
+// |a1/2 + a2/6| is far below the O(m^3) error of the start itself.
 BIA_HDF double kep_series_start(const KepPair& K, const double a[3]) {
   const double s0 = K.tau * K.ir0;
   const double den = fmax(1 + a[0] + 0.5 * a[1], 0.5);
-  const double sig = fmin(fmax(1 - (0.5 * a[0] + a[1] * (1.0 / 6)) * fast_rcp(den), 0.05), 2.0);
+  const double sig = fmin(fmax(1 - (0.5 * a[0] + a[1] * (1.0 / 6)) * rcp_estimate(den), 0.05), 2.0);
   return s0 * sig;
 }
 
-// scale of the learned target: log(s / s_start) / min(|a|, 1)^3 is O(1) for every strength
+// scale of the learned target: log(s / s_start) / min(|a|, 1)^3 is O(1) for every strength (the
+// Euclidean norm keeps the target a smooth function of the features)
 BIA_HDF double kep_target_scale(const double a[3]) {
   const double m = fmin(sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]), 1.0);
   return m * m * m;
 }
 
-// One Halley step on t(s) = tau at the point g: returns the step d; *est = the error left after
-// it, |C^2 - t3 / (6 t1)| d^3 with C = t2 / (2 t1) (t1, t2, t3 the derivatives of t); *valid when
-// the step is usable (finite, t1 > 0, positive Halley denominator); *F = t(s) - tau.
-BIA_HDF double kep_halley(const KepPair& K, const KepG& g, double* F, double* est, bool* valid) {
+// One Halley step on t(s) = tau at the point g: returns the step d = -f t1 / (t1^2 - f t2 / 2)
+// (t1, t2, t3 the derivatives of t, f = t(s) - tau).  The error left after it is about
+// |C^2 - t3 / (6 t1)| |d|^3 with C = t2 / (2 t1); *est = t1^2 times that,
+// |t2^2 / 4 - t1 t3 / 6| |d|^3, and *t1sq = t1^2, so the caller compares without dividing.
+// *valid when the step is usable (finite, t1 > 0, positive finite Halley denominator); *F = f.
+BIA_HDF double kep_halley(const KepPair& K, const KepG& g, double* F, double* est, double* t1sq, bool* valid) {
+  constexpr double kInf = __builtin_huge_val();
   const double s = g.s;
   const double f = K.r0 * s + K.eta * g.G2 + K.zeta * g.G3 - K.tau;
   const double F1 = K.r0 + K.eta * g.G1 + K.zeta * g.G2;
   const double F2 = K.eta * g.c0 + K.zeta * g.G1;
   const double F3 = K.zeta * g.c0 - K.beta * K.eta * g.G1;
-  const double iF1 = fast_rcp(F1);
-  const double den = F1 - 0.5 * f * iF1 * F2;
-  const double d = -f * fast_rcp(den);
-  const double C = 0.5 * F2 * iF1;
+  const double F1sq = F1 * F1;
+  const double D = F1sq - (0.5 * f) * F2;
+  const double d = -(f * F1) * fast_rcp(D);
   const double ad = fabs(d);
   *F = f;
-  *est = fabs(C * C - F3 * iF1 * (1.0 / 6)) * ad * ad * ad;
-  *valid = isfinite(s + d) && F1 > 0 && den > 0;
+  *t1sq = F1sq;
+  *est = fabs((0.25 * F2) * F2 - F1 * F3 * (1.0 / 6)) * ad * ad * ad;
+  *valid = isfinite(s + d) && F1 > 0 && D > 0 && D < kInf;
   return d;
 }
 
 // Halley iteration on t(s) = tau inside a bracket; returns s, *evals = evaluations of t, and in
 // *last the G functions at the last point where t was evaluated.  The first step is taken apart
 // from the loop: nearly every pair converges in it, and it needs no bracket bookkeeping.
+//
+// The start s must lie on tau's side of 0 (s tau > 0, or s = 0 when tau = 0): the series start
+// does, and the learned start is accepted only on the same side as the series start.
 BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool* ok, KepG* last) {
   constexpr double kInf = __builtin_huge_val();
-  double lo = -kInf, hi = kInf;
-  if (K.tau > 0) lo = 0;
-  else hi = 0;
-  if (!(s > lo && s < hi)) s = K.tau * K.ir0;
+  double lo, hi;
   *ok = false;
   {
     const KepG g = kep_g(K.beta, s);
     *last = g;
-    double F, est;
+    double F, est, t1sq;
     bool valid;
-    const double d = kep_halley(K, g, &F, &est, &valid);
+    const double d = kep_halley(K, g, &F, &est, &t1sq, &valid);
     const double sn = s + d;
-    if ((valid && est <= 1e-16 * fabs(sn) && sn * K.tau >= 0) || F == 0) {  // root on the side of s = 0 that tau is
+    if ((valid && est <= 1e-16 * fabs(sn) * t1sq && sn * K.tau >= 0) || F == 0) {  // root on the side of s = 0 that tau is
       *ok = true;
       *evals = 1;
       return F == 0 ? s : sn;
     }
+    lo = K.tau > 0 ? 0.0 : -kInf;
+    hi = K.tau > 0 ? kInf : 0.0;
     if (F < 0) lo = s;
     else hi = s;
     s = (valid && sn > lo && sn < hi) ? sn : (hi == kInf) ? 2 * s + K.ir0 * fabs(K.tau)
@@ -455,9 +480,9 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
   while (k < kmax) {
     const KepG g = kep_g(K.beta, s);
     *last = g;
-    double F, est;
+    double F, est, t1sq;
     bool valid;
-    const double d = kep_halley(K, g, &F, &est, &valid);
+    const double d = kep_halley(K, g, &F, &est, &t1sq, &valid);
     k++;
     if (F == 0) {
       *ok = true;
@@ -470,7 +495,7 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
       break;
     }
     const double sn = s + d;
-    if (valid && est <= 1e-16 * fabs(sn)) {  // converged; a root at an end of the bracket may round just outside it
+    if (valid && est <= 1e-16 * fabs(sn) * t1sq) {  // converged; a root at an end of the bracket may round just outside it
       s = fmin(fmax(sn, lo), hi);
       *ok = true;
       break;
@@ -523,16 +548,18 @@ BIA_HDF void kep_output(const KepPair& K, double s, const KepG& e, const double 
   }
 }
 
-// learned start: trilinear lookup of the target log(s / s_start) / min(|a|, 1)^3 in a table over the
-// squashed features u_k = asinh(a_k / kFeatScale) / asinh(kFeatMax / kFeatScale), clamped to
-// [-1, 1]
+// learned start: trilinear lookup of the target log(s / s_start) / min(|a|, 1)^3 in a table
+// over the squashed features u_k = sign(a_k) log2(1 + |a_k| / kFeatScale) / log2(1 + kFeatMax /
+// kFeatScale), clamped to [-1, 1].  The logarithm is taken in single precision (one hardware
+// instruction): the features index a table of 64 nodes per axis.
 constexpr double kFeatScale = 1e-2;
 constexpr double kFeatMax = 1e2;
+constexpr float kFeatNorm = 13.287856641840545f;  // log2(1 + kFeatMax / kFeatScale)
 constexpr double kWeak = 1e-2;  // all |a_k| below this: the series start (one evaluation)
 
 BIA_HDF double feat_squash(double a) {
-  const double u = asinh(a * (1 / kFeatScale)) * (1 / 9.903487555036127);  // asinh(1e4)
-  return fmin(fmax(u, -1.0), 1.0);
+  const float u = fminf(log2f(1.0f + (float)(fabs(a) * (1 / kFeatScale))) * (1.0f / kFeatNorm), 1.0f);
+  return copysign((double)u, a);
 }
 
 BIA_HDF double table_lookup3(const LearnTable& T, const double a[3]) {
@@ -566,8 +593,9 @@ BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6]
   kep_features(K, a);
   double s = kep_series_start(K, a);
   if (MODE == kLearn && fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))) >= kWeak) {
-    const double sl = s * exp(table_lookup3(T, a) * kep_target_scale(a));
-    if (isfinite(sl) && (sl > 0) == (s > 0)) {
+    // exp in single precision: the start only has to be close to the root
+    const double sl = s * (double)exp2f((float)(table_lookup3(T, a) * kep_target_scale(a) * 1.4426950408889634));
+    if (isfinite(sl) && sl * s > 0) {
       s = sl;
       *fl |= kLearned;
     }

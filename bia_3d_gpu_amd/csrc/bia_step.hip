@@ -188,10 +188,10 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
 #pragma unroll 1
     for (int k = k0; k < k1; ++k) {
       const int jl = (lane + k) & (kWave - 1);
-      double d[6] = {0, 0, 0, 0, 0, 0};
-      double wi = 1, wj = 1;  // multiples of the half deltas credited to i and debited from j
       const bool active = row_ok & (jl < jn) & (!diag | (k < 32) | (lane < 32));
       if (active) {
+        double d[6];
+        double wi = 1, wj = 1;  // multiples of the half deltas credited to i and debited from j
         double eps = S.eps;
         if (COUP != kScalar) {
           const double mj = cmq[0][jl];
@@ -207,17 +207,19 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
         unsigned fl = 0;
         pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T);
         if (STATS) ps.add(br, it, fl);
-      }
 #pragma unroll
-      for (int c = 0; c < 6; c++) {
-        if (COUP == kScalar) {
-          acc[c] += d[c];
-          col[c] = rot_down(col[c] - d[c]);
-        } else {
-          acc[c] += wi * d[c];
-          col[c] = rot_down(col[c] - wj * d[c]);
+        for (int c = 0; c < 6; c++) {
+          if (COUP == kScalar) {
+            acc[c] += d[c];
+            col[c] -= d[c];
+          } else {
+            acc[c] += wi * d[c];
+            col[c] -= wj * d[c];
+          }
         }
       }
+#pragma unroll
+      for (int c = 0; c < 6; c++) col[c] = rot_down(col[c]);
     }
     if (diag) {
       // lane l holds the column sum of particle (l + 33) mod 64; particle l's is in lane l - 33

@@ -185,10 +185,11 @@ int bia_dim_rkf45_host(bia_ctx *ctx, int np, double eps, double *x0, double xe, 
 /* ---------------------------------------------------------------------------------------- */
 /* A 3 -> 64 -> 64 -> 1 GELU network proposes the starting universal anomaly of the Kepler drift
  * (BIA_MODE_LEARN) for pairs that are not weak over dt.  Inputs: the squashed dimensionless
- * numbers u_k = asinh(a_k / 1e-2) / asinh(1e4) of a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3,
+ * numbers u_k = sign(a_k) log2(1 + |a_k| / 1e-2) / log2(1 + 1e4) (single precision) of
+ * a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3,
  * a3 = beta dt^2 / r0^2 (eta = r.v, beta = 2 mu / r0 - v^2, zeta = mu - beta r0, mu = -eps);
- * output log(s / s_start) / min(|a|, 1)^3, the converged anomaly relative to the rational series
- * start.  Sample rows are float [u1, u2, u3, target].  Weights are 4481
+ * output log(s / s_start) / min(|a|, 1)^3, the converged anomaly relative to the rational
+ * series start.  Sample rows are float [u1, u2, u3, target].  Weights are 4481
  * floats: W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3.  Products run on BF16 MFMA with FP32
  * accumulation; the master weights are FP32. */
 int bia_mlp_nparam(void);
