@@ -251,8 +251,18 @@ __global__ __launch_bounds__(256) void k_reduce(const int2* __restrict__ chunks,
   const int c = blockIdx.y;
   const int ldp = S.nt * kWave;
   const double* p = part + S.poff + (size_t)c * ldp + il;
+  const size_t stride = (size_t)6 * ldp;
   double s = 0;
-  for (int K = 0; K < S.nt; ++K) s += p[(size_t)K * 6 * ldp];
+  int K = 0;
+  // 16 loads in flight per thread, then the adds in K order (the sum is the sequential one)
+  for (; K + 16 <= S.nt; K += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = __builtin_nontemporal_load(p + (size_t)(K + u) * stride);
+#pragma unroll
+    for (int u = 0; u < 16; u++) s += v[u];
+  }
+  for (; K < S.nt; ++K) s += __builtin_nontemporal_load(p + (size_t)K * stride);
   const int g = S.off + il;
   if (c < 3) s += st[(size_t)(3 + c) * lds + g] * dt;
   delta[(size_t)c * ldd + g] = s;
