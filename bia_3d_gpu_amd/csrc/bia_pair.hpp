@@ -557,30 +557,34 @@ constexpr double kFeatMax = 1e2;
 constexpr float kFeatNorm = 13.287856641840545f;  // log2(1 + kFeatMax / kFeatScale)
 constexpr double kWeak = 1e-2;  // all |a_k| below this: the series start (one evaluation)
 
-BIA_HDF double feat_squash(double a) {
+BIA_HDF float feat_squash(double a) {
   const float u = fminf(log2f(1.0f + (float)(fabs(a) * (1 / kFeatScale))) * (1.0f / kFeatNorm), 1.0f);
-  return copysign((double)u, a);
+  return copysignf(u, (float)a);
 }
 
-BIA_HDF double table_lookup3(const LearnTable& T, const double a[3]) {
+// log2(s / s_start) proposed by the table, in single precision (the start only has to be close
+// to the root): trilinear lookup of the target at the squashed features, times min(|a|, 1)^3 and
+// log2(e)
+BIA_HDF float learned_log2_ratio(const LearnTable& T, const double a[3]) {
   const int n = T.n;
-  double f[3];
+  float f[3];
   int i[3];
   for (int k = 0; k < 3; k++) {
-    double x = (feat_squash(a[k]) + 1) * 0.5 * (n - 1);
-    x = fmin(fmax(x, 0.0), n - 1.000001);
-    i[k] = (int)x;
-    f[k] = x - i[k];
+    const float x = fminf(fmaxf((feat_squash(a[k]) + 1.0f) * 0.5f * (float)(n - 1), 0.0f), (float)(n - 1));
+    const int ik = (int)x;
+    i[k] = ik < n - 2 ? ik : n - 2;  // x = n - 1 interpolates to the last node with weight 1
+    f[k] = x - (float)i[k];
   }
   const float* p = T.t + ((size_t)i[2] * n + i[1]) * n + i[0];
   const size_t s1 = n, s2 = (size_t)n * n;
-  const double c00 = (1 - f[0]) * p[0] + f[0] * p[1];
-  const double c01 = (1 - f[0]) * p[s1] + f[0] * p[s1 + 1];
-  const double c10 = (1 - f[0]) * p[s2] + f[0] * p[s2 + 1];
-  const double c11 = (1 - f[0]) * p[s2 + s1] + f[0] * p[s2 + s1 + 1];
-  const double c0 = (1 - f[1]) * c00 + f[1] * c01;
-  const double c1 = (1 - f[1]) * c10 + f[1] * c11;
-  return (1 - f[2]) * c0 + f[2] * c1;
+  const float c00 = (1 - f[0]) * p[0] + f[0] * p[1];
+  const float c01 = (1 - f[0]) * p[s1] + f[0] * p[s1 + 1];
+  const float c10 = (1 - f[0]) * p[s2] + f[0] * p[s2 + 1];
+  const float c11 = (1 - f[0]) * p[s2 + s1] + f[0] * p[s2 + s1 + 1];
+  const float c0 = (1 - f[1]) * c00 + f[1] * c01;
+  const float c1 = (1 - f[1]) * c10 + f[1] * c11;
+  const float m = fminf(sqrtf((float)(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])), 1.0f);
+  return ((1 - f[2]) * c0 + f[2] * c1) * (m * m * m) * 1.44269504f;
 }
 
 template <int MODE>
@@ -593,8 +597,7 @@ BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6]
   kep_features(K, a);
   double s = kep_series_start(K, a);
   if (MODE == kLearn && fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))) >= kWeak) {
-    // exp in single precision: the start only has to be close to the root
-    const double sl = s * (double)exp2f((float)(table_lookup3(T, a) * kep_target_scale(a) * 1.4426950408889634));
+    const double sl = s * (double)exp2f(learned_log2_ratio(T, a));
     if (isfinite(sl) && sl * s > 0) {
       s = sl;
       *fl |= kLearned;
