@@ -18,7 +18,9 @@ counters of an untimed step on the same input; bia_3d_gpu_amd.flops_per_step) ov
 duration, measured with HIP events recorded on the kernel's own stream.  ``energy`` gives
 |dE/E0| over the timed steps of each mode from the library's double-double invariants.
 ``cpu_baseline`` times the reference's own Fortran CPU pair loop (oracle/_ref) on bounded row
-samples, on one core and on one process per host core (at most 16).
+samples, on one core and on one process per host core (at most 16).  ``agreement_vs_rkf45``
+compares every mode's first step with an untimed per-pair RKF45 step of the same system (the
+reference's own integrator, an independent solution of the same pairs).
 """
 from __future__ import annotations
 
@@ -61,6 +63,7 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
     ctx.step_(s, d, eps, dt, mode=mode, itr=itr, stats=stats)
     torch.cuda.synchronize()
     st = stats.cpu().numpy().astype(np.uint64)
+    first_delta = d.cpu().numpy()  # the first step's deltas, compared across modes
     flops = flops_per_step(st, mode)
     tuned = None
     if autotune:  # persistent grid size and register-occupancy variant, timed on this input
@@ -92,7 +95,19 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
     e_0, e_1 = a[0] + a[1], b[0] + b[1]  # KE + PE = 2E/m
     kms = float(np.mean(kms))
     return dict(ms_per_step=ms, wall_s=wall, kernel_ms=kms, stats=st.tolist(), flops=flops, tuned=tuned,
-                tflops=flops / (kms / 1e3) / 1e12, dE_rel=abs(e_1 - e_0) / abs(e_0), dE_steps=steps)
+                tflops=flops / (kms / 1e3) / 1e12, dE_rel=abs(e_1 - e_0) / abs(e_0), dE_steps=steps,
+                first_delta=first_delta)
+
+
+def agreement(st_host, d, ref):
+    """Normwise agreement of one step's deltas with a reference step: max |(x + d) - (x + ref)| /
+    max |x + ref| over positions and over velocities, and the particles whose deltas differ by
+    more than 1e-6 and 1e-8 of the largest delta of that component."""
+    a, b = st_host + d, st_host + ref
+    off = (np.abs(d - ref) / np.abs(ref).max(axis=1, keepdims=True)).max(axis=0)
+    return {"positions": float(np.abs(a[:3] - b[:3]).max() / np.abs(b[:3]).max()),
+            "velocities": float(np.abs(a[3:] - b[3:]).max() / np.abs(b[3:]).max()),
+            "particles_off_1e-6": int((off > 1e-6).sum()), "particles_off_1e-8": int((off > 1e-8).sum())}
 
 
 def run_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 512):
@@ -251,7 +266,7 @@ def main():
 
     import torch
 
-    from bia_3d_gpu_amd import Context, MODE_GUESS, MODE_LEARN, MODE_PLAIN, learn
+    from bia_3d_gpu_amd import METHOD_RKF, Context, MODE_GUESS, MODE_LEARN, MODE_PLAIN, learn
 
     n = args.n
     eps = -2.0 / n
@@ -310,6 +325,19 @@ def main():
             f"{fixture['guess']['s_per_step'] * 1e3:.3f} ms/step, RKF {fixture['rkf']['s_per_step'] * 1e3:.1f} "
             f"ms/step, DIM {fixture['dim']['s']:.3f} s ({fixture['dim']['accepted_steps']} steps)")
     plain = res.get("plain")
+    # every mode's first step against an independent solution of the same pairs: the per-pair
+    # RKF45 (the reference's FEHL integrator and step control), one untimed step at full size
+    s0 = torch.from_numpy(st).to("cuda:0")
+    d0 = torch.empty_like(s0)
+    ctx.step_(s0, d0, eps, args.dt, method=METHOD_RKF, itr=args.itr)
+    torch.cuda.synchronize()
+    rkf_delta = d0.cpu().numpy()
+    del s0, d0
+    parity = {k: agreement(st, v["first_delta"], rkf_delta) for k, v in res.items()}
+    for k, v in parity.items():
+        log(f"{k} vs per-pair RKF45, first step: normwise positions {v['positions']:.2e}, velocities "
+            f"{v['velocities']:.2e}; particles off by more than 1e-6 (1e-8) of the largest delta: "
+            f"{v['particles_off_1e-6']} ({v['particles_off_1e-8']})")
     head_name = "learned" if "learned" in res else "guess" if "guess" in res else "plain"
     head = res[head_name]
     # roofline of the dominant kernel (the pair kernel of the reported mode), its own algorithm's flops
@@ -332,6 +360,7 @@ def main():
                                          "stats", "tuned")} for k, v in res.items()},
         "speedup_vs_plain": None if plain is None else {k: v["pairs_per_s"] / plain["pairs_per_s"]
                                                         for k, v in res.items()},
+        "agreement_vs_rkf45": parity,
         "training": train,
         "ensemble": ensemble,
         "reference_fixture": fixture,
