@@ -2,16 +2,22 @@
 // (reference dim/rkf_kernel.cu:29-53 force, :75-145 Fehlberg stages, :194-495 stepper;
 // driven by dim/dim_3D.f90:30).
 //
-// Device layout: SoA rows x,y,z,u,v,w of stride ld.  One Fehlberg stage is one kernel: a
-// workgroup of 16 waves evaluates the direct-sum acceleration of 64 particles at the stage input
-// w_{s-1} (each wave one contiguous range of j, the 16 partial sums added in wave order), stores
-// k_s[i] and in the same pass forms the particles' next stage input
-// w_s[i] = y0[i] + h sum_l b_sl k_l[i].  The last stage forms y_n and, for the error-controlled
-// step, the per-block maximum of |err_i| / (relerr (|y0|+|yn|)/2 + abserr), so only nblocks
-// doubles cross PCIe per trial step.
+// Device layout: SoA rows x,y,z,u,v,w of stride ld.  One Fehlberg stage is two kernels:
+//   k_dim_force   one wave per (block of 64 particles, chunk of jc source particles): the partial
+//                 direct-sum acceleration of its 64 particles over its chunk of j, in ascending j,
+//                 into part[chunk][3][i].  The chunk length is chosen so the launch has thousands
+//                 of waves whatever N (N = 1332: 21 x 42 waves of 32 sources each), so a small
+//                 system still spreads over the whole GPU;
+//   k_dim_combine one thread per particle and component: an acceleration component is the
+//                 chunks' partials added in chunk order (deterministic), k_s[i] is stored and the
+//                 next stage input w_s[i] = y0[i] + h sum_l b_sl k_l[i] formed in the same pass.  The last
+//                 stage forms y_n and, for the error-controlled step, the per-block maximum of
+//                 |err_i| / (relerr (|y0|+|yn|)/2 + abserr), so only nblocks doubles cross PCIe
+//                 per trial step.
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "bia_pair.hpp"
@@ -19,153 +25,182 @@
 
 namespace bia {
 
-constexpr int kDimRows = 64;   // particles per workgroup (one per lane)
-constexpr int kDimSplit = 16;  // waves per workgroup, each summing one contiguous range of j
-constexpr int kDimBlock = kDimRows * kDimSplit;
-constexpr int kDimPad = 256;   // row stride ld is a multiple of this
+constexpr int kDimRows = 64;        // particles per force wave (one per lane)
+constexpr int kDimMinChunk = 32;    // sources per force wave at least
+constexpr int kDimTargetWaves = 4096;
+constexpr int kDimCombine = 256;    // particles per combine block
+constexpr int kDimPad = 256;        // row stride ld is a multiple of this
 
 struct DimBufs {
   double* y[4];   // yl0, yln, yh0, yhn (swapped by pointer)
   double* k;      // 6 stages x 6 rows x ld
   double* w[2];   // ping-pong stage inputs
   double* er;     // 6 x ld
+  double* part;   // nchunk x 3 x ld partial accelerations
   double* bmax;   // per-block error maxima (+1 flag slot per block)
 };
 
-// direct-sum acceleration of the workgroup's 64 particles at the stage input w (rkf_kernel.cu:29-53:
-// a_i = sum_j eps (r_i - r_j) / (|r_i - r_j|^2 + 1e-99)^(3/2)).  Wave q sums its contiguous range of
-// j in ascending order (j is wave-uniform: scalar loads); the 16 partial sums are then added in
-// wave order, so the result is deterministic.  Valid in wave 0 only.
-__device__ __forceinline__ void dim_accel(int np, int ld, double eps, const double* __restrict__ w, int i,
-                                          double& ax, double& ay, double& az) {
-  __shared__ double part[kDimSplit][3][kDimRows];
-  const int lane = threadIdx.x % kDimRows, q = threadIdx.x / kDimRows;
+// partial direct-sum acceleration (rkf_kernel.cu:29-53: a_i = sum_j eps (r_i - r_j) /
+// (|r_i - r_j|^2 + 1e-99)^(3/2)) of the 64 particles of block blockIdx.x over the sources
+// [blockIdx.y jc, min(np, (blockIdx.y + 1) jc)), j ascending; the sources pass through LDS 64 at a
+// time (one coalesced load per lane, then broadcast reads)
+__global__ __launch_bounds__(kDimRows) void k_dim_force(int np, int ld, double eps, int jc,
+                                                        const double* __restrict__ w, double* __restrict__ part) {
+  __shared__ double src[3][kDimRows];
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * kDimRows + lane;
   const bool ok = i < np;
   const double xi = ok ? w[i] : 0, yi = ok ? w[ld + i] : 0, zi = ok ? w[2 * (size_t)ld + i] : 0;
-  const int jc = (np + kDimSplit - 1) / kDimSplit;
-  const int j0 = q * jc, j1 = min(np, j0 + jc);
+  const int j0 = blockIdx.y * jc, j1 = min(np, j0 + jc);
   double sx = 0, sy = 0, sz = 0;
-  for (int j = j0; j < j1; ++j) {
-    const double xij = xi - w[j], yij = yi - w[ld + j], zij = zi - w[2 * (size_t)ld + j];
-    const double rsq = xij * xij + yij * yij + zij * zij + 1e-99;  // rkf_kernel.cu:44
-    const double e = eps / (rsq * sqrt(rsq));
-    sx += xij * e;
-    sy += yij * e;
-    sz += zij * e;
-  }
-  part[q][0][lane] = sx;
-  part[q][1][lane] = sy;
-  part[q][2][lane] = sz;
-  __syncthreads();
-  ax = ay = az = 0;
-  if (q == 0) {
-    for (int p = 0; p < kDimSplit; p++) {
-      ax += part[p][0][lane];
-      ay += part[p][1][lane];
-      az += part[p][2][lane];
+  for (int jb = j0; jb < j1; jb += kDimRows) {
+    const int nj = min(kDimRows, j1 - jb);
+    __syncthreads();  // the previous batch's readers are done
+    if (lane < nj) {
+      src[0][lane] = w[jb + lane];
+      src[1][lane] = w[ld + jb + lane];
+      src[2][lane] = w[2 * (size_t)ld + jb + lane];
+    }
+    __syncthreads();
+    for (int jj = 0; jj < nj; ++jj) {
+      const double xij = xi - src[0][jj], yij = yi - src[1][jj], zij = zi - src[2][jj];
+      const double rsq = xij * xij + yij * yij + zij * zij + 1e-99;  // rkf_kernel.cu:44
+      const double e = eps / (rsq * sqrt(rsq));
+      sx += xij * e;
+      sy += yij * e;
+      sz += zij * e;
     }
   }
+  double* p = part + (size_t)blockIdx.y * 3 * ld + i;  // i < ld: the padding columns take zeros
+  p[0] = sx;
+  p[ld] = sy;
+  p[2 * (size_t)ld] = sz;
 }
 
-// stage S (1..6) of one Fehlberg step of size h from y0 (wave 0 of each workgroup writes):
+// acceleration component q of particle i: the nchunk partials in chunk order (loads issued 16
+// chunks at a time, the adds in chunk order)
+__device__ __forceinline__ double dim_accel(int nchunk, int ld, const double* __restrict__ part, int i, int q) {
+  const size_t cs = (size_t)3 * ld;
+  const double* p = part + (size_t)q * ld + i;
+  double a = 0;
+  int c = 0;
+  for (; c + 16 <= nchunk; c += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = p[(c + u) * cs];
+#pragma unroll
+    for (int u = 0; u < 16; u++) a += v[u];
+  }
+  for (; c < nchunk; ++c) a += p[c * cs];
+  return a;
+}
+
+// stage S (1..6) of one Fehlberg step of size h from y0, after k_dim_force on win; one thread per
+// particle and component (component blockIdx.y):
 //   k_S = f(win);  S < 6: wout = y0 + h sum_{l<=S} b_{S+1,l} k_l
 //                  S = 6: yn = y0 + h sum ga_l k_l; ERR: er, block max of |er|/et
 template <int S, bool ERR>
-__global__ __launch_bounds__(kDimBlock) void k_dim_stage(int np, int ld, double eps, double h,
-                                                         const double* __restrict__ y0,
-                                                         const double* __restrict__ win, double* __restrict__ k,
-                                                         double* __restrict__ wout, double* __restrict__ yn,
-                                                         double* __restrict__ er, double* __restrict__ bmax,
-                                                         double relerr, double abserr) {
+__global__ __launch_bounds__(kDimCombine) void k_dim_combine(int np, int ld, int nchunk, const double* __restrict__ part,
+                                                             double h, const double* __restrict__ y0,
+                                                             const double* __restrict__ win, double* __restrict__ k,
+                                                             double* __restrict__ wout, double* __restrict__ yn,
+                                                             double* __restrict__ er, double* __restrict__ bmax,
+                                                             double relerr, double abserr) {
   using namespace fehl;
-  const int i = blockIdx.x * kDimRows + threadIdx.x % kDimRows;
-  double ax, ay, az;
-  dim_accel(np, ld, eps, win, i, ax, ay, az);
-  if (threadIdx.x >= kDimRows) return;  // waves 1.. only helped with the sums
+  const int i = blockIdx.x * kDimCombine + threadIdx.x;
+  const int c = blockIdx.y;
   double emax = 0, bad = 0;
   if (i < np) {
     const size_t sl = (size_t)6 * ld;  // stride between stages
-    double* ks = k + (size_t)(S - 1) * sl;
-    ks[i] = win[3 * (size_t)ld + i];
-    ks[ld + i] = win[4 * (size_t)ld + i];
-    ks[2 * (size_t)ld + i] = win[5 * (size_t)ld + i];
-    ks[3 * (size_t)ld + i] = ax;
-    ks[4 * (size_t)ld + i] = ay;
-    ks[5 * (size_t)ld + i] = az;
-#pragma unroll
-    for (int c = 0; c < 6; c++) {
-      const size_t o = (size_t)c * ld + i;
-      const double k1 = k[o], k2 = (S >= 2) ? k[sl + o] : 0, k3 = (S >= 3) ? k[2 * sl + o] : 0,
-                   k4 = (S >= 4) ? k[3 * sl + o] : 0, k5 = (S >= 5) ? k[4 * sl + o] : 0,
-                   k6 = (S >= 6) ? k[5 * sl + o] : 0;
-      const double y = y0[o];
-      if (S == 1) wout[o] = y + h * B21 * k1;  // rkf_kernel.cu:82
-      if (S == 2) wout[o] = y + h * (B31 * k1 + B32 * k2);
-      if (S == 3) wout[o] = y + h * (B41 * k1 + B42 * k2 + B43 * k3);
-      if (S == 4) wout[o] = y + h * (B51 * k1 + B52 * k2 + B53 * k3 + B54 * k4);
-      if (S == 5) wout[o] = y + h * (B61 * k1 + B62 * k2 + B63 * k3 + B64 * k4 + B65 * k5);
-      if (S == 6) {
-        const double ynv = y + h * (GA1 * k1 + GA3 * k3 + GA4 * k4 + GA5 * k5 + GA6 * k6);
-        yn[o] = ynv;
-        if (ERR) {
-          const double ev = h * (DA1 * k1 + DA3 * k3 + DA4 * k4 + DA5 * k5 + DA6 * k6);
-          er[o] = ev;
-          const double et = relerr * (fabs(y) + fabs(ynv)) / 2 + abserr;
-          if (et == 0) bad = 1;
-          else emax = fmax(emax, fabs(ev) / et);
-        }
+    const size_t o = (size_t)c * ld + i;
+    const double ks = c < 3 ? win[3 * (size_t)ld + o] : dim_accel(nchunk, ld, part, i, c - 3);
+    k[(size_t)(S - 1) * sl + o] = ks;
+    const double k1 = S == 1 ? ks : k[o], k2 = (S >= 2) ? (S == 2 ? ks : k[sl + o]) : 0,
+                 k3 = (S >= 3) ? (S == 3 ? ks : k[2 * sl + o]) : 0, k4 = (S >= 4) ? (S == 4 ? ks : k[3 * sl + o]) : 0,
+                 k5 = (S >= 5) ? (S == 5 ? ks : k[4 * sl + o]) : 0, k6 = (S >= 6) ? ks : 0;
+    const double y = y0[o];
+    if (S == 1) wout[o] = y + h * B21 * k1;  // rkf_kernel.cu:82
+    if (S == 2) wout[o] = y + h * (B31 * k1 + B32 * k2);
+    if (S == 3) wout[o] = y + h * (B41 * k1 + B42 * k2 + B43 * k3);
+    if (S == 4) wout[o] = y + h * (B51 * k1 + B52 * k2 + B53 * k3 + B54 * k4);
+    if (S == 5) wout[o] = y + h * (B61 * k1 + B62 * k2 + B63 * k3 + B64 * k4 + B65 * k5);
+    if (S == 6) {
+      const double ynv = y + h * (GA1 * k1 + GA3 * k3 + GA4 * k4 + GA5 * k5 + GA6 * k6);
+      yn[o] = ynv;
+      if (ERR) {
+        const double ev = h * (DA1 * k1 + DA3 * k3 + DA4 * k4 + DA5 * k5 + DA6 * k6);
+        er[o] = ev;
+        const double et = relerr * (fabs(y) + fabs(ynv)) / 2 + abserr;
+        if (et == 0) bad = 1;
+        else emax = fabs(ev) / et;
       }
     }
   }
   if (S == 6 && ERR) {
+    __shared__ double red[2][kDimCombine / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       emax = fmax(emax, __shfl_xor(emax, off, 64));
       bad = fmax(bad, __shfl_xor(bad, off, 64));
     }
+    if (threadIdx.x % 64 == 0) {
+      red[0][threadIdx.x / 64] = emax;
+      red[1][threadIdx.x / 64] = bad;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-      bmax[2 * blockIdx.x] = emax;
-      bmax[2 * blockIdx.x + 1] = bad;
+      for (int q = 1; q < kDimCombine / 64; q++) {
+        emax = fmax(emax, red[0][q]);
+        bad = fmax(bad, red[1][q]);
+      }
+      const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+      bmax[2 * b] = emax;
+      bmax[2 * b + 1] = bad;
     }
   }
 }
 
-// derivative only (initial step-size estimate): f = (v, a) at y
-__global__ __launch_bounds__(kDimBlock) void k_dim_func(int np, int ld, double eps, const double* __restrict__ y,
-                                                        double* __restrict__ f) {
-  const int i = blockIdx.x * kDimRows + threadIdx.x % kDimRows;
-  double ax, ay, az;
-  dim_accel(np, ld, eps, y, i, ax, ay, az);
-  if (threadIdx.x < kDimRows && i < np) {
-    f[i] = y[3 * (size_t)ld + i];
-    f[ld + i] = y[4 * (size_t)ld + i];
-    f[2 * (size_t)ld + i] = y[5 * (size_t)ld + i];
-    f[3 * (size_t)ld + i] = ax;
-    f[4 * (size_t)ld + i] = ay;
-    f[5 * (size_t)ld + i] = az;
-  }
+// derivative only (initial step-size estimate): f = (v, a) at y, after k_dim_force on y
+__global__ __launch_bounds__(kDimCombine) void k_dim_func(int np, int ld, int nchunk, const double* __restrict__ part,
+                                                          const double* __restrict__ y, double* __restrict__ f) {
+  const int i = blockIdx.x * kDimCombine + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= np) return;
+  f[(size_t)c * ld + i] = c < 3 ? y[(size_t)(3 + c) * ld + i] : dim_accel(nchunk, ld, part, i, c - 3);
 }
 
 struct DimRun {
   bia_ctx* ctx;
-  int np, ld, nb;
+  int np, ld, nb;   // nb: force blocks of kDimRows particles
+  int jc, nchunk;   // sources per force wave, chunks of j
+  int ncb;          // combine blocks of kDimCombine particles
   double eps, relerr, abserr;
   DimBufs b;
   int evals = 0;
 
+  void force(const double* w) {
+    hipLaunchKernelGGL(k_dim_force, dim3(nb, nchunk), dim3(kDimRows), 0, ctx->stream, np, ld, eps, jc, w, b.part);
+  }
+
   int launch_fehl(const double* y0, double* yn, double h, bool err) {
-    dim3 g(nb), t(kDimBlock);
+    const dim3 g(ncb, 6), t(kDimCombine);
     hipStream_t s = ctx->stream;
     double *w0 = b.w[0], *w1 = b.w[1];
-    hipLaunchKernelGGL((k_dim_stage<1, false>), g, t, 0, s, np, ld, eps, h, y0, y0, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
-    hipLaunchKernelGGL((k_dim_stage<2, false>), g, t, 0, s, np, ld, eps, h, y0, w0, b.k, w1, nullptr, nullptr, nullptr, relerr, abserr);
-    hipLaunchKernelGGL((k_dim_stage<3, false>), g, t, 0, s, np, ld, eps, h, y0, w1, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
-    hipLaunchKernelGGL((k_dim_stage<4, false>), g, t, 0, s, np, ld, eps, h, y0, w0, b.k, w1, nullptr, nullptr, nullptr, relerr, abserr);
-    hipLaunchKernelGGL((k_dim_stage<5, false>), g, t, 0, s, np, ld, eps, h, y0, w1, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
+    force(y0);
+    hipLaunchKernelGGL((k_dim_combine<1, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, y0, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
+    force(w0);
+    hipLaunchKernelGGL((k_dim_combine<2, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w0, b.k, w1, nullptr, nullptr, nullptr, relerr, abserr);
+    force(w1);
+    hipLaunchKernelGGL((k_dim_combine<3, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w1, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
+    force(w0);
+    hipLaunchKernelGGL((k_dim_combine<4, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w0, b.k, w1, nullptr, nullptr, nullptr, relerr, abserr);
+    force(w1);
+    hipLaunchKernelGGL((k_dim_combine<5, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w1, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
+    force(w0);
     if (err)
-      hipLaunchKernelGGL((k_dim_stage<6, true>), g, t, 0, s, np, ld, eps, h, y0, w0, b.k, nullptr, yn, b.er, b.bmax, relerr, abserr);
+      hipLaunchKernelGGL((k_dim_combine<6, true>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w0, b.k, nullptr, yn, b.er, b.bmax, relerr, abserr);
     else
-      hipLaunchKernelGGL((k_dim_stage<6, false>), g, t, 0, s, np, ld, eps, h, y0, w0, b.k, nullptr, yn, nullptr, nullptr, relerr, abserr);
+      hipLaunchKernelGGL((k_dim_combine<6, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w0, b.k, nullptr, yn, nullptr, nullptr, relerr, abserr);
     evals += 6;
     BIA_HIP_TRY(ctx, hipGetLastError());
     return BIA_OK;
@@ -180,18 +215,24 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
   if (np < 1 || !x0 || !y || !hio || !ier) return BIA_ERR_ARG;
   const int ld = ((np + kDimPad - 1) / kDimPad) * kDimPad;
   const int nb = (np + kDimRows - 1) / kDimRows;
+  // chunks of j: enough force waves to fill the GPU, at least kDimMinChunk sources each
+  const int nchunk = std::max(1, std::min((kDimTargetWaves + nb - 1) / nb, (np + kDimMinChunk - 1) / kDimMinChunk));
+  const int jc = (np + nchunk - 1) / nchunk;
+  const int ncb = (np + kDimCombine - 1) / kDimCombine;
   const size_t row = (size_t)6 * ld;
-  // layout: yl0 yln yh0 yhn | k (6 stages) | w0 w1 | er | bmax
-  const size_t ndbl = 4 * row + 6 * row + 2 * row + row + 2 * (size_t)nb;
+  // layout: yl0 yln yh0 yhn | k (6 stages) | w0 w1 | er | part | bmax
+  const size_t npart = (size_t)nchunk * 3 * ld;
+  const size_t ndbl = 4 * row + 6 * row + 2 * row + row + npart + 2 * 6 * (size_t)ncb;
   BIA_TRY(ctx, ctx->dim_work.ensure(ndbl * sizeof(double)));
   double* base = ctx->dim_work.as<double>();
-  DimRun R{ctx, np, ld, nb, eps, relerr, abserr, {}, 0};
+  DimRun R{ctx, np, ld, nb, jc, nchunk, ncb, eps, relerr, abserr, {}, 0};
   for (int q = 0; q < 4; q++) R.b.y[q] = base + q * row;
   R.b.k = base + 4 * row;
   R.b.w[0] = base + 10 * row;
   R.b.w[1] = base + 11 * row;
   R.b.er = base + 12 * row;
-  R.b.bmax = base + 13 * row;
+  R.b.part = base + 13 * row;
+  R.b.bmax = base + 13 * row + npart;
   double *yl0 = R.b.y[0], *yln = R.b.y[1], *yh0 = R.b.y[2], *yhn = R.b.y[3];
   hipStream_t s = ctx->stream;
   // AoS host -> SoA device (padding rows zeroed once)
@@ -202,7 +243,8 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
   BIA_HIP_TRY(ctx, hipMemcpyAsync(yl0, yh0, row * sizeof(double), hipMemcpyDeviceToDevice, s));
   double H = *hio;
   if (init && *init == 1) {  // initial step from the derivative, :237-265
-    hipLaunchKernelGGL(k_dim_func, dim3(nb), dim3(kDimBlock), 0, s, np, ld, eps, yl0, R.b.k);
+    R.force(yl0);
+    hipLaunchKernelGGL(k_dim_func, dim3(ncb, 6), dim3(kDimCombine), 0, s, np, ld, nchunk, R.b.part, yl0, R.b.k);
     BIA_HIP_TRY(ctx, hipGetLastError());
     std::vector<double> f(row);
     BIA_HIP_TRY(ctx, hipMemcpyAsync(f.data(), R.b.k, row * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -222,7 +264,7 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
     *init = 2;
     R.evals = 1;
   }
-  std::vector<double> bm(2 * (size_t)nb);
+  std::vector<double> bm(2 * 6 * (size_t)ncb);
   std::vector<double> p0(6);
   auto trace_row = [&](double t, const double* dy) -> int {
     if (!trace) return BIA_OK;
@@ -247,7 +289,7 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
     BIA_HIP_TRY(ctx, hipStreamSynchronize(s));
     double erret = 0;
     bool bad = false;
-    for (int q = 0; q < nb; q++) {
+    for (int q = 0; q < 6 * ncb; q++) {
       erret = fmax(erret, bm[2 * q]);
       bad = bad || bm[2 * q + 1] != 0;
     }
