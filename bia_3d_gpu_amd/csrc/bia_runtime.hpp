@@ -122,8 +122,9 @@ struct bia_ctx {
   bia::DevBuf inv_state, inv_part, inv_out, inv_tiles, inv_sys;
   std::vector<long long> inv_key;
   int inv_ntiles = 0, inv_npe = 0, inv_nlin = 0, inv_maxnpe = 0, inv_maxnlin = 0;
-  // DIM workspace
+  // DIM workspace; dim_h: pinned staging of the step sizes of the captured steps
   bia::DevBuf dim_work, dim_ctl;
+  bia::HostBuf dim_h;
   // learned initialiser: fp32 master weights + Adam moments, training workspace, tabulated output
   bia::DevBuf mlp_w, mlp_ws, mlp_table;
   int mlp_init = 0;   // weights present

@@ -32,7 +32,7 @@ constexpr int kDimCombine = 256;    // particles per combine block
 constexpr int kDimPad = 256;        // row stride ld is a multiple of this
 
 struct DimBufs {
-  double* y[4];   // yl0, yln, yh0, yhn (swapped by pointer)
+  double* y[4];   // yl0, yln (trial step), yh0 (the high solution), yh1 (between its two half steps)
   double* k;      // 6 stages x 6 rows x ld
   double* w[2];   // ping-pong stage inputs
   double* er;     // 6 x ld
@@ -101,7 +101,8 @@ __device__ __forceinline__ double dim_accel(int nchunk, int ld, const double* __
 //                  S = 6: yn = y0 + h sum ga_l k_l; ERR: er, block max of |er|/et
 template <int S, bool ERR>
 __global__ __launch_bounds__(kDimCombine) void k_dim_combine(int np, int ld, int nchunk, const double* __restrict__ part,
-                                                             double h, const double* __restrict__ y0,
+                                                             const double* __restrict__ hp, double hs,
+                                                             const double* __restrict__ y0,
                                                              const double* __restrict__ win, double* __restrict__ k,
                                                              double* __restrict__ wout, double* __restrict__ yn,
                                                              double* __restrict__ er, double* __restrict__ bmax,
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(kDimCombine) void k_dim_combine(int np, int ld, int
   using namespace fehl;
   const int i = blockIdx.x * kDimCombine + threadIdx.x;
   const int c = blockIdx.y;
+  const double h = hs * hp[0];  // hs = 1 or 1/2 (exact)
   double emax = 0, bad = 0;
   if (i < np) {
     const size_t sl = (size_t)6 * ld;  // stride between stages
@@ -160,6 +162,12 @@ __global__ __launch_bounds__(kDimCombine) void k_dim_combine(int np, int ld, int
   }
 }
 
+// dst = src over 6 rows of stride ld (the accepted low solution becomes the next step's start)
+__global__ __launch_bounds__(256) void k_dim_copy(int n, const double* __restrict__ src, double* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 // derivative only (initial step-size estimate): f = (v, a) at y, after k_dim_force on y
 __global__ __launch_bounds__(kDimCombine) void k_dim_func(int np, int ld, int nchunk, const double* __restrict__ part,
                                                           const double* __restrict__ y, double* __restrict__ f) {
@@ -169,6 +177,10 @@ __global__ __launch_bounds__(kDimCombine) void k_dim_func(int np, int ld, int nc
   f[(size_t)c * ld + i] = c < 3 ? y[(size_t)(3 + c) * ld + i] : dim_accel(nchunk, ld, part, i, c - 3);
 }
 
+// One call's launch plan.  A trial step (error-controlled Fehlberg step of yl0 into yln) and an
+// accepted step (two half steps of yh0 in place through yh1, then yl0 = yln) are each captured
+// once into a hipGraph and replayed; the step size is read by the kernels from device memory
+// (hdev), written before each replay, so the graphs do not change from step to step.
 struct DimRun {
   bia_ctx* ctx;
   int np, ld, nb;   // nb: force blocks of kDimRows particles
@@ -176,35 +188,93 @@ struct DimRun {
   int ncb;          // combine blocks of kDimCombine particles
   double eps, relerr, abserr;
   DimBufs b;
+  double* hdev;     // device: step size of the trial step [0] and of the accepted step [1]
+  double* hpin;     // pinned host staging of the same two values
   int evals = 0;
+  hipGraphExec_t trial = nullptr, accept = nullptr;
 
   void force(const double* w) {
     hipLaunchKernelGGL(k_dim_force, dim3(nb, nchunk), dim3(kDimRows), 0, ctx->stream, np, ld, eps, jc, w, b.part);
   }
 
-  int launch_fehl(const double* y0, double* yn, double h, bool err) {
+  // Fehlberg step of size hs * hdev from y0 into yn (yn may be y0: each thread reads its own
+  // elements of y0 before stage 6 writes them, and the force kernels that read all of y0 run
+  // before)
+  void fehl(const double* y0, double* yn, const double* hp, double hs, bool err) {
     const dim3 g(ncb, 6), t(kDimCombine);
     hipStream_t s = ctx->stream;
     double *w0 = b.w[0], *w1 = b.w[1];
     force(y0);
-    hipLaunchKernelGGL((k_dim_combine<1, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, y0, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
+    hipLaunchKernelGGL((k_dim_combine<1, false>), g, t, 0, s, np, ld, nchunk, b.part, hp, hs, y0, y0, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
     force(w0);
-    hipLaunchKernelGGL((k_dim_combine<2, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w0, b.k, w1, nullptr, nullptr, nullptr, relerr, abserr);
+    hipLaunchKernelGGL((k_dim_combine<2, false>), g, t, 0, s, np, ld, nchunk, b.part, hp, hs, y0, w0, b.k, w1, nullptr, nullptr, nullptr, relerr, abserr);
     force(w1);
-    hipLaunchKernelGGL((k_dim_combine<3, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w1, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
+    hipLaunchKernelGGL((k_dim_combine<3, false>), g, t, 0, s, np, ld, nchunk, b.part, hp, hs, y0, w1, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
     force(w0);
-    hipLaunchKernelGGL((k_dim_combine<4, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w0, b.k, w1, nullptr, nullptr, nullptr, relerr, abserr);
+    hipLaunchKernelGGL((k_dim_combine<4, false>), g, t, 0, s, np, ld, nchunk, b.part, hp, hs, y0, w0, b.k, w1, nullptr, nullptr, nullptr, relerr, abserr);
     force(w1);
-    hipLaunchKernelGGL((k_dim_combine<5, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w1, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
+    hipLaunchKernelGGL((k_dim_combine<5, false>), g, t, 0, s, np, ld, nchunk, b.part, hp, hs, y0, w1, b.k, w0, nullptr, nullptr, nullptr, relerr, abserr);
     force(w0);
     if (err)
-      hipLaunchKernelGGL((k_dim_combine<6, true>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w0, b.k, nullptr, yn, b.er, b.bmax, relerr, abserr);
+      hipLaunchKernelGGL((k_dim_combine<6, true>), g, t, 0, s, np, ld, nchunk, b.part, hp, hs, y0, w0, b.k, nullptr, yn, b.er, b.bmax, relerr, abserr);
     else
-      hipLaunchKernelGGL((k_dim_combine<6, false>), g, t, 0, s, np, ld, nchunk, b.part, h, y0, w0, b.k, nullptr, yn, nullptr, nullptr, relerr, abserr);
-    evals += 6;
-    BIA_HIP_TRY(ctx, hipGetLastError());
+      hipLaunchKernelGGL((k_dim_combine<6, false>), g, t, 0, s, np, ld, nchunk, b.part, hp, hs, y0, w0, b.k, nullptr, yn, nullptr, nullptr, relerr, abserr);
+  }
+
+  int capture(hipGraphExec_t* exec, bool accept_step) {
+    hipStream_t s = ctx->stream;
+    BIA_HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    if (accept_step) {
+      fehl(b.y[2], b.y[3], hdev + 1, 0.5, false);
+      fehl(b.y[3], b.y[2], hdev + 1, 0.5, false);
+      const int n6 = 6 * ld;
+      hipLaunchKernelGGL(k_dim_copy, dim3((n6 + 255) / 256), dim3(256), 0, s, n6, b.y[1], b.y[0]);
+    } else {
+      fehl(b.y[0], b.y[1], hdev, 1.0, true);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      BIA_HIP_TRY(ctx, e);
+    }
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    BIA_HIP_TRY(ctx, ei);
     return BIA_OK;
   }
+
+  // stage the step size of slot q (0 trial, 1 accepted step) and replay that step's graph.  The
+  // host rewrites a slot only after the stream has been synchronised past its last copy: each
+  // trial step is followed by a synchronisation before the next accepted or trial step is issued.
+  int run(int q, double h, int nevals) {
+    hpin[q] = h;
+    BIA_HIP_TRY(ctx, hipMemcpyAsync(hdev + q, hpin + q, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    BIA_HIP_TRY(ctx, hipGraphLaunch(q == 0 ? trial : accept, ctx->stream));
+    evals += nevals;
+    return BIA_OK;
+  }
+
+  ~DimRun() {
+    if (trial) (void)hipGraphExecDestroy(trial);
+    if (accept) (void)hipGraphExecDestroy(accept);
+  }
+};
+
+// graph capture is not possible on the null stream: a call on it runs on the context's private
+// blocking stream, which stays ordered with null-stream work on both sides
+struct CaptureStream {
+  bia_ctx* ctx;
+  hipStream_t user;
+  explicit CaptureStream(bia_ctx* c) : ctx(c), user(c->stream) {}
+  int enter() {
+    if (!user) {
+      if (!ctx->cap_stream) BIA_HIP_TRY(ctx, hipStreamCreate(&ctx->cap_stream));
+      ctx->stream = ctx->cap_stream;
+    }
+    return BIA_OK;
+  }
+  ~CaptureStream() { ctx->stream = user; }
 };
 
 static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double xe, double* y, int* init,
@@ -213,6 +283,8 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
   const double EPSMIN = 1e-15;  // rkf_kernel.cu:201-202
   const int ITEMAX = 10000;
   if (np < 1 || !x0 || !y || !hio || !ier) return BIA_ERR_ARG;
+  CaptureStream cs(ctx);
+  BIA_TRY(ctx, cs.enter());
   const int ld = ((np + kDimPad - 1) / kDimPad) * kDimPad;
   const int nb = (np + kDimRows - 1) / kDimRows;
   // chunks of j: enough force waves to fill the GPU, at least kDimMinChunk sources each
@@ -220,12 +292,13 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
   const int jc = (np + nchunk - 1) / nchunk;
   const int ncb = (np + kDimCombine - 1) / kDimCombine;
   const size_t row = (size_t)6 * ld;
-  // layout: yl0 yln yh0 yhn | k (6 stages) | w0 w1 | er | part | bmax
+  // layout: yl0 yln yh0 yh1 | k (6 stages) | w0 w1 | er | part | bmax | h (2)
   const size_t npart = (size_t)nchunk * 3 * ld;
-  const size_t ndbl = 4 * row + 6 * row + 2 * row + row + npart + 2 * 6 * (size_t)ncb;
+  const size_t ndbl = 4 * row + 6 * row + 2 * row + row + npart + 2 * 6 * (size_t)ncb + 2;
   BIA_TRY(ctx, ctx->dim_work.ensure(ndbl * sizeof(double)));
+  BIA_TRY(ctx, ctx->dim_h.ensure(2 * sizeof(double)));
   double* base = ctx->dim_work.as<double>();
-  DimRun R{ctx, np, ld, nb, jc, nchunk, ncb, eps, relerr, abserr, {}, 0};
+  DimRun R{ctx, np, ld, nb, jc, nchunk, ncb, eps, relerr, abserr, {}, nullptr, ctx->dim_h.host<double>()};
   for (int q = 0; q < 4; q++) R.b.y[q] = base + q * row;
   R.b.k = base + 4 * row;
   R.b.w[0] = base + 10 * row;
@@ -233,7 +306,8 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
   R.b.er = base + 12 * row;
   R.b.part = base + 13 * row;
   R.b.bmax = base + 13 * row + npart;
-  double *yl0 = R.b.y[0], *yln = R.b.y[1], *yh0 = R.b.y[2], *yhn = R.b.y[3];
+  R.hdev = R.b.bmax + 2 * 6 * (size_t)ncb;
+  double *yl0 = R.b.y[0], *yh0 = R.b.y[2];
   hipStream_t s = ctx->stream;
   // AoS host -> SoA device (padding rows zeroed once)
   std::vector<double> soa(row, 0.0);
@@ -282,9 +356,11 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
   int acc = 0, rej = 0;
   *ier = 10000;
   double X = *x0;
+  BIA_TRY(ctx, R.capture(&R.trial, false));
+  BIA_TRY(ctx, R.capture(&R.accept, true));
+  double* yhi = R.b.y[2];  // the high solution, advanced in place by the accepted step
   for (int iter = 0; iter < ITEMAX; iter++) {
-    const double h = H;
-    BIA_TRY(ctx, R.launch_fehl(yl0, yln, h, true));
+    BIA_TRY(ctx, R.run(0, H, 6));
     BIA_HIP_TRY(ctx, hipMemcpyAsync(bm.data(), R.b.bmax, bm.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     BIA_HIP_TRY(ctx, hipStreamSynchronize(s));
     double erret = 0;
@@ -305,29 +381,23 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
         break;
       }
     } else if (X + H < xe) {
-      // high solution: two half steps, :318-363
-      BIA_TRY(ctx, R.launch_fehl(yh0, yhn, H / 2, false));
-      std::swap(yh0, yhn);
-      BIA_TRY(ctx, R.launch_fehl(yh0, yhn, H / 2, false));
+      // high solution: two half steps, then the low solution moves on (:318-363)
+      BIA_TRY(ctx, R.run(1, H, 12));
       X += H;
       *x0 = X;
-      std::swap(yl0, yln);
-      std::swap(yh0, yhn);
       acc++;
-      BIA_TRY(ctx, trace_row(X, yh0));
+      BIA_TRY(ctx, trace_row(X, yhi));
       H *= (erret <= 1.889568e-4) ? 5.0 : 0.9 / pow(erret, 0.2);
     } else {
       H = xe - X;  // last phase, :387-483
-      BIA_TRY(ctx, R.launch_fehl(yl0, yln, H, true));
-      BIA_TRY(ctx, R.launch_fehl(yh0, yhn, H / 2, false));
-      std::swap(yh0, yhn);
-      BIA_TRY(ctx, R.launch_fehl(yh0, yhn, H / 2, false));
+      BIA_TRY(ctx, R.run(0, H, 6));
+      BIA_TRY(ctx, R.run(1, H, 12));
       acc++;
       *x0 = xe;
       // results back to AoS host arrays
       std::vector<double> outh(row), outl(row);
-      BIA_HIP_TRY(ctx, hipMemcpyAsync(outh.data(), yhn, row * sizeof(double), hipMemcpyDeviceToHost, s));
-      BIA_HIP_TRY(ctx, hipMemcpyAsync(outl.data(), yln, row * sizeof(double), hipMemcpyDeviceToHost, s));
+      BIA_HIP_TRY(ctx, hipMemcpyAsync(outh.data(), yhi, row * sizeof(double), hipMemcpyDeviceToHost, s));
+      BIA_HIP_TRY(ctx, hipMemcpyAsync(outl.data(), R.b.y[1], row * sizeof(double), hipMemcpyDeviceToHost, s));
       BIA_HIP_TRY(ctx, hipStreamSynchronize(s));
       for (int i = 0; i < np; i++)
         for (int c = 0; c < 6; c++) {
@@ -335,7 +405,7 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
           y[(size_t)6 * i + c] = vh;
           if (ese) ese[(size_t)6 * i + c] = (vl - vh) / 31;
         }
-      BIA_TRY(ctx, trace_row(xe, yhn));
+      BIA_TRY(ctx, trace_row(xe, yhi));
       *ier = 0;
       break;
     }
