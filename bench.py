@@ -143,6 +143,38 @@ def run_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 512):
                           "evaluations_per_pair": h["evaluations_per_pair"], "iteration_cap": h["iteration cap"]}}
 
 
+def run_mass_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 4096):
+    """Side measurement: 64 Plummer systems, N log-uniform in [4096, 32768] (9.6e9 pairs per
+    step), masses log-uniform in [0.1, 10] normalised to 1 per system, gravity coupling
+    eps_ij = -(m_i + m_j) with the centre-of-mass split, dt = 5e-4; steps replayed from a captured
+    hipGraph."""
+    import torch
+
+    from bia_3d_gpu_amd import COUPLING_GRAVITY, ic
+
+    sizes = [int(n) for n in ic.ensemble_sizes(64, 4096, 32768, seed)]
+    rng = np.random.default_rng(seed + 1)
+    states, masses = [], []
+    for k, n in enumerate(sizes):
+        states.append(ic.plummer_system(n, seed + 904 + k))
+        m = np.exp(rng.uniform(np.log(0.1), np.log(10), n))
+        masses.append(m / m.sum())
+    s = torch.from_numpy(np.ascontiguousarray(np.concatenate(states, axis=1))).to("cuda:0")
+    mc = torch.from_numpy(np.concatenate(masses)).to("cuda:0")
+    d = torch.empty_like(s)
+    kw = dict(dt=5e-4, mode=mode, coupling=COUPLING_GRAVITY, coup=1.0, mass=mc)
+    ctx.autotune_(s, d, sizes, **kw)
+    ctx.run_steps_(s, d, sizes, max(warmup, 1), use_graph=True, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.run_steps_(s, d, sizes, steps, use_graph=True, **kw)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    pairs = float(sum(n * (n - 1) // 2 for n in sizes))
+    return {"systems": 64, "particles": int(sum(sizes)), "pairs_per_step": pairs, "ms_per_step": sec * 1e3,
+            "pairs_per_s": pairs / sec, "steps": steps, "graph": True, "coupling": "gravity, mass spectrum"}
+
+
 def run_fixture(ctx, steps: int):
     """Side measurement on the reference's own N = 1332 input (tests/golden/0001332_10^0.bbb), the
     only case with published reference timings (one BIA step, dt = 1, eps = -6e-7, TITAN V: ALG
@@ -250,7 +282,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learn", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--no-ensemble", action="store_true", help="skip the 512-system ensemble side measurement")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble side measurements")
     ap.add_argument("--no-fixture", action="store_true", help="skip the N=1332 reference-fixture side measurement")
     ap.add_argument("--modes", default="plain,guess,learned",
                     help="comma-separated subset of plain,guess,learned (profiling runs)")
@@ -319,6 +351,13 @@ def main():
         log(f"ensemble (512 systems, {ensemble['particles']} particles): {ensemble['ms_per_step']:.2f} ms/step, "
             f"{ensemble['pairs_per_s']:.3e} pairs/s; monitored driver {ensemble['monitored']['ms_per_step']:.2f} "
             f"ms/step, max |dE/E0| {ensemble['monitored']['max_abs_energy_error']:.2e}")
+    mass_ensemble = None
+    if not args.no_ensemble:
+        emode = MODE_LEARN if "learned" in res else MODE_GUESS
+        mass_ensemble = run_mass_ensemble(ctx, emode, max(args.steps, 3), args.warmup)
+        mass_ensemble["mode"] = "learned" if emode == MODE_LEARN else "guess"
+        log(f"mass-spectrum ensemble (64 systems, {mass_ensemble['particles']} particles, gravity coupling): "
+            f"{mass_ensemble['ms_per_step']:.2f} ms/step, {mass_ensemble['pairs_per_s']:.3e} pairs/s")
     fixture = None if args.no_fixture else run_fixture(ctx, 20)
     if fixture:
         log(f"reference fixture N=1332: plain {fixture['plain']['s_per_step'] * 1e3:.3f} ms/step, guess "
@@ -363,6 +402,7 @@ def main():
         "agreement_vs_rkf45": parity,
         "training": train,
         "ensemble": ensemble,
+        "mass_spectrum_ensemble": mass_ensemble,
         "reference_fixture": fixture,
     }
     print(json.dumps(out), flush=True)

@@ -144,3 +144,58 @@ def test_benchmark_system_65536_against_rkf45(ctx):
               f"iteration caps {stats[name][10]}")
     assert res["kepler"][0] == 0 and res["kepler"][2] < 1e-11 and res["kepler"][3] < 1e-12
     assert stats["kepler"][10] == 0 and stats["kepler"][13] == 0
+
+
+def test_mass_spectrum_ensemble_64_systems(ctx):
+    """64 Plummer systems, N log-uniform in [4096, 32768] (9.6e9 pairs per step), masses
+    log-uniform in [0.1, 10] normalised to 1 per system, gravity coupling with the centre-of-mass
+    split, dt = 5e-4, one launch per step.  The first step against the per-pair RKF45 of the same
+    pairs (no particle off by more than 1e-6 of the largest delta), then three Kepler-drift steps:
+    the mass-weighted momentum of every system is kept to rounding and no solver flag is raised."""
+    import torch
+    from bia_3d_gpu_amd import COUPLING_GRAVITY, METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_PLAIN, STATS_LEN, ic
+    sizes = [int(n) for n in ic.ensemble_sizes(64, 4096, 32768, seed=4096)]
+    rng = np.random.default_rng(4097)
+    states, masses = [], []
+    for k, n in enumerate(sizes):
+        states.append(ic.plummer_system(n, seed=5000 + k))
+        m = np.exp(rng.uniform(np.log(0.1), np.log(10), n))
+        masses.append(m / m.sum())
+    st = np.ascontiguousarray(np.concatenate(states, axis=1))
+    m_all = np.concatenate(masses)
+    s = torch.from_numpy(st).cuda()
+    mc = torch.from_numpy(m_all).cuda()
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    out, stats = {}, {}
+    for name, method, mode in (("kepler", METHOD_ALG, MODE_GUESS), ("rkf", METHOD_RKF, MODE_PLAIN)):
+        d = torch.empty_like(s)
+        c = torch.zeros(STATS_LEN, dtype=torch.int64, device="cuda")
+        ctx.step_batched_(s, d, sizes, dt=5e-4, method=method, mode=mode, coupling=COUPLING_GRAVITY, coup=1.0,
+                          mass=mc, stats=c)
+        torch.cuda.synchronize()
+        out[name], stats[name] = d.cpu().numpy(), c.cpu().numpy()
+    npairs = sum(n * (n - 1) // 2 for n in sizes)
+    assert stats["kepler"][0:4].sum() == npairs and stats["rkf"][12] == 0
+    off = (np.abs(out["kepler"] - out["rkf"]) / np.abs(out["rkf"]).max(axis=1, keepdims=True)).max(axis=0)
+    print(f"{len(sizes)} systems, {sum(sizes)} particles, {npairs:.3e} pairs: Kepler drift vs RKF45 particles off by "
+          f"more than 1e-6 / 1e-8 of the largest delta {(off > 1e-6).sum()} / {(off > 1e-8).sum()}")
+    assert (off > 1e-6).sum() == 0
+    d = torch.empty_like(s)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    p0 = [(m_all[a:b] * st[3:, a:b]).sum(axis=1) for a, b in zip(offs[:-1], offs[1:])]
+    c = torch.zeros(STATS_LEN, dtype=torch.int64, device="cuda")
+    for _ in range(3):
+        ctx.step_batched_(s, d, sizes, dt=5e-4, mode=MODE_GUESS, coupling=COUPLING_GRAVITY, coup=1.0, mass=mc,
+                          stats=c)
+        s.add_(d)
+    torch.cuda.synchronize()
+    sn = s.cpu().numpy()
+    worst = 0.0
+    for k, (a, b) in enumerate(zip(offs[:-1], offs[1:])):
+        p1 = (m_all[a:b] * sn[3:, a:b]).sum(axis=1)
+        scale = (m_all[a:b] * np.abs(st[3:, a:b])).sum()
+        worst = max(worst, np.abs(p1 - p0[k]).max() / scale)
+    c = c.cpu().numpy()
+    print(f"3 steps: momentum change at most {worst:.1e} of sum m|v| per system; flags {c[8:14]}; "
+          f"{c[4:8].sum() / c[0:4].sum():.7f} evaluations per pair")
+    assert worst < 1e-14 and c[10] == 0 and c[13] == 0
