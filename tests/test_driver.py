@@ -54,3 +54,15 @@ def test_checkpoint_writer_writes_one_record_per_system(tmp_path):
     w.submit(str(tmp_path / "missing_dir" / "ck.bbb"), torch.from_numpy(st), sizes, Done())
     with pytest.raises(OSError):
         w.join()
+
+
+def test_report_without_statistics():
+    inv = np.zeros((2, 1, 11))
+    inv[:, 0, 0] = [1.0, 1.0]
+    inv[:, 0, 1] = [-2.0, -2.0]
+    r = driver.Report(sizes=[3], steps=[0, 5], invariants=inv, stats=None)
+    assert np.all(r.energy_error() == 0)
+    with pytest.raises(ValueError):
+        r.histogram()
+    with pytest.raises(ValueError):
+        r.interval_stats()
