@@ -57,6 +57,8 @@ def test_reference_dim_f90_on_library(tmp_path):
     (_, ref), = io.read_timed(os.path.join(GOLDEN, "0001332_10^0.DIM"))
     dp, dv = np.abs(got[:3] - ref[:3]).max(), np.abs(got[3:] - ref[3:]).max()
     print(f"reference dim_3D.f90 + libbia_hip vs OUT/0001332_10^0.DIM: pos {dp:.3e} vel {dv:.3e}")
+    print("timing of the unmodified dim_3D.f90 on libbia_hip (its own cpu_time, particle-1 trace on):",
+          [ln.strip() for ln in p.stdout.splitlines() if ln.strip().startswith("CPU =")])
     assert t == 1.0 and dp < 1e-13 and dv < 1e-13
     mine = np.loadtxt(trace, skiprows=1)
     theirs = np.loadtxt(os.path.join(GOLDEN, "0001332.DIM"), skiprows=1)
