@@ -3,8 +3,10 @@
 ! particle-1 text line, compare with the .DIM result when present) with the per-particle loop
 ! replaced by one device-resident whole step.
 !
-! usage: bia_run <prefix> <eps> <dt> [method 1=ALG|0=RKF] [mode 0=plain|1=learned]
-!   reads <prefix>.bbb (and <prefix>.DIM if it exists), writes <prefix>.run.BIA
+! usage: bia_run <prefix> <eps> <dt> [method 1=ALG|0=RKF] [mode 0=plain|2=Kepler drift]
+!   reads <prefix>.bbb (and <prefix>.DIM if it exists), writes <prefix>.run.BIA; mode 1 (learned
+!   start) needs a network trained and tabulated on the context first (bia_mlp_*), which this
+!   driver does not do, so the step call reports BIA_ERR_STATE for it
 program bia_run
   use iso_c_binding
   use bia_hip

@@ -66,3 +66,30 @@ def test_reference_dim_f90_on_library(tmp_path):
     # accepted step times: the controller works on error ratios at the rounding floor
     # (ABSERR 1e-16), so step sizes agree to ~1e-9 relative, not to the last bit
     assert np.abs(mine[:, 0] - theirs[:, 0]).max() < 1e-7
+
+
+FORTRAN_DRIVER = os.path.join(ROOT, "bia_3d_gpu_amd", "fortran", "bia_run")
+
+
+@pytest.mark.skipif(not os.path.exists(FORTRAN_DRIVER), reason="Fortran driver not built")
+@pytest.mark.parametrize("mode", [0, 2])
+def test_fortran_driver_over_the_bind_c_module(ctx, tmp_path, mode):
+    """bia_3d_gpu_amd/fortran/bia_run (the bind(C) module's host driver: read .bbb, invariants,
+    one whole-step call, write .BIA) on the reference's N=1332 fixture: the reference solvers'
+    deltas within the fixture tolerances of the reference's own .BIA, and both modes bitwise
+    equal to the same step through the Python binding of the same entry point."""
+    from bia_3d_gpu_amd import io
+    prefix = tmp_path / "fx"
+    shutil.copy(os.path.join(GOLDEN, "0001332_10^0.bbb"), str(prefix) + ".bbb")
+    shutil.copy(os.path.join(GOLDEN, "0001332_10^0.DIM"), str(prefix) + ".DIM")
+    p = subprocess.run([FORTRAN_DRIVER, str(prefix), "-6e-7", "1.0", "1", str(mode)], capture_output=True,
+                       text=True, timeout=300)
+    print(p.stdout[-2000:])
+    assert p.returncode == 0, p.stderr[-2000:]
+    (dt, got), = io.read_timed(str(prefix) + ".run.BIA")
+    st = io.read_bbb(str(prefix) + ".bbb")[0]
+    assert dt == 1.0
+    assert np.array_equal(got, ctx.step_host(st, -6e-7, 1.0, mode=mode))
+    if mode == 0:
+        (_, ref), = io.read_timed(os.path.join(GOLDEN, "0001332_10^0.BIA"))
+        assert np.abs(got[:3] - ref[:3]).max() < 3e-11 and np.abs(got[3:] - ref[3:]).max() < 1e-14
