@@ -72,12 +72,11 @@ struct PairStats {
 
 enum Coupling : int { kScalar = 0, kGravity = 1, kCoulomb = 2 };
 
-// lane l receives x from lane l + 1 (lane 63 from lane 0): DPP wave_rol:1 on both halves
-__device__ __forceinline__ double rot_down(double x) {
-  const long long v = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(v & 0xffffffffLL), 0x134, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), 0x134, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+// LDS add without return (ds_add_f64): the add runs in the LDS unit, not on the VALU.  The LDS
+// operations of one wave complete in issue order, so a column sum built from one add per step
+// is the sequential sum in step order, rounded like v_add_f64.
+__device__ __forceinline__ void lds_add(double* p, double x) {
+  __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // OCC: minimum waves per SIMD asked of the register allocator (1 = its own choice; 4 = at most 128
@@ -86,10 +85,10 @@ __device__ __forceinline__ double rot_down(double x) {
 // One wave per tile.  Lane l holds row i = 64 I + l.  Plain mode walks the columns in order
 // (below).  The Kepler modes stage the column tile's states in LDS and at step k lane l takes
 // column j = (l + k) mod 64, so the 64 lanes work on 64 different
-// columns and no column needs a cross-lane reduction: the column sums travel with their column,
-// one lane down per step (rot_down), and after 64 steps lane l holds column l's sum.  A diagonal
-// tile (I = J) takes steps k = 1..32 (at k = 32 lanes 0..31 only), which covers each unordered
-// pair once; its column sums end in lane (l - 33) mod 64 and are fetched once per tile.
+// columns and no column needs a cross-lane reduction: each lane adds its pair's column share
+// into the column's LDS sum (lds_add; one address per lane per step, the steps in order).  A
+// diagonal tile (I = J) takes steps k = 1..32 (at k = 32 lanes 0..31 only), which covers each
+// unordered pair once.
 template <int METHOD, int MODE, int COUP, bool STATS, int OCC>
 __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const int4* __restrict__ tiles,
                                                  const SysDesc* __restrict__ sys, const double* __restrict__ st,
@@ -99,6 +98,7 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
   __shared__ double hist_lds[kMaxItr * kWave];
   __shared__ double cst[6][kWave];  // column tile state
   __shared__ double cmq[2][kWave];  // column masses and charges (couplings only)
+  __shared__ double csum[6][kWave];  // column sums of the tile's debits (Kepler modes)
   const int lane = threadIdx.x;
   Hist hist{hist_lds + lane, kWave};
   PairStats ps;
@@ -175,14 +175,17 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
       const bool ok = lane < jn;
       const int gj = S.off + j0 + lane;
 #pragma unroll
-      for (int c = 0; c < 6; c++) cst[c][lane] = ok ? st[(size_t)c * lds + gj] : 0.0;
+      for (int c = 0; c < 6; c++) {
+        cst[c][lane] = ok ? st[(size_t)c * lds + gj] : 0.0;
+        csum[c][lane] = 0.0;
+      }
       if (COUP != kScalar) {
         cmq[0][lane] = (ok && mass) ? mass[gj] : 1.0;
         cmq[1][lane] = (ok && COUP == kCoulomb) ? charge[gj] : 0.0;
       }
     }
     __syncthreads();
-    double acc[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
+    double acc[6] = {0, 0, 0, 0, 0, 0};
     const bool diag = (I == J);
     const int k0 = diag ? 1 : 0, k1 = diag ? 33 : kWave;
 #pragma unroll 1
@@ -211,28 +214,23 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
         for (int c = 0; c < 6; c++) {
           if (COUP == kScalar) {
             acc[c] += d[c];
-            col[c] -= d[c];
+            lds_add(&csum[c][jl], d[c]);
           } else {
             acc[c] += wi * d[c];
-            col[c] -= wj * d[c];
+            lds_add(&csum[c][jl], wj * d[c]);
           }
         }
       }
-#pragma unroll
-      for (int c = 0; c < 6; c++) col[c] = rot_down(col[c]);
     }
+    __syncthreads();  // the column sums are complete
     if (diag) {
-      // lane l holds the column sum of particle (l + 33) mod 64; particle l's is in lane l - 33
 #pragma unroll
-      for (int c = 0; c < 6; c++) {
-        const double cl = __shfl(col[c], (lane + 31) & (kWave - 1), kWave);
-        P[((size_t)I * 6 + c) * ldp + il] = acc[c] + cl;
-      }
+      for (int c = 0; c < 6; c++) P[((size_t)I * 6 + c) * ldp + il] = acc[c] - csum[c][lane];
     } else {
 #pragma unroll
       for (int c = 0; c < 6; c++) {
         P[((size_t)J * 6 + c) * ldp + il] = acc[c];
-        P[((size_t)I * 6 + c) * ldp + j0 + lane] = col[c];
+        P[((size_t)I * 6 + c) * ldp + j0 + lane] = -csum[c][lane];
       }
     }
   }
