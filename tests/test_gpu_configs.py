@@ -1,6 +1,7 @@
-"""Two of the benchmark configurations at their full size, checked through properties that do not
-need the oracle at that size: agreement of independent solvers pair by pair, conservation of
-momentum, determinism of the multi-step driver, and the invariants it monitors."""
+"""The benchmark configurations at their full size, and one system four times the benchmark's N,
+checked through properties that do not need the oracle at that size: agreement of independent
+solvers pair by pair, conservation of momentum, invariance under reordering, determinism of the
+multi-step driver, the invariants it monitors, and the oracle on a few particles."""
 import numpy as np
 import pytest
 
@@ -199,3 +200,53 @@ def test_mass_spectrum_ensemble_64_systems(ctx):
     print(f"3 steps: momentum change at most {worst:.1e} of sum m|v| per system; flags {c[8:14]}; "
           f"{c[4:8].sum() / c[0:4].sum():.7f} evaluations per pair")
     assert worst < 1e-14 and c[10] == 0 and c[13] == 0
+
+
+def test_system_262144_with_64_bit_partial_offsets():
+    """The largest single system tested: Plummer N = 262144 (4096 tiles a side; the tile partials
+    hold 6.4e9 doubles, 51.5 GB, so their offsets pass 2^32), eps = -2/N, dt = 1e-3, one
+    Kepler-drift step (3.4e10 pairs).  Checked by (1) the same system in a shuffled order: every
+    particle's delta agrees with the unshuffled one's up to summation order; (2) six particles,
+    the first and last ones among them, against the oracle's reference-algorithm row for that
+    particle (the particle moved to the front, so that its row covers all its pairs): velocities
+    to 1e-9 of the largest velocity delta, positions normwise to 1e-11 — the reference
+    algorithm's floor: it advances the whole relative state and subtracts the straight-line
+    motion, leaving up to ~1e-14 of rounding in each weak pair's position departure, ~1e-12 summed
+    over a particle's 262143 pairs; (3) the total velocity change is rounding."""
+    import torch
+    import oracle as O
+    from bia_3d_gpu_amd import MODE_GUESS, Context, ic
+    n, dt = 262144, 1e-3
+    eps = -2.0 / n
+    st = ic.plummer_system(n, seed=n)
+    perm = np.random.default_rng(7).permutation(n)
+    with Context(0) as c:
+        c.set_stream(torch.cuda.current_stream().cuda_stream)
+        out = []
+        for order in (None, perm):
+            s = torch.from_numpy(np.ascontiguousarray(st if order is None else st[:, order])).cuda()
+            d = torch.empty_like(s)
+            c.step_(s, d, eps, dt, mode=MODE_GUESS)
+            torch.cuda.synchronize()
+            out.append(d.cpu().numpy())
+            del s, d
+    d, dp = out
+    pair_part = d.copy()
+    pair_part[:3] -= st[3:] * dt  # the position deltas without the drift v dt
+    scale = np.abs(pair_part).max(axis=1, keepdims=True)
+    shuffled = dp.copy()
+    shuffled[:3] -= st[3:, perm] * dt
+    err_shuffle = (np.abs(shuffled - pair_part[:, perm]) / scale).max()
+    assert err_shuffle < 1e-10, err_shuffle
+    mom = np.abs(d[3:].sum(axis=1)).max() / scale[3:].max()
+    assert mom < 1e-9, mom
+    worst_v, worst_x = 0.0, 0.0
+    xmax = np.abs(st[:3]).max()
+    for p in (0, 1, n // 2, n - 65, n - 2, n - 1):
+        order = np.concatenate([[p], np.delete(np.arange(n), p)])
+        ref = O.bia_step(st[:, order], eps, dt, method=1, rows=(0, 1))[:, 0]
+        worst_v = max(worst_v, float((np.abs(d[3:, p] - ref[3:]) / scale[3:, 0]).max()))
+        worst_x = max(worst_x, float(np.abs(d[:3, p] - ref[:3]).max() / xmax))
+    print(f"N={n}: shuffled order {err_shuffle:.2e}, momentum {mom:.2e}; six particles against the oracle: "
+          f"velocities {worst_v:.2e} of the largest velocity delta, positions {worst_x:.2e} normwise")
+    assert worst_v < 1e-9 and worst_x < 1e-11, (worst_v, worst_x)
