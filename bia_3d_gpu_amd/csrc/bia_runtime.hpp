@@ -105,8 +105,9 @@ struct bia_ctx {
   hipStream_t stream = nullptr;  // default: the null stream (torch's default stream too)
   hipStream_t cap_stream = nullptr;  // private blocking stream for graph capture when stream is null
   std::mutex mu;                 // one call at a time per context
-  // step workspace
-  bia::DevBuf part, state, delta, stats;
+  // step workspace; stat_slots: per-workgroup statistics slots of the pair kernel, zero between
+  // launches (the fold kernel clears what it reads)
+  bia::DevBuf part, state, delta, stat_slots;
   // batched plans: device tile list (int4 I, J, sys, 0), system descriptors, reduce chunks
   bia::Plan plan;
   bia::DevBuf plan_tiles, plan_sys, plan_chunks;

@@ -30,6 +30,7 @@
 namespace bia {
 
 constexpr int kWave = 64;
+constexpr int kStatSlots = 4096;  // statistics slots of the pair kernel (a power of two)
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -46,9 +47,14 @@ struct PairStats {
   unsigned cnt[4];
   unsigned its[4];
   unsigned fl[8];
+  // Kepler modes: this tile's pair counts and evaluations per branch in 16-bit fields of tc and
+  // ti (a lane takes at most 64 pairs of a tile and at most 60 evaluations per pair, so a field
+  // holds at most 3840), folded into cnt/its once per tile
+  unsigned long long tc, ti;
   __device__ void clear() {
     for (int k = 0; k < 4; k++) cnt[k] = its[k] = 0;
     for (int k = 0; k < 8; k++) fl[k] = 0;
+    tc = ti = 0;
   }
   __device__ void add(int br, int it, unsigned f) {
     for (int k = 0; k < 4; k++) {
@@ -57,18 +63,61 @@ struct PairStats {
     }
     for (int k = 0; k < 8; k++) fl[k] += (f >> k) & 1u;
   }
-  __device__ void flush(unsigned long long* out) {
+  // br in [0, 4), it <= 60; flags are rare on these paths and go straight to their totals
+  __device__ void add_tile(int br, int it, unsigned f) {
+    const unsigned sh = 16u * (unsigned)br;
+    tc += 1ull << sh;
+    ti += (unsigned long long)(unsigned)it << sh;
+    if (f)
+      for (int k = 0; k < 8; k++) fl[k] += (f >> k) & 1u;
+  }
+  __device__ void fold_tile() {
+    for (int k = 0; k < 4; k++) {
+      cnt[k] += (unsigned)(tc >> (16 * k)) & 0xffffu;
+      its[k] += (unsigned)(ti >> (16 * k)) & 0xffffu;
+    }
+    tc = ti = 0;
+  }
+  // the workgroup's totals into its slot (workgroup b uses slot b mod kStatSlots, so a few
+  // hundred workgroups at most share the 16 counters of one slot; k_stats_fold adds the slots up)
+  __device__ void flush(unsigned long long* slots) {
     unsigned v[16];
     for (int k = 0; k < 4; k++) {
       v[k] = wave_sum_u(cnt[k]);
       v[4 + k] = wave_sum_u(its[k]);
     }
     for (int k = 0; k < 8; k++) v[8 + k] = wave_sum_u(fl[k]);
+    unsigned long long* out = slots + (size_t)(blockIdx.x & (kStatSlots - 1)) * BIA_STATS_LEN;
     if (threadIdx.x == 0)
       for (int k = 0; k < 16; k++)
         if (v[k]) atomicAdd(out + k, (unsigned long long)v[k]);
   }
 };
+
+// stats[k] += sum over the slots of counter k, and the slots back to zero; one block per counter
+__global__ __launch_bounds__(256) void k_stats_fold(unsigned long long* __restrict__ slots,
+                                                    unsigned long long* __restrict__ stats) {
+  __shared__ unsigned long long red[256];
+  const int k = blockIdx.x;
+  unsigned long long s = 0;
+  for (int i = threadIdx.x; i < kStatSlots; i += 256) {
+    unsigned long long* p = slots + (size_t)i * BIA_STATS_LEN + k;
+    s += *p;
+    *p = 0;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) stats[k] += red[0];
+}
+
+__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
 
 enum Coupling : int { kScalar = 0, kGravity = 1, kCoulomb = 2 };
 
@@ -209,7 +258,7 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
         int br = 0, it = 0;
         unsigned fl = 0;
         pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T);
-        if (STATS) ps.add(br, it, fl);
+        if (STATS) ps.add_tile(br, it, fl);
 #pragma unroll
         for (int c = 0; c < 6; c++) {
           if (COUP == kScalar) {
@@ -222,6 +271,7 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
         }
       }
     }
+    if (STATS) ps.fold_tile();
     __syncthreads();  // the column sums are complete
     if (diag) {
 #pragma unroll
@@ -374,6 +424,13 @@ static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
   if (dirty)
     BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_sys.p, P.hsys.data(), P.hsys.size() * sizeof(SysDesc), hipMemcpyHostToDevice));
   BIA_TRY(ctx, ctx->part.ensure(P.part_doubles * sizeof(double)));
+  if (a.d_stats && !ctx->stat_slots.p) {
+    const int n = kStatSlots * BIA_STATS_LEN;
+    BIA_TRY(ctx, ctx->stat_slots.ensure((size_t)n * sizeof(unsigned long long)));
+    hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->stat_slots.as<unsigned long long>(), n);
+    BIA_HIP_TRY(ctx, hipGetLastError());
+  }
   return BIA_OK;
 }
 
@@ -410,10 +467,12 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   const SysDesc* sys = ctx->plan_sys.as<SysDesc>();
   double* part = ctx->part.as<double>();
   const bool st = a.d_stats != nullptr;
+  unsigned long long* slots = ctx->stat_slots.as<unsigned long long>();
+  if (st && !slots) return BIA_ERR_STATE;  // ensure_plan allocates the slots for a step with statistics
   BIA_TRY(ctx, prof_begin(ctx));
 #define L_OCC(M, MO, CP, S, O)                                                                                     \
   hipLaunchKernelGGL((k_tiles<M, MO, CP, S, O>), dim3(grid), dim3(kWave), 0, ctx->stream, P.ntiles, a.ld, tiles, sys, \
-                     a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, a.d_stats, T)
+                     a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, slots, T)
 #define L_(M, MO, CP, S)                                    \
   do {                                                      \
     if (M == 1 && occ == 4) L_OCC(M, MO, CP, S, 4);         \
@@ -444,6 +503,10 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
 #undef L_OCC
   BIA_HIP_TRY(ctx, hipGetLastError());
   BIA_TRY(ctx, prof_end(ctx));
+  if (st) {
+    hipLaunchKernelGGL(k_stats_fold, dim3(BIA_STATS_LEN), dim3(256), 0, ctx->stream, slots, a.d_stats);
+    BIA_HIP_TRY(ctx, hipGetLastError());
+  }
   hipLaunchKernelGGL(k_reduce, dim3(P.nchunks, 6), dim3(256), 0, ctx->stream, ctx->plan_chunks.as<int2>(), sys, a.ld,
                      a.ld, a.dt, a.d_state, part, a.d_delta);
   BIA_HIP_TRY(ctx, hipGetLastError());
@@ -543,7 +606,8 @@ static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, 
                                 (long long)(size_t)a.d_delta, (long long)(size_t)a.d_mass,
                                 (long long)(size_t)a.d_charge, a.ld, (long long)(size_t)ctx->stream, tuned.first, tuned.second,
                                 ctx->mlp_ready, (long long)(size_t)ctx->part.p,
-                                (long long)(size_t)ctx->mlp_table.p, (long long)(size_t)a.d_stats};
+                                (long long)(size_t)ctx->mlp_table.p, (long long)(size_t)a.d_stats,
+                                (long long)(size_t)ctx->stat_slots.p};
   long long dtb, cb;
   memcpy(&dtb, &a.dt, 8);
   memcpy(&cb, &a.coup, 8);

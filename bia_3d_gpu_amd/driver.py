@@ -150,7 +150,7 @@ def run(ctx: Context, state, sizes, nsteps: int, *, eps, dt: float, method: int 
     ``checkpoint_path.format(step=k)`` as a ``.bbb`` file with one record per system.
     ``log(line)``, if given, receives one line per monitoring step (this copies the monitored
     values to the host at that step).  ``collect_stats=False`` runs the steps without the per-pair
-    statistics (their kernel variant costs about a quarter more time per step).
+    statistics (the pair kernel's statistics variant costs 3-10 % more time per step).
 
     The invariants are those of the reference's equal-mass scalar coupling (its
     ``Invariance_GPU``); with the gravity or Coulomb couplings they are left NaN and only the
