@@ -114,7 +114,7 @@ def flops_per_step(stats, mode: int = MODE_PLAIN) -> float:
     one square root), comparisons and min/max not counted: 109 per pair (relative state 6,
     kep_setup 23, features 7, series start 9, kep_output 52 with its Taylor carry of the G
     functions, accumulation 12) + 58 per evaluation of the time equation (kep_g 22, kep_halley
-    32, step and convergence test 4), + 120 per learned start (three asinh, trilinear lookup,
+    32, step and convergence test 4), + 120 per learned start (three log2 features, trilinear lookup,
     exp, scale).  The bound-orbit period reduction, taken by few pairs, is not counted.
     """
     s = np.asarray(stats, dtype=np.float64)
