@@ -121,6 +121,10 @@ __global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* __restrict
 
 enum Coupling : int { kScalar = 0, kGravity = 1, kCoulomb = 2 };
 
+// Tile partials are written once and read once, by k_reduce after the launch: streaming stores,
+// so that 3 GB of them do not evict the particle states the tiles re-read from L2.
+__device__ __forceinline__ void put_partial(double* p, double v) { __builtin_nontemporal_store(v, p); }
+
 // LDS add without return (ds_add_f64): the add runs in the LDS unit, not on the VALU.  The LDS
 // operations of one wave complete in issue order, so a column sum built from one add per step
 // is the sequential sum in step order, rounded like v_add_f64.
@@ -209,12 +213,12 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
       }
       if (I == J) {
 #pragma unroll
-        for (int c = 0; c < 6; c++) P[((size_t)I * 6 + c) * ldp + il] = acc[c] + col[c];
+        for (int c = 0; c < 6; c++) put_partial(P + ((size_t)I * 6 + c) * ldp + il, acc[c] + col[c]);
       } else {
 #pragma unroll
         for (int c = 0; c < 6; c++) {
-          P[((size_t)J * 6 + c) * ldp + il] = acc[c];
-          P[((size_t)I * 6 + c) * ldp + j0 + lane] = col[c];
+          put_partial(P + ((size_t)J * 6 + c) * ldp + il, acc[c]);
+          put_partial(P + ((size_t)I * 6 + c) * ldp + j0 + lane, col[c]);
         }
       }
       continue;
@@ -275,12 +279,12 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
     __syncthreads();  // the column sums are complete
     if (diag) {
 #pragma unroll
-      for (int c = 0; c < 6; c++) P[((size_t)I * 6 + c) * ldp + il] = acc[c] - csum[c][lane];
+      for (int c = 0; c < 6; c++) put_partial(P + ((size_t)I * 6 + c) * ldp + il, acc[c] - csum[c][lane]);
     } else {
 #pragma unroll
       for (int c = 0; c < 6; c++) {
-        P[((size_t)J * 6 + c) * ldp + il] = acc[c];
-        P[((size_t)I * 6 + c) * ldp + j0 + lane] = -csum[c][lane];
+        put_partial(P + ((size_t)J * 6 + c) * ldp + il, acc[c]);
+        put_partial(P + ((size_t)I * 6 + c) * ldp + j0 + lane, -csum[c][lane]);
       }
     }
   }
