@@ -14,7 +14,10 @@
  *     and run on the context's stream.  Device-pointer variants (_dev) take state already
  *     resident in HBM in structure-of-arrays layout: six rows x[ld], y[ld], z[ld], u[ld],
  *     v[ld], w[ld] with row stride ld >= n.  Host variants (_host) copy in and out.
- *     A context serialises its own calls; use one context per host thread.
+ *     A context serialises its own calls; use one context per host thread.  Its device
+ *     workspace (tile partials, statistics slots, captured graphs) is shared by its calls, so
+ *     work a call enqueued on one stream must finish before a call on another stream of the
+ *     same context runs; calls on one stream are ordered by the stream.
  *
  * Units and conventions follow the reference: eps is the two-body coupling of the relative
  * equation of motion r'' = eps r/|r|^3 (eps = -2Gm for equal-mass gravity, bia/bia.f90:37),
