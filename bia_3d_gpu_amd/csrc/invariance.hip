@@ -14,6 +14,7 @@
 #include <climits>
 #include <cstring>
 
+#include "bia_pair.hpp"
 #include "bia_runtime.hpp"
 
 namespace bia {
@@ -93,7 +94,10 @@ __global__ __launch_bounds__(64) void k_inv_pe_tiles(int nsys, const InvSys* __r
     const double xj = st[j], yj = st[ld + j], zj = st[2 * (size_t)ld + j];
     if (ok && (I != J || lane < jj)) {
       double dx = xi - xj, dy = yi - yj, dz = zi - zj;
-      dd_add(s, 1.0 / sqrt(dx * dx + dy * dy + dz * dz));
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      // 1/r within an ulp from the hardware estimate and one correction (the reference takes
+      // rsqrt, bia/invariance.cu:23); coincident particles give +inf as 1/sqrt(0) does
+      dd_add(s, r2 > 0 ? fast_rsqrt(r2) : __builtin_huge_val());
     }
   }
   s = dd_wave_sum(s);
