@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
                                                  unsigned long long* __restrict__ stats, LearnTable T) {
   __shared__ double hist_lds[kMaxItr * kWave];
   __shared__ double cst[6][kWave];  // column tile state
-  __shared__ double cmq[2][kWave];  // column masses and charges (couplings only)
+  __shared__ double cmq[3][kWave];  // column masses, charges and reciprocal masses (couplings only)
   __shared__ double csum[6][kWave];  // column sums of the tile's debits (Kepler modes)
   const int lane = threadIdx.x;
   Hist hist{hist_lds + lane, kWave};
@@ -166,10 +166,13 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
     double ri[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) ri[c] = row_ok ? st[(size_t)c * lds + gi] : 0.0;
-    double mi = 1, qi = 0;
+    double mi = 1, qi = 0, rmi = 1;
     if (COUP != kScalar) {
       mi = (row_ok && mass) ? mass[gi] : 1.0;
-      if (COUP == kCoulomb) qi = row_ok ? charge[gi] : 0.0;
+      if (COUP == kCoulomb) {
+        qi = row_ok ? charge[gi] : 0.0;
+        rmi = 1 / mi;
+      }
     }
     const int j0 = J * kWave;
     const int jn = min(kWave, S.n - j0);
@@ -233,8 +236,10 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
         csum[c][lane] = 0.0;
       }
       if (COUP != kScalar) {
-        cmq[0][lane] = (ok && mass) ? mass[gj] : 1.0;
+        const double mj = (ok && mass) ? mass[gj] : 1.0;
+        cmq[0][lane] = mj;
         cmq[1][lane] = (ok && COUP == kCoulomb) ? charge[gj] : 0.0;
+        if (COUP == kCoulomb) cmq[2][lane] = 1 / mj;
       }
     }
     __syncthreads();
@@ -252,9 +257,12 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
         if (COUP != kScalar) {
           const double mj = cmq[0][jl];
           if (COUP == kGravity) eps = -coup * (mi + mj);
-          else eps = coup * qi * cmq[1][jl] * (1 / mi + 1 / mj);
-          wi = 2 * mj / (mi + mj);
-          wj = 2 * mi / (mi + mj);
+          else eps = coup * qi * cmq[1][jl] * (rmi + cmq[2][jl]);
+          // the centre-of-mass split 2 m_j/(m_i+m_j), 2 m_i/(m_i+m_j) from one reciprocal (within
+          // an ulp); equal masses split exactly in half, as the scalar coupling does
+          const double rs = fast_rcp(mi + mj);
+          wi = (mi == mj) ? 1.0 : 2 * mj * rs;
+          wj = (mi == mj) ? 1.0 : 2 * mi * rs;
         }
         double rel[6];
 #pragma unroll
