@@ -78,8 +78,8 @@ struct PairStats {
     }
     tc = ti = 0;
   }
-  // the workgroup's totals into its slot (workgroup b uses slot b mod kStatSlots, so a few
-  // hundred workgroups at most share the 16 counters of one slot; k_stats_fold adds the slots up)
+  // the workgroup's totals into its slot: workgroup b uses slot b mod kStatSlots, which spreads
+  // a launch's atomics over kStatSlots x 16 addresses instead of 16; k_stats_fold adds the slots up
   __device__ void flush(unsigned long long* slots) {
     unsigned v[16];
     for (int k = 0; k < 4; k++) {
