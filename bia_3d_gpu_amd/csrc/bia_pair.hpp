@@ -456,6 +456,7 @@ BIA_HDF double kep_halley(const KepPair& K, const KepG& g, double* F, double* es
 BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool* ok, KepG* last) {
   constexpr double kInf = __builtin_huge_val();
   double lo, hi;
+  float dx, dxold = __builtin_huge_valf();  // the last two step lengths (only compared)
   *ok = false;
   {
     const KepG g = kep_g(K.beta, s);
@@ -473,8 +474,10 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
     hi = K.tau > 0 ? kInf : 0.0;
     if (F < 0) lo = s;
     else hi = s;
-    s = (valid && sn > lo && sn < hi) ? sn : (hi == kInf) ? 2 * s + K.ir0 * fabs(K.tau)
-                                         : (lo == -kInf) ? 2 * s - K.ir0 * fabs(K.tau) : 0.5 * (lo + hi);
+    const bool take = valid && sn > lo && sn < hi;
+    dx = take ? (float)fabs(d) : __builtin_huge_valf();
+    s = take ? sn : (hi == kInf) ? 2 * s + K.ir0 * fabs(K.tau)
+                  : (lo == -kInf) ? 2 * s - K.ir0 * fabs(K.tau) : 0.5 * (lo + hi);
   }
   int k = 1;
   while (k < kmax) {
@@ -500,9 +503,17 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
       *ok = true;
       break;
     }
-    if (valid && sn > lo && sn < hi) {
+    // a Halley step inside the bracket; once the bracket is closed, only while the steps at
+    // least halve every second iteration, else bisection (a step that does not shrink, as from
+    // far up the exponential branch of a hyperbola, would creep towards the root)
+    const bool closed = lo > -kInf && hi < kInf;
+    if (valid && sn > lo && sn < hi && (!closed || (float)fabs(d) <= 0.5f * dxold)) {
+      dxold = dx;
+      dx = (float)fabs(d);
       s = sn;
-    } else if (lo > -kInf && hi < kInf) {
+    } else if (closed) {
+      dxold = dx;
+      dx = (float)(0.5 * (hi - lo));
       s = 0.5 * (lo + hi);  // bisection
     } else {  // no upper (lower) end yet: move out
       s = (hi == kInf) ? 2 * fmax(s, lo) + K.ir0 * fabs(K.tau) : 2 * fmin(s, hi) - K.ir0 * fabs(K.tau);
