@@ -453,6 +453,11 @@ BIA_HDF double kep_halley(const KepPair& K, const KepG& g, double* F, double* es
 //
 // The start s must lie on tau's side of 0 (s tau > 0, or s = 0 when tau = 0): the series start
 // does, and the learned start is accepted only on the same side as the series start.
+// whether s lies above the root, given f = t(s) - tau.  A non-finite f means the G functions
+// overflowed, which happens only far beyond the root on tau's side (|G_k| grows with |s|, and at
+// the root they are finite): the sign of inf - inf or of the overflowed sum says nothing.
+BIA_HDF bool above(const KepPair& K, double f) { return isfinite(f) ? f >= 0 : K.tau > 0; }
+
 BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool* ok, KepG* last) {
   constexpr double kInf = __builtin_huge_val();
   double lo, hi;
@@ -472,8 +477,8 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
     }
     lo = K.tau > 0 ? 0.0 : -kInf;
     hi = K.tau > 0 ? kInf : 0.0;
-    if (F < 0) lo = s;
-    else hi = s;
+    if (above(K, F)) hi = s;
+    else lo = s;
     const bool take = valid && sn > lo && sn < hi;
     dx = take ? (float)fabs(d) : __builtin_huge_valf();
     s = take ? sn : (hi == kInf) ? 2 * s + K.ir0 * fabs(K.tau)
@@ -491,8 +496,8 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
       *ok = true;
       break;
     }
-    if (F < 0) lo = s;
-    else hi = s;
+    if (above(K, F)) hi = s;
+    else lo = s;
     if (hi - lo <= 4.5e-16 * fabs(s)) {  // bracket closed to rounding: s is the root
       *ok = true;
       break;
@@ -608,7 +613,9 @@ BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6]
   kep_features(K, a);
   double s = kep_series_start(K, a);
   if (MODE == kLearn && fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))) >= kWeak) {
-    const double sl = s * (double)exp2f(learned_log2_ratio(T, a));
+    // the network is trained on targets |log(s / s_start)| <= 4 (a ratio within e^4); the
+    // proposal is held to a ratio of 2^6 so that no table can start the solver far away
+    const double sl = s * (double)exp2f(fminf(fmaxf(learned_log2_ratio(T, a), -6.0f), 6.0f));
     if (isfinite(sl) && sl * s > 0) {
       s = sl;
       *fl |= kLearned;

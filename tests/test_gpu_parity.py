@@ -276,3 +276,57 @@ def test_kepler_drift_on_device_against_40_digit_solution(ctx):
             assert (ex_pos <= 1e-14 * scale).all(), (e, t, k, ex_pos.max() / scale)
     print(f"device Kepler drift vs 40 digits: velocity deltas {worst_v:.1e}, position deltas beyond the "
           f"rounding of v dt {worst_x:.1e} of the pair's largest delta")
+
+
+def test_kepler_drift_on_device_strong_pairs(ctx):
+    """The safeguarded iteration on the device: random strong pairs (max|a_k| up to 1e3, every
+    geometry; two-particle systems, one launch per group) finish without the iteration cap or a
+    non-finite output, and three near-radial incoming pairs that pass pericentre within dt (they
+    once stopped at the 60-evaluation cap) agree with the closed-form solution in 80 digits."""
+    pytest.importorskip("mpmath")
+    import torch
+    from bia_3d_gpu_amd import MODE_GUESS, STATS_LEN
+    from test_hostsim import _drift_closed_form
+    hard = [([0.0026037201609559564, -0.0070822235211471804, 0.024403191399523627, 1.6446235016284003,
+              1.8344757887165375, -20.8830149885328], -3.9285356109812746, 0.0012847944556660402),
+            ([-0.2446222194714971, -0.05091575606808398, -0.48917818595901047, 9.262602111071313,
+              4.075932064638315, 17.254861329258418], -7.267704183781688e-08, 0.04293330885457181),
+            ([0.0001719005647052639, -0.00304250995202669, -0.0009257702154520646, -0.028437539851761458,
+              0.268365026562335, 0.09567271803022319], -7.267704183781688e-08, 0.04293330885457181)]
+
+    def run(rels, eps, dt):
+        m = len(rels)
+        st = np.zeros((6, 2 * m))
+        st[:, 0::2] = np.asarray(rels).T  # particle 2 at rest at the origin: rel = state of particle 1
+        s = torch.from_numpy(st).cuda()
+        d = torch.empty_like(s)
+        c = torch.zeros(STATS_LEN, dtype=torch.int64, device="cuda")
+        ctx.step_batched_(s, d, [2] * m, eps=eps, dt=dt, mode=MODE_GUESS, stats=c)
+        torch.cuda.synchronize()
+        return d.cpu().numpy()[:, 0::2], c.cpu().numpy()
+
+    worst = 0.0
+    for dt in sorted({h[2] for h in hard}):
+        group = [h for h in hard if h[2] == dt]
+        got, stats = run([h[0] for h in group], [h[1] for h in group], dt)
+        assert stats[10] == 0 and stats[13] == 0, stats
+        for k, (rel, eps, _) in enumerate(group):
+            ex = _drift_closed_form(np.array(rel), eps, dt)
+            err = np.abs(got[3:, k] - ex[3:]).max() / np.abs(ex).max()
+            worst = max(worst, err)
+            assert err < 1e-10, (k, err)
+    rng = np.random.default_rng(79)
+    for trial in range(12):
+        m = 4096
+        r = rng.standard_normal((m, 3)) * 10.0 ** rng.uniform(-3, 1, (m, 1))
+        v = rng.standard_normal((m, 3)) * 10.0 ** rng.uniform(-3, 1, (m, 1))
+        eps = -10.0 ** rng.uniform(-4, 0.5) * (1 if trial % 4 else -1)
+        dt = 10.0 ** rng.uniform(-3, 0.5)
+        r0 = np.linalg.norm(r, axis=1)
+        beta = -2 * eps / r0 - (v * v).sum(1)
+        a = np.abs([(r * v).sum(1) * dt / r0 ** 2, (-eps - beta * r0) * dt ** 2 / r0 ** 3, beta * dt ** 2 / r0 ** 2])
+        keep = a.max(axis=0) <= 1e3
+        rels = np.hstack([r, v])[keep]
+        got, stats = run(rels, [eps] * len(rels), dt)
+        assert np.isfinite(got).all() and stats[10] == 0 and stats[13] == 0, (trial, stats)
+    print(f"device: strong random pairs without the iteration cap; hard pairs vs closed form {worst:.1e}")
