@@ -25,7 +25,11 @@ def read_records(path: str) -> list[bytes]:
         data = f.read()
     p = 0
     while p < len(data):
+        if p + 4 > len(data):
+            raise ValueError(f"{path}: truncated record marker at byte {p}")
         (n,) = struct.unpack_from("<i", data, p)
+        if n < 0 or p + 8 + n > len(data):
+            raise ValueError(f"{path}: record of {n} bytes at byte {p} does not fit the file ({len(data)} bytes)")
         rec = data[p + 4:p + 4 + n]
         (n2,) = struct.unpack_from("<i", data, p + 4 + n)
         if n2 != n:

@@ -2,6 +2,7 @@
 import os
 
 import numpy as np
+import pytest
 
 from conftest import GOLDEN
 
@@ -81,3 +82,22 @@ def test_coulomb_bound_fraction():
     d = st[:, i] - st[:, j]
     E = (d[3:] ** 2).sum(0) + 2 * 2 * q[i] * q[j] / np.linalg.norm(d[:3], axis=0)
     assert abs((E < 0).mean() - 0.4) < 0.02
+
+
+def test_malformed_record_files(tmp_path):
+    """Truncated files, negative or oversized record markers and mismatched trailing markers
+    raise ValueError (a negative marker once made the reader loop for ever)."""
+    import struct
+    from bia_3d_gpu_amd import io
+    good = tmp_path / "good.bbb"
+    io.write_bbb(str(good), [np.arange(12, dtype=float).reshape(6, 2)])
+    data = good.read_bytes()
+    cases = {"truncated": data[:-3], "short marker": data + b"\x01\x00",
+             "negative": struct.pack("<i", -8) + data, "oversized": struct.pack("<i", 1 << 30) + data[4:],
+             "mismatch": data[:-4] + struct.pack("<i", 7)}
+    for name, blob in cases.items():
+        p = tmp_path / f"{name.replace(' ', '_')}.bbb"
+        p.write_bytes(blob)
+        with pytest.raises(ValueError):
+            io.read_records(str(p))
+    assert len(io.read_records(str(good))) == 1
