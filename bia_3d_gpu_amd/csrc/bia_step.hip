@@ -487,7 +487,7 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
                      a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, slots, T)
 #define L_(M, MO, CP, S)                                    \
   do {                                                      \
-    if (M == 1 && occ == 4) L_OCC(M, MO, CP, S, 4);         \
+    if (occ == 4) L_OCC(M, MO, CP, S, 4);                   \
     else if (M == 1 && MO != 0 && occ == 5) L_OCC(M, MO, CP, S, 5); \
     else L_OCC(M, MO, CP, S, 1);                            \
   } while (0)
@@ -552,8 +552,7 @@ static int autotune_locked(bia_ctx* ctx, const StepArgs& a, float* best_ms, int*
   float best = 1e30f;
   int bestg = ntiles, besto = 1;
   int status = BIA_OK;
-  std::vector<int> occs = {1};
-  if (a.method == BIA_METHOD_ALG) occs.push_back(4);
+  std::vector<int> occs = {1, 4};
   if (a.method == BIA_METHOD_ALG && a.mode != BIA_MODE_PLAIN) occs.push_back(5);
   for (int o : occs)
     for (int g : cands) {
