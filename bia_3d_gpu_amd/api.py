@@ -171,6 +171,14 @@ class Context:
         check(load().bia_last_kernel_ms(self._h, C.byref(ms)), "bia_last_kernel_ms", self._h)
         return ms.value
 
+    def kernel_ms(self, max_launches: int = 64) -> list[float]:
+        """Durations of the pair-kernel launches recorded since profiling was switched on (the
+        last 64 at most), oldest first; waits for them."""
+        buf = (C.c_float * max_launches)()
+        n = C.c_int()
+        check(load().bia_kernel_ms(self._h, buf, int(max_launches), C.byref(n)), "bia_kernel_ms", self._h)
+        return [buf[i] for i in range(n.value)]
+
     def synchronize(self):
         check(load().bia_synchronize(self._h), "bia_synchronize", self._h)
 

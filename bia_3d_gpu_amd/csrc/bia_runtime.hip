@@ -1,4 +1,6 @@
 // bia_runtime.hip -- context lifetime, error strings, the default context of the legacy ABI.
+#include <algorithm>
+
 #include "bia_runtime.hpp"
 
 namespace bia {
@@ -28,14 +30,14 @@ void legacy_fail(const char* where, int code, bia_ctx* ctx) {
 
 int prof_begin(bia_ctx* ctx) {
   if (!ctx->profiling) return BIA_OK;
-  BIA_HIP_TRY(ctx, hipEventRecord(ctx->ev_begin, ctx->stream));
+  BIA_HIP_TRY(ctx, hipEventRecord(ctx->ev_begin[ctx->ev_count % kProfSlots], ctx->stream));
   return BIA_OK;
 }
 
 int prof_end(bia_ctx* ctx) {
   if (!ctx->profiling) return BIA_OK;
-  BIA_HIP_TRY(ctx, hipEventRecord(ctx->ev_end, ctx->stream));
-  ctx->ev_pending = 1;
+  BIA_HIP_TRY(ctx, hipEventRecord(ctx->ev_end[ctx->ev_count % kProfSlots], ctx->stream));
+  ctx->ev_count++;
   return BIA_OK;
 }
 
@@ -91,8 +93,10 @@ int bia_destroy(bia_ctx* c) {
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->graph) (void)hipGraphDestroy(c->graph);
-  if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
-  if (c->ev_end) (void)hipEventDestroy(c->ev_end);
+  for (int k = 0; k < bia::kProfSlots; k++) {
+    if (c->ev_begin[k]) (void)hipEventDestroy(c->ev_begin[k]);
+    if (c->ev_end[k]) (void)hipEventDestroy(c->ev_end[k]);
+  }
   delete c;
   return BIA_OK;
 }
@@ -108,21 +112,38 @@ int bia_set_profiling(bia_ctx* c, int on) {
   if (!c) return BIA_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
   BIA_HIP_TRY(c, hipSetDevice(c->device));
-  if (on && !c->ev_begin) {
-    BIA_HIP_TRY(c, hipEventCreate(&c->ev_begin));
-    BIA_HIP_TRY(c, hipEventCreate(&c->ev_end));
+  if (on && !c->ev_begin[0]) {
+    for (int k = 0; k < bia::kProfSlots; k++) {
+      BIA_HIP_TRY(c, hipEventCreate(&c->ev_begin[k]));
+      BIA_HIP_TRY(c, hipEventCreate(&c->ev_end[k]));
+    }
   }
   c->profiling = on ? 1 : 0;
-  c->ev_pending = 0;
+  c->ev_count = 0;
   return BIA_OK;
 }
 
 int bia_last_kernel_ms(bia_ctx* c, float* ms) {
   if (!c || !ms) return BIA_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
-  if (!c->ev_pending) return BIA_ERR_STATE;
-  BIA_HIP_TRY(c, hipEventSynchronize(c->ev_end));
-  BIA_HIP_TRY(c, hipEventElapsedTime(ms, c->ev_begin, c->ev_end));
+  if (c->ev_count == 0) return BIA_ERR_STATE;
+  const int k = (int)((c->ev_count - 1) % bia::kProfSlots);
+  BIA_HIP_TRY(c, hipEventSynchronize(c->ev_end[k]));
+  BIA_HIP_TRY(c, hipEventElapsedTime(ms, c->ev_begin[k], c->ev_end[k]));
+  return BIA_OK;
+}
+
+int bia_kernel_ms(bia_ctx* c, float* ms, int max, int* n) {
+  if (!c || !ms || !n || max < 0) return BIA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  const long long avail = std::min<long long>(c->ev_count, bia::kProfSlots);
+  const int cnt = (int)std::min<long long>(avail, max);
+  for (int i = 0; i < cnt; i++) {  // oldest first
+    const int k = (int)((c->ev_count - cnt + i) % bia::kProfSlots);
+    BIA_HIP_TRY(c, hipEventSynchronize(c->ev_end[k]));
+    BIA_HIP_TRY(c, hipEventElapsedTime(ms + i, c->ev_begin[k], c->ev_end[k]));
+  }
+  *n = cnt;
   return BIA_OK;
 }
 

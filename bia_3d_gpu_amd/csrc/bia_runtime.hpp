@@ -13,6 +13,8 @@
 
 namespace bia {
 
+constexpr int kProfSlots = 64;  // pair-kernel launches whose timing events are kept
+
 // one system of a (possibly batched) step: particles [off, off + n) of the SoA state, nt = ceil(n/64)
 // tiles, partial sums at part + poff laid out [nt][6][64 nt]
 struct SysDesc {
@@ -133,9 +135,11 @@ struct bia_ctx {
   int table_n = 0;
   long long mlp_step = 0;
   // optional timing of the dominant (pair) kernel with HIP events on the context's stream
+  // pair-kernel timing: event pairs recorded around the last kProfSlots launches (a ring), and
+  // the launches recorded since profiling was switched on
   int profiling = 0;
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  int ev_pending = 0;
+  hipEvent_t ev_begin[bia::kProfSlots] = {}, ev_end[bia::kProfSlots] = {};
+  long long ev_count = 0;
   char last_error[256] = {0};
 };
 

@@ -117,6 +117,10 @@ int bia_synchronize(bia_ctx *ctx);
  * such pair and returns the elapsed milliseconds (BIA_ERR_STATE if none was recorded). */
 int bia_set_profiling(bia_ctx *ctx, int on);
 int bia_last_kernel_ms(bia_ctx *ctx, float *ms);
+/* Durations in milliseconds of the last min(max, 64) pair-kernel launches recorded since
+ * profiling was switched on, oldest first (waits for them); *n receives how many were written.
+ * For timing steps enqueued back to back, with no synchronisation per step. */
+int bia_kernel_ms(bia_ctx *ctx, float *ms, int max, int *n);
 
 /* One whole BIA step of every unordered pair of an n-particle system.  No counterpart in the
  * reference ABI: it replaces the per-particle loop of bia/bia.f90:128-202 (N calls of

@@ -225,3 +225,29 @@ def test_monitored_driver_matches_plain_steps(ctx, mode, tmp_path):
     assert h["pairs"] == nsteps * npairs and h["evaluations_per_pair"] > 0
     assert np.isfinite(rep.energy_error()).all() and np.abs(rep.energy_error()).max() < 1e-3
     print(lines[-1])
+
+
+def test_kernel_timing_ring(ctx):
+    """With profiling on, every step's pair kernel is timed by its own pair of HIP events and
+    bia_kernel_ms returns the last 64 durations, oldest first, after steps enqueued with no
+    synchronisation in between; the newest equals bia_last_kernel_ms."""
+    import torch
+    from bia_3d_gpu_amd import MODE_GUESS, ic
+    n = 700
+    s = torch.from_numpy(ic.plummer_system(n, seed=31)).cuda()
+    d = torch.empty_like(s)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    ctx.set_profiling(True)
+    try:
+        for _ in range(3):
+            ctx.step_(s, d, -2.0 / n, 1e-3, mode=MODE_GUESS)
+        three = ctx.kernel_ms()
+        assert len(three) == 3 and all(t > 0 for t in three)
+        assert three[-1] == ctx.last_kernel_ms()
+        for _ in range(67):
+            ctx.step_(s, d, -2.0 / n, 1e-3, mode=MODE_GUESS)
+        ring = ctx.kernel_ms()
+        assert len(ring) == 64 and all(t > 0 for t in ring) and ring[-1] == ctx.last_kernel_ms()
+        assert len(ctx.kernel_ms(5)) == 5
+    finally:
+        ctx.set_profiling(False)
