@@ -279,8 +279,14 @@ BIA_HDF Stumpff stumpff(double y) {
   Stumpff c;
   if (fabs(y) <= 1e-4) {
     const double z = -y;
-    c.c2 = ((i8 * z + i6) * z + i4) * z + i2;
-    c.c3 = ((i9 * z + i7) * z + i5) * z + i3;
+    // the innermost product and sum as two roundings (a multiply and an add, each with one
+    // constant operand, need no constant moved into a vector register first); |z| <= 1e-4
+    // scales their rounding by z^2 below the last bit of c2 and c3
+    double t2 = i8 * z, t3 = i9 * z;
+    t2 = t2 + i6;
+    t3 = t3 + i7;
+    c.c2 = (t2 * z + i4) * z + i2;
+    c.c3 = (t3 * z + i5) * z + i3;
     c.c1 = 1 - y * c.c3;
     c.c0 = 1 - y * c.c2;
   } else {
