@@ -70,6 +70,8 @@ SIGNATURES = {
                                          C.c_ulonglong, _vp]),
     "bia_dim_rkf45_host": (C.c_int, [_vp, C.c_int, C.c_double, _dptr, C.c_double, _dp, _ip, C.c_double,
                                      C.c_double, _dptr, _vp, _ip, _ip, _ip, _ip]),
+    "bia_dim_rkf45_coupled_host": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, _vp, _vp, _dptr, C.c_double, _dp,
+                                             _ip, C.c_double, C.c_double, _dptr, _vp, _ip, _ip, _ip, _ip]),
 }
 
 _lib = None

@@ -325,17 +325,35 @@ class Context:
         return out
 
     def dim_rkf45(self, y_aos: np.ndarray, eps: float, x0: float, xe: float, relerr: float = 0.0,
-                  abserr: float = 1e-16, h: float = 0.0, init: int = 1) -> dict:
+                  abserr: float = 1e-16, h: float = 0.0, init: int = 1, coupling: int = _lib.COUPLING_SCALAR,
+                  coup: float = 1.0, mass=None, charge=None) -> dict:
+        """DIM RKF45 of the AoS state y_aos [N, 6] from x0 to xe.  With the scalar coupling every
+        pair has the reference's eps; with COUPLING_GRAVITY / COUPLING_COULOMB (coup = G or k) the
+        force is weighted by the host arrays mass / charge and eps is unused."""
         y = np.ascontiguousarray(y_aos, dtype=np.float64).reshape(-1).copy()
         ese = np.zeros_like(y)
         x = C.c_double(x0)
         hh = C.c_double(h)
         ini = C.c_int(init)
         nde, nacc, nrej, ier = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
-        s = load().bia_dim_rkf45_host(self._h, y.size // 6, eps, C.byref(x), xe, y, C.byref(ini), relerr, abserr,
-                                      C.byref(hh), ese.ctypes.data_as(C.c_void_p), C.byref(nde), C.byref(nacc),
-                                      C.byref(nrej), C.byref(ier))
+        tail = (C.byref(hh), ese.ctypes.data_as(C.c_void_p), C.byref(nde), C.byref(nacc), C.byref(nrej), C.byref(ier))
+        if coupling == _lib.COUPLING_SCALAR:
+            name = "bia_dim_rkf45_host"
+            s = load().bia_dim_rkf45_host(self._h, y.size // 6, eps, C.byref(x), xe, y, C.byref(ini), relerr, abserr,
+                                          *tail)
+        else:
+            def host(a):
+                return None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            m, q = host(mass), host(charge)
+            for a in (m, q):
+                if a is not None and a.size != y.size // 6:
+                    raise ValueError("mass / charge need one value per particle")
+            name = "bia_dim_rkf45_coupled_host"
+            s = load().bia_dim_rkf45_coupled_host(self._h, coupling, coup, y.size // 6,
+                                                  None if m is None else m.ctypes.data_as(C.c_void_p),
+                                                  None if q is None else q.ctypes.data_as(C.c_void_p),
+                                                  C.byref(x), xe, y, C.byref(ini), relerr, abserr, *tail)
         if s not in (_lib.BIA_OK, _lib.BIA_ERR_RKF):
-            check(s, "bia_dim_rkf45_host", self._h)
+            check(s, name, self._h)
         return dict(y=y, ese=ese, x=x.value, h=hh.value, nde=nde.value, naccept=nacc.value,
                     nreject=nrej.value, ier=ier.value)

@@ -43,10 +43,14 @@ struct DimBufs {
 // partial direct-sum acceleration (rkf_kernel.cu:29-53: a_i = sum_j eps (r_i - r_j) /
 // (|r_i - r_j|^2 + 1e-99)^(3/2)) of the 64 particles of block blockIdx.x over the sources
 // [blockIdx.y jc, min(np, (blockIdx.y + 1) jc)), j ascending; the sources pass through LDS 64 at a
-// time (one coalesced load per lane, then broadcast reads)
+// time (one coalesced load per lane, then broadcast reads).  CPL: the coupling of a pair is a
+// per-source factor sw[j] in place of eps and a per-target factor tw[i] applied to the sums
+// (gravity: sw = -G m, tw = 1; Coulomb: sw = q, tw = k q / m).
+template <bool CPL>
 __global__ __launch_bounds__(kDimRows) void k_dim_force(int np, int ld, double eps, int jc,
-                                                        const double* __restrict__ w, double* __restrict__ part) {
-  __shared__ double src[3][kDimRows];
+                                                        const double* __restrict__ w, const double* __restrict__ tw,
+                                                        const double* __restrict__ sw, double* __restrict__ part) {
+  __shared__ double src[4][kDimRows];
   const int lane = threadIdx.x;
   const int i = blockIdx.x * kDimRows + lane;
   const bool ok = i < np;
@@ -60,16 +64,23 @@ __global__ __launch_bounds__(kDimRows) void k_dim_force(int np, int ld, double e
       src[0][lane] = w[jb + lane];
       src[1][lane] = w[ld + jb + lane];
       src[2][lane] = w[2 * (size_t)ld + jb + lane];
+      if (CPL) src[3][lane] = sw[jb + lane];
     }
     __syncthreads();
     for (int jj = 0; jj < nj; ++jj) {
       const double xij = xi - src[0][jj], yij = yi - src[1][jj], zij = zi - src[2][jj];
       const double rsq = xij * xij + yij * yij + zij * zij + 1e-99;  // rkf_kernel.cu:44
-      const double e = eps / (rsq * sqrt(rsq));
+      const double e = (CPL ? src[3][jj] : eps) / (rsq * sqrt(rsq));
       sx += xij * e;
       sy += yij * e;
       sz += zij * e;
     }
+  }
+  if (CPL && ok) {
+    const double t = tw[i];
+    sx *= t;
+    sy *= t;
+    sz *= t;
   }
   double* p = part + (size_t)blockIdx.y * 3 * ld + i;  // i < ld: the padding columns take zeros
   p[0] = sx;
@@ -193,8 +204,16 @@ struct DimRun {
   int evals = 0;
   hipGraphExec_t trial = nullptr, accept = nullptr;
 
+  const double* tw = nullptr;  // coupled force: per-target and per-source factors (device, ld), else NULL
+  const double* sw = nullptr;
+
   void force(const double* w) {
-    hipLaunchKernelGGL(k_dim_force, dim3(nb, nchunk), dim3(kDimRows), 0, ctx->stream, np, ld, eps, jc, w, b.part);
+    if (sw)
+      hipLaunchKernelGGL(k_dim_force<true>, dim3(nb, nchunk), dim3(kDimRows), 0, ctx->stream, np, ld, eps, jc, w, tw,
+                         sw, b.part);
+    else
+      hipLaunchKernelGGL(k_dim_force<false>, dim3(nb, nchunk), dim3(kDimRows), 0, ctx->stream, np, ld, eps, jc, w,
+                         nullptr, nullptr, b.part);
   }
 
   // Fehlberg step of size hs * hdev from y0 into yn (yn may be y0: each thread reads its own
@@ -277,9 +296,11 @@ struct CaptureStream {
   ~CaptureStream() { ctx->stream = user; }
 };
 
-static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double xe, double* y, int* init,
-                            double relerr, double abserr, double* hio, double* ese, int* nde, int* naccept,
-                            int* nreject, int* ier, FILE* trace) {
+// tw, sw: host arrays (np) of the coupled force's per-target and per-source factors, or NULL for
+// the reference's scalar eps
+static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, const double* tw, const double* sw, double* x0,
+                            double xe, double* y, int* init, double relerr, double abserr, double* hio, double* ese,
+                            int* nde, int* naccept, int* nreject, int* ier, FILE* trace) {
   const double EPSMIN = 1e-15;  // rkf_kernel.cu:201-202
   const int ITEMAX = 10000;
   if (np < 1 || !x0 || !y || !hio || !ier) return BIA_ERR_ARG;
@@ -292,9 +313,9 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
   const int jc = (np + nchunk - 1) / nchunk;
   const int ncb = (np + kDimCombine - 1) / kDimCombine;
   const size_t row = (size_t)6 * ld;
-  // layout: yl0 yln yh0 yh1 | k (6 stages) | w0 w1 | er | part | bmax | h (2)
+  // layout: yl0 yln yh0 yh1 | k (6 stages) | w0 w1 | er | part | bmax | h (2) | tw sw (coupled)
   const size_t npart = (size_t)nchunk * 3 * ld;
-  const size_t ndbl = 4 * row + 6 * row + 2 * row + row + npart + 2 * 6 * (size_t)ncb + 2;
+  const size_t ndbl = 4 * row + 6 * row + 2 * row + row + npart + 2 * 6 * (size_t)ncb + 2 + (sw ? 2 * (size_t)ld : 0);
   BIA_TRY(ctx, ctx->dim_work.ensure(ndbl * sizeof(double)));
   BIA_TRY(ctx, ctx->dim_h.ensure(2 * sizeof(double)));
   double* base = ctx->dim_work.as<double>();
@@ -315,6 +336,16 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, double* x0, double
     for (int c = 0; c < 6; c++) soa[(size_t)c * ld + i] = y[(size_t)6 * i + c];
   BIA_HIP_TRY(ctx, hipMemcpyAsync(yh0, soa.data(), row * sizeof(double), hipMemcpyHostToDevice, s));
   BIA_HIP_TRY(ctx, hipMemcpyAsync(yl0, yh0, row * sizeof(double), hipMemcpyDeviceToDevice, s));
+  std::vector<double> fac;
+  if (sw) {
+    double* d = R.hdev + 2;
+    fac.assign(2 * (size_t)ld, 0.0);
+    std::copy(tw, tw + np, fac.begin());
+    std::copy(sw, sw + np, fac.begin() + ld);
+    BIA_HIP_TRY(ctx, hipMemcpyAsync(d, fac.data(), fac.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    R.tw = d;
+    R.sw = d + ld;
+  }
   double H = *hio;
   if (init && *init == 1) {  // initial step from the derivative, :237-265
     R.force(yl0);
@@ -429,8 +460,31 @@ int bia_dim_rkf45_host(bia_ctx* ctx, int np, double eps, double* x0, double xe, 
   std::lock_guard<std::mutex> lk(ctx->mu);
   ctx->last_error[0] = 0;
   BIA_HIP_TRY(ctx, hipSetDevice(ctx->device));
-  return dim_rkf45_locked(ctx, np, eps, x0, xe, y, init, relerr, abserr, h, ese, nde, naccept, nreject, ier,
-                          nullptr);
+  return dim_rkf45_locked(ctx, np, eps, nullptr, nullptr, x0, xe, y, init, relerr, abserr, h, ese, nde, naccept,
+                          nreject, ier, nullptr);
+}
+
+int bia_dim_rkf45_coupled_host(bia_ctx* ctx, int coupling, double coup, int np, const double* mass,
+                               const double* charge, double* x0, double xe, double* y, int* init, double relerr,
+                               double abserr, double* h, double* ese, int* nde, int* naccept, int* nreject, int* ier) {
+  if (!ctx) return BIA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->last_error[0] = 0;
+  if (np < 1 || !(coupling == BIA_COUPLING_GRAVITY ? mass != nullptr : coupling == BIA_COUPLING_COULOMB && charge))
+    return BIA_ERR_ARG;
+  BIA_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::vector<double> tw(np), sw(np);
+  for (int i = 0; i < np; i++) {
+    if (coupling == BIA_COUPLING_GRAVITY) {  // a_i = -G sum_j m_j r_ij / r^3
+      tw[i] = 1.0;
+      sw[i] = -coup * mass[i];
+    } else {  // a_i = (k q_i / m_i) sum_j q_j r_ij / r^3
+      tw[i] = coup * charge[i] / (mass ? mass[i] : 1.0);
+      sw[i] = charge[i];
+    }
+  }
+  return dim_rkf45_locked(ctx, np, 0.0, tw.data(), sw.data(), x0, xe, y, init, relerr, abserr, h, ese, nde, naccept,
+                          nreject, ier, nullptr);
 }
 
 // Reference-compatible rkf_c_.  When BIA_DIM_TRACE names a file, the accepted-step trajectory of
@@ -456,8 +510,8 @@ void rkf_c_(int neq, double eps_th, double* X0, double* XE, double* YH0, int* IN
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->last_error[0] = 0;
     if (hipSetDevice(ctx->device) != hipSuccess) legacy_fail("rkf_c_", BIA_ERR_HIP, ctx);
-    s = dim_rkf45_locked(ctx, np, eps_th, X0, *XE, y.data(), INIT, *RELERR, *ABSERR, H, ESE, &nde, nullptr,
-                         nullptr, &ier, tr);
+    s = dim_rkf45_locked(ctx, np, eps_th, nullptr, nullptr, X0, *XE, y.data(), INIT, *RELERR, *ABSERR, H, ESE, &nde,
+                         nullptr, nullptr, &ier, tr);
   }
   if (tr) fclose(tr);
   *NDE = nde;

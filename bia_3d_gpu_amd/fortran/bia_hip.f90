@@ -6,6 +6,7 @@ module bia_hip
   implicit none
   integer(c_int), parameter :: BIA_OK = 0, BIA_METHOD_RKF = 0, BIA_METHOD_ALG = 1
   integer(c_int), parameter :: BIA_MODE_PLAIN = 0, BIA_MODE_LEARN = 1, BIA_STATS_LEN = 16
+  integer(c_int), parameter :: BIA_COUPLING_GRAVITY = 1, BIA_COUPLING_COULOMB = 2
 
   interface
     integer(c_int) function bia_create(ctx, device) bind(C, name='bia_create')
@@ -46,6 +47,18 @@ module bia_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: np
       real(c_double), value :: eps, xe, relerr, abserr
+      real(c_double) :: x0, y(*), h, ese(*)
+      integer(c_int) :: init, nde, naccept, nreject, ier
+    end function
+
+    ! mass / charge: pass c_null_ptr for an array the coupling does not use, else c_loc(array)
+    integer(c_int) function bia_dim_rkf45_coupled_host(ctx, coupling, coup, np, mass, charge, x0, xe, y, init, &
+                                                       relerr, abserr, h, ese, nde, naccept, nreject, ier) &
+        bind(C, name='bia_dim_rkf45_coupled_host')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, mass, charge
+      integer(c_int), value :: coupling, np
+      real(c_double), value :: coup, xe, relerr, abserr
       real(c_double) :: x0, y(*), h, ese(*)
       integer(c_int) :: init, nde, naccept, nreject, ier
     end function

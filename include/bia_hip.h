@@ -187,6 +187,20 @@ int bia_dim_rkf45_host(bia_ctx *ctx, int np, double eps, double *x0, double xe, 
                        double relerr, double abserr, double *h, double *ese, int *nde, int *naccept,
                        int *nreject, int *ier);
 
+/* The same DIM integration with the pair couplings of the batched step: the direct-sum
+ * acceleration of particle i is
+ *   BIA_COUPLING_GRAVITY  a_i = -G sum_j m_j (r_i - r_j) / |r_i - r_j|^3            (coup = G)
+ *   BIA_COUPLING_COULOMB  a_i = (k q_i / m_i) sum_j q_j (r_i - r_j) / |r_i - r_j|^3  (coup = k,
+ *                         unit masses if mass is NULL)
+ * so that a pair's relative motion has the eps_ij of the step's coupling.  mass, charge: host
+ * arrays (np).  No counterpart in the reference, whose DIM has one eps for all pairs
+ * (dim/rkf_kernel.cu:29-53): the accuracy reference for ensembles with a mass spectrum or
+ * charges.  Returns BIA_ERR_ARG for another coupling or a missing array. */
+int bia_dim_rkf45_coupled_host(bia_ctx *ctx, int coupling, double coup, int np, const double *mass,
+                               const double *charge, double *x0, double xe, double *y, int *init, double relerr,
+                               double abserr, double *h, double *ese, int *nde, int *naccept, int *nreject,
+                               int *ier);
+
 /* ---------------------------------------------------------------------------------------- */
 /* learned initialiser (no counterpart in the reference)                                     */
 /* ---------------------------------------------------------------------------------------- */
