@@ -58,6 +58,8 @@ SIGNATURES = {
     "bia_invariants_dev": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_int, _vp]),
     "bia_invariants_host": (C.c_int, [_vp, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "bia_invariants_batched_dev": (C.c_int, [_vp, C.c_int, _ip32, _dp, _vp, C.c_int, _vp]),
+    "bia_invariants_coupled_batched_dev": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, _ip32, _vp, C.c_int, _vp,
+                                                     _vp, _vp]),
     "bia_mlp_nparam": (C.c_int, []),
     "bia_mlp_init": (C.c_int, [_vp, C.c_ulonglong]),
     "bia_mlp_set_weights": (C.c_int, [_vp, _fp, C.c_int]),

@@ -283,6 +283,19 @@ class Context:
                                                 out.data_ptr()), "bia_invariants_batched_dev", self._h)
         return out
 
+    def invariants_coupled_(self, state, sizes, out, coupling: int, coup: float = 1.0, mass=None, charge=None):
+        """Mass- and charge-weighted invariants of every system (``mass`` / ``charge`` float64
+        CUDA tensors with the state's columns): KE = sum m|v|^2, PE = -2G sum m_i m_j / r or
+        2k sum q_i q_j / r, so KE + PE = 2E; L and the sums weighted by the masses."""
+        sz, _ = self._batched_args(sizes, None, coupling)
+        if out.numel() < 11 * len(sz) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous tensor of at least 11 * len(sizes) float64")
+        ptr = lambda t: None if t is None else t.data_ptr()
+        check(load().bia_invariants_coupled_batched_dev(self._h, coupling, coup, len(sz), sz, state.data_ptr(),
+                                                        state.shape[1], ptr(mass), ptr(charge), out.data_ptr()),
+              "bia_invariants_coupled_batched_dev", self._h)
+        return out
+
     # -- learned initialiser ---------------------------------------------------------------------
     def mlp_init(self, seed: int = 1):
         check(load().bia_mlp_init(self._h, int(seed)), "bia_mlp_init", self._h)

@@ -75,9 +75,12 @@ __device__ __forceinline__ int sys_of_tile(const InvSys* __restrict__ sys, int n
   return lo;
 }
 
-// potential tiles: part[t] = sum over the tile's pairs of 1/r (double-double)
+// potential tiles: part[t] = sum over the tile's pairs of 1/r, or of pw_i pw_j / r with W
+// (double-double)
+template <bool W>
 __global__ __launch_bounds__(64) void k_inv_pe_tiles(int nsys, const InvSys* __restrict__ sys, int ld,
-                                                     const double* __restrict__ state, DD* __restrict__ part) {
+                                                     const double* __restrict__ state, const double* __restrict__ pw,
+                                                     DD* __restrict__ part) {
   const int lane = threadIdx.x;
   const long long t = blockIdx.x;
   const InvSys S = sys[sys_of_tile(sys, nsys, t)];
@@ -87,6 +90,7 @@ __global__ __launch_bounds__(64) void k_inv_pe_tiles(int nsys, const InvSys* __r
   const int i = I * 64 + lane;
   const bool ok = i < n;
   const double xi = ok ? st[i] : 0, yi = ok ? st[ld + i] : 0, zi = ok ? st[2 * (size_t)ld + i] : 0;
+  const double wi = (W && ok) ? pw[S.col + i] : 1.0;
   DD s{0, 0};
   const int j0 = J * 64, jn = min(64, n - j0);
   for (int jj = 0; jj < jn; ++jj) {
@@ -97,20 +101,23 @@ __global__ __launch_bounds__(64) void k_inv_pe_tiles(int nsys, const InvSys* __r
       const double r2 = dx * dx + dy * dy + dz * dz;
       // 1/r within an ulp from the hardware estimate and one correction (the reference takes
       // rsqrt, bia/invariance.cu:23); coincident particles give +inf as 1/sqrt(0) does
-      dd_add(s, r2 > 0 ? fast_rsqrt(r2) : __builtin_huge_val());
+      const double ir = r2 > 0 ? fast_rsqrt(r2) : __builtin_huge_val();
+      dd_add(s, W ? (wi * pw[S.col + j]) * ir : ir);
     }
   }
   s = dd_wave_sum(s);
   if (lane == 0) part[t] = s;
 }
 
-// ten linear sums per block (Xa Ya Za Ua Va Wa Lx Ly Lz KE); block b of a system takes particles
-// b*256 + thread, stepping by kLinStride, so the partition does not depend on how many blocks
-// of the grid are empty
+// ten linear sums per block (Xa Ya Za Ua Va Wa Lx Ly Lz KE), each term weighted by kw_i with W;
+// block b of a system takes particles b*256 + thread, stepping by kLinStride, so the partition
+// does not depend on how many blocks of the grid are empty
 constexpr int kLinBlocks = 256;
 constexpr int kLinStride = kLinBlocks * 256;
+template <bool W>
 __global__ __launch_bounds__(256) void k_inv_linear(const InvSys* __restrict__ sys, int ld,
-                                                    const double* __restrict__ state, DD* __restrict__ part) {
+                                                    const double* __restrict__ state, const double* __restrict__ kw,
+                                                    DD* __restrict__ part) {
   const InvSys S = sys[blockIdx.y];
   if ((int)blockIdx.x >= S.nlin) return;
   const double* st = state + S.col;
@@ -122,12 +129,22 @@ __global__ __launch_bounds__(256) void k_inv_linear(const InvSys* __restrict__ s
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kLinStride) {
     double x = st[i], y = st[ld + i], z = st[2 * (size_t)ld + i];
     double u = st[3 * (size_t)ld + i], v = st[4 * (size_t)ld + i], w = st[5 * (size_t)ld + i];
-    dd_add(a[0], x); dd_add(a[1], y); dd_add(a[2], z);
-    dd_add(a[3], u); dd_add(a[4], v); dd_add(a[5], w);
-    dd_add(a[6], y * w - z * v);
-    dd_add(a[7], z * u - x * w);
-    dd_add(a[8], x * v - y * u);
-    dd_add(a[9], u * u + v * v + w * w);
+    if (W) {
+      const double m = kw[S.col + i];
+      dd_add(a[0], m * x); dd_add(a[1], m * y); dd_add(a[2], m * z);
+      dd_add(a[3], m * u); dd_add(a[4], m * v); dd_add(a[5], m * w);
+      dd_add(a[6], m * (y * w - z * v));
+      dd_add(a[7], m * (z * u - x * w));
+      dd_add(a[8], m * (x * v - y * u));
+      dd_add(a[9], m * (u * u + v * v + w * w));
+    } else {
+      dd_add(a[0], x); dd_add(a[1], y); dd_add(a[2], z);
+      dd_add(a[3], u); dd_add(a[4], v); dd_add(a[5], w);
+      dd_add(a[6], y * w - z * v);
+      dd_add(a[7], z * u - x * w);
+      dd_add(a[8], x * v - y * u);
+      dd_add(a[9], u * u + v * v + w * w);
+    }
   }
   const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
 #pragma unroll
@@ -242,9 +259,11 @@ static int ensure_inv_plan(bia_ctx* ctx, int nsys, const int* sizes, const doubl
   return BIA_OK;
 }
 
-// invariants of nsys systems side by side in a [6, ld] state (columns in the order of sizes)
+// invariants of nsys systems side by side in a [6, ld] state (columns in the order of sizes);
+// kw, pw: per-particle weights of the linear sums and of the potential (device, same columns as
+// the state), or NULL for the reference's unweighted sums
 static int invariants_locked(bia_ctx* ctx, int nsys, const int* sizes, const double* eps, const double* d_state,
-                             int ld, double* d_out) {
+                             int ld, double* d_out, const double* kw = nullptr, const double* pw = nullptr) {
   if (nsys < 1 || !sizes || !eps || !d_state || !d_out) return BIA_ERR_ARG;
   long long total = 0;
   for (int s = 0; s < nsys; s++) {
@@ -259,12 +278,21 @@ static int invariants_locked(bia_ctx* ctx, int nsys, const int* sizes, const dou
   DD* pe = ctx->inv_part.as<DD>();
   DD* lin = pe + ctx->inv_ntiles;
   DD* pe2 = lin + (size_t)ctx->inv_nlin * 10;
-  hipLaunchKernelGGL(k_inv_pe_tiles, dim3((unsigned)ctx->inv_ntiles), dim3(64), 0, ctx->stream, nsys, sys, ld,
-                     d_state, pe);
+  if (pw)
+    hipLaunchKernelGGL(k_inv_pe_tiles<true>, dim3((unsigned)ctx->inv_ntiles), dim3(64), 0, ctx->stream, nsys, sys, ld,
+                       d_state, pw, pe);
+  else
+    hipLaunchKernelGGL(k_inv_pe_tiles<false>, dim3((unsigned)ctx->inv_ntiles), dim3(64), 0, ctx->stream, nsys, sys,
+                       ld, d_state, nullptr, pe);
   BIA_HIP_TRY(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_inv_pe_reduce, dim3(ctx->inv_maxnpe, nsys), dim3(256), 0, ctx->stream, sys, pe, pe2);
   BIA_HIP_TRY(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_inv_linear, dim3(ctx->inv_maxnlin, nsys), dim3(256), 0, ctx->stream, sys, ld, d_state, lin);
+  if (kw)
+    hipLaunchKernelGGL(k_inv_linear<true>, dim3(ctx->inv_maxnlin, nsys), dim3(256), 0, ctx->stream, sys, ld, d_state,
+                       kw, lin);
+  else
+    hipLaunchKernelGGL(k_inv_linear<false>, dim3(ctx->inv_maxnlin, nsys), dim3(256), 0, ctx->stream, sys, ld,
+                       d_state, nullptr, lin);
   BIA_HIP_TRY(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_inv_final, dim3(nsys), dim3(64), 0, ctx->stream, sys, lin, pe2, d_out);
   BIA_HIP_TRY(ctx, hipGetLastError());
@@ -305,6 +333,22 @@ int bia_invariants_batched_dev(bia_ctx* ctx, int nsys, const int* sizes, const d
   ctx->last_error[0] = 0;
   BIA_HIP_TRY(ctx, hipSetDevice(ctx->device));
   return invariants_locked(ctx, nsys, sizes, eps, d_state, ld, d_out);
+}
+
+int bia_invariants_coupled_batched_dev(bia_ctx* ctx, int coupling, double coup, int nsys, const int* sizes,
+                                       const double* d_state, int ld, const double* d_mass, const double* d_charge,
+                                       double* d_out) {
+  if (!ctx) return BIA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->last_error[0] = 0;
+  if (nsys < 1 || nsys > 65535 ||
+      !(coupling == BIA_COUPLING_GRAVITY ? d_mass != nullptr : coupling == BIA_COUPLING_COULOMB && d_charge))
+    return BIA_ERR_ARG;
+  BIA_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  // KE + PE = 2 E: gravity PE = -2 G sum m_i m_j / r, Coulomb PE = 2 k sum q_i q_j / r
+  const std::vector<double> eps(nsys, coupling == BIA_COUPLING_GRAVITY ? -2.0 * coup : 2.0 * coup);
+  return invariants_locked(ctx, nsys, sizes, eps.data(), d_state, ld, d_out, d_mass,
+                           coupling == BIA_COUPLING_GRAVITY ? d_mass : d_charge);
 }
 
 int bia_invariants_host(bia_ctx* ctx, int n, double eps, const double* X, const double* Y, const double* Z,

@@ -179,6 +179,17 @@ int bia_invariants_host(bia_ctx *ctx, int n, double eps, const double *X, const 
 int bia_invariants_batched_dev(bia_ctx *ctx, int nsys, const int *sizes, const double *eps, const double *d_state,
                                int ld, double *d_out);
 
+/* The invariants of bia_invariants_batched_dev with the couplings of the batched step: KE, L and
+ * the position and velocity sums weighted by the masses (unit masses for COULOMB with d_mass
+ * NULL), and PE = -2 G sum_{i<j} m_i m_j / r (GRAVITY, coup = G) or 2 k sum_{i<j} q_i q_j / r
+ * (COULOMB, coup = k), so that KE + PE is twice the total energy, as KE + PE of the reference's
+ * equal-mass form is 2 E / m.  d_mass, d_charge: device arrays with the state's columns.  No
+ * counterpart in the reference (its invariants are for equal masses, bia/invariance.cu:15-48):
+ * the energy check of ensembles with a mass spectrum or charges. */
+int bia_invariants_coupled_batched_dev(bia_ctx *ctx, int coupling, double coup, int nsys, const int *sizes,
+                                       const double *d_state, int ld, const double *d_mass, const double *d_charge,
+                                       double *d_out);
+
 /* DIM direct-sum RKF45 over [*x0, xe] on an AoS state y[6 np] (host, in/out), with the
  * reference's step controller (dim/rkf_kernel.cu:237-491).  *h is the step (in/out; set
  * when *init == 1).  ese (may be NULL) receives (YLN - YHN)/31.  Counters may be NULL.

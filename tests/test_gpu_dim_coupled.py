@@ -131,3 +131,70 @@ def _scalar_through_coupled(ctx, st):
                                           C.byref(x), 0.1, y, C.byref(ini), 0.0, 1e-16, C.byref(h), None,
                                           C.byref(i0), C.byref(i1), C.byref(i2), C.byref(i3))
     check(s, "bia_dim_rkf45_coupled_host", ctx._h)
+
+
+# ---- mass- and charge-weighted invariants (bia_invariants_coupled_batched_dev) ----------------
+
+def _inv_coupled(ctx, st, sizes, coupling, coup, mass=None, charge=None):
+    import torch
+    cu = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    out = torch.zeros(len(sizes), 11, dtype=torch.float64, device="cuda")
+    ctx.invariants_coupled_(cu(st), sizes, out, coupling, coup, mass=cu(mass), charge=cu(charge))
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def _inv_longdouble(st, w_k, w_p, f):
+    """KE PE Lx Ly Lz Ua Va Wa Xa Ya Za by direct sums in 80-bit long double."""
+    s = st.astype(np.longdouble)
+    wk, wp = w_k.astype(np.longdouble), w_p.astype(np.longdouble)
+    x, v = s[:3], s[3:]
+    pe = np.longdouble(0)
+    for i in range(s.shape[1] - 1):
+        r = np.sqrt(((x[:, i + 1:] - x[:, i:i + 1]) ** 2).sum(axis=0))
+        pe += (wp[i] * wp[i + 1:] / r).sum()
+    L = (wk * np.cross(x.T, v.T).T).sum(axis=1)
+    return np.concatenate([[(wk * (v ** 2).sum(axis=0)).sum(), np.longdouble(f) * pe], L, (wk * v).sum(axis=1),
+                           (wk * x).sum(axis=1)])
+
+
+def test_weighted_invariants_reduce_to_the_reference_form(ctx):
+    """Equal masses m = 1/N: every weighted invariant is the reference-form one (eps = -2 G m)
+    divided by N, to rounding."""
+    import torch
+    from bia_3d_gpu_amd import ic
+    n = 1000
+    st = ic.plummer_system(n, seed=812)
+    got = _inv_coupled(ctx, st, [n], COUPLING_GRAVITY, 1.0, mass=np.full(n, 1.0 / n))[0]
+    ref = torch.zeros(11, dtype=torch.float64, device="cuda")
+    ctx.invariants_(torch.from_numpy(st).cuda(), ref, -2.0 / n)
+    ref = ref.cpu().numpy() / n
+    scale = np.array([abs(ref[0]), abs(ref[1])] + [np.abs(st).sum() / n] * 9)
+    assert np.all(np.abs(got - ref) <= 1e-15 * scale)
+
+
+@pytest.mark.parametrize("coulomb", [False, True])
+def test_weighted_invariants_against_long_double(ctx, coulomb):
+    """Unequal masses (and charges): the device sums against 80-bit direct sums, two systems in
+    one call."""
+    from bia_3d_gpu_amd import ic
+    rng = np.random.default_rng(813)
+    sizes = [700, 333]
+    st = np.concatenate([ic.plummer_system(n, seed=814 + k) for k, n in enumerate(sizes)], axis=1)
+    m = np.exp(rng.uniform(np.log(0.1), np.log(10), sum(sizes)))
+    q = rng.choice([-1.0, 1.0], sum(sizes)) if coulomb else None
+    cpl, coup = (COUPLING_COULOMB, 0.7) if coulomb else (COUPLING_GRAVITY, 1.3)
+    got = _inv_coupled(ctx, st, sizes, cpl, coup, mass=m, charge=q)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    for k, (a, b) in enumerate(zip(offs[:-1], offs[1:])):
+        f = 2 * coup if coulomb else -2 * coup
+        want = _inv_longdouble(st[:, a:b], m[a:b], (q if coulomb else m)[a:b], f)
+        ab = np.abs(st[:, a:b])
+        pw = np.abs(q[a:b]) if coulomb else m[a:b]
+        scale = np.array([(m[a:b] * (st[3:, a:b] ** 2).sum(axis=0)).sum(),
+                          abs(f) * float(_inv_longdouble(st[:, a:b], m[a:b], pw, 1.0)[1])]
+                         + [(m[a:b] * ab[:3].sum(axis=0) * ab[3:].sum(axis=0)).sum()] * 3
+                         + [(m[a:b] * ab[3:]).sum()] * 3 + [(m[a:b] * ab[:3]).sum()] * 3)
+        err = np.abs(got[k] - want.astype(np.float64)) / scale
+        print(f"{'Coulomb' if coulomb else 'gravity'} system {k}: worst relative error {err.max():.1e}")
+        assert err.max() < 1e-16
