@@ -172,4 +172,10 @@ bia_ctx* default_ctx();
 // record the profiling events around the pair kernel (no-ops unless profiling is on)
 int prof_begin(bia_ctx* ctx);
 int prof_end(bia_ctx* ctx);
+// waits for the context's stream when it goes out of scope, so that a call returning early on
+// an error leaves no copy pending from or into host memory that the caller may then release
+struct StreamDrain {
+  bia_ctx* ctx;
+  ~StreamDrain() { (void)hipStreamSynchronize(ctx->stream); }
+};
 }  // namespace bia

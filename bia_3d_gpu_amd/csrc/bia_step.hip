@@ -707,6 +707,7 @@ int bia_step_host(bia_ctx* ctx, int method, int mode, int n, double eps, double 
   for (int c = 0; c < 6; c++)
     if (!in[c] || !out[c]) return BIA_ERR_ARG;
   BIA_LOCKED(ctx);
+  StreamDrain drain{ctx};  // the copies below read and write the caller's host arrays
   const int ld = n;
   // device staging: [state 6n][delta 6n][stats]
   BIA_TRY(ctx, ctx->delta.ensure((size_t)12 * ld * sizeof(double) + BIA_STATS_LEN * 8));

@@ -306,6 +306,10 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, const double* tw, 
   if (np < 1 || !x0 || !y || !hio || !ier) return BIA_ERR_ARG;
   CaptureStream cs(ctx);
   BIA_TRY(ctx, cs.enter());
+  // host vectors copied from and into asynchronously, declared before the drain so that they
+  // outlive it on every return path
+  std::vector<double> soa, fac, f, bm, p0, outh, outl;
+  StreamDrain drain{ctx};
   const int ld = ((np + kDimPad - 1) / kDimPad) * kDimPad;
   const int nb = (np + kDimRows - 1) / kDimRows;
   // chunks of j: enough force waves to fill the GPU, at least kDimMinChunk sources each
@@ -331,12 +335,11 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, const double* tw, 
   double *yl0 = R.b.y[0], *yh0 = R.b.y[2];
   hipStream_t s = ctx->stream;
   // AoS host -> SoA device (padding rows zeroed once)
-  std::vector<double> soa(row, 0.0);
+  soa.assign(row, 0.0);
   for (int i = 0; i < np; i++)
     for (int c = 0; c < 6; c++) soa[(size_t)c * ld + i] = y[(size_t)6 * i + c];
   BIA_HIP_TRY(ctx, hipMemcpyAsync(yh0, soa.data(), row * sizeof(double), hipMemcpyHostToDevice, s));
   BIA_HIP_TRY(ctx, hipMemcpyAsync(yl0, yh0, row * sizeof(double), hipMemcpyDeviceToDevice, s));
-  std::vector<double> fac;
   if (sw) {
     double* d = R.hdev + 2;
     fac.assign(2 * (size_t)ld, 0.0);
@@ -351,7 +354,7 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, const double* tw, 
     R.force(yl0);
     hipLaunchKernelGGL(k_dim_func, dim3(ncb, 6), dim3(kDimCombine), 0, s, np, ld, nchunk, R.b.part, yl0, R.b.k);
     BIA_HIP_TRY(ctx, hipGetLastError());
-    std::vector<double> f(row);
+    f.assign(row, 0.0);
     BIA_HIP_TRY(ctx, hipMemcpyAsync(f.data(), R.b.k, row * sizeof(double), hipMemcpyDeviceToHost, s));
     BIA_HIP_TRY(ctx, hipStreamSynchronize(s));
     double toldy = 1.7976931348623157e308;
@@ -369,8 +372,8 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, const double* tw, 
     *init = 2;
     R.evals = 1;
   }
-  std::vector<double> bm(2 * 6 * (size_t)ncb);
-  std::vector<double> p0(6);
+  bm.assign(2 * 6 * (size_t)ncb, 0.0);
+  p0.assign(6, 0.0);
   auto trace_row = [&](double t, const double* dy) -> int {
     if (!trace) return BIA_OK;
     for (int c = 0; c < 6; c++)
@@ -426,7 +429,8 @@ static int dim_rkf45_locked(bia_ctx* ctx, int np, double eps, const double* tw, 
       acc++;
       *x0 = xe;
       // results back to AoS host arrays
-      std::vector<double> outh(row), outl(row);
+      outh.assign(row, 0.0);
+      outl.assign(row, 0.0);
       BIA_HIP_TRY(ctx, hipMemcpyAsync(outh.data(), yhi, row * sizeof(double), hipMemcpyDeviceToHost, s));
       BIA_HIP_TRY(ctx, hipMemcpyAsync(outl.data(), R.b.y[1], row * sizeof(double), hipMemcpyDeviceToHost, s));
       BIA_HIP_TRY(ctx, hipStreamSynchronize(s));

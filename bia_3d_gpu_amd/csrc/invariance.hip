@@ -300,6 +300,7 @@ static int invariants_locked(bia_ctx* ctx, int nsys, const int* sizes, const dou
 }
 
 static int invariants_host_locked(bia_ctx* ctx, int n, double eps, const double* const in[6], double* out) {
+  StreamDrain drain{ctx};  // the copies below read and write the caller's host arrays
   BIA_TRY(ctx, ctx->inv_state.ensure((size_t)6 * n * sizeof(double) + 16 * sizeof(double)));
   double* d = ctx->inv_state.as<double>();
   double* dout = d + (size_t)6 * n;
