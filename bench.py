@@ -168,14 +168,20 @@ def run_mass_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 4096):
     kw = dict(dt=5e-4, mode=mode, coupling=COUPLING_GRAVITY, coup=1.0, mass=mc)
     ctx.autotune_(s, d, sizes, **kw)
     ctx.run_steps_(s, d, sizes, max(warmup, 1), use_graph=True, **kw)
+    inv = torch.zeros(2, len(sizes), 11, dtype=torch.float64, device="cuda:0")
+    ctx.invariants_coupled_(s, sizes, inv[0], COUPLING_GRAVITY, 1.0, mass=mc)  # mass-weighted, KE + PE = 2E
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ctx.run_steps_(s, d, sizes, steps, use_graph=True, **kw)
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / steps
+    ctx.invariants_coupled_(s, sizes, inv[1], COUPLING_GRAVITY, 1.0, mass=mc)
+    e = (inv[:, :, 0] + inv[:, :, 1]).cpu().numpy()
+    de = np.abs(e[1] / e[0] - 1)
     pairs = float(sum(n * (n - 1) // 2 for n in sizes))
     return {"systems": 64, "particles": int(sum(sizes)), "pairs_per_step": pairs, "ms_per_step": sec * 1e3,
-            "pairs_per_s": pairs / sec, "steps": steps, "graph": True, "coupling": "gravity, mass spectrum"}
+            "pairs_per_s": pairs / sec, "steps": steps, "graph": True, "coupling": "gravity, mass spectrum",
+            "dE_rel_median": float(np.median(de)), "dE_rel_max": float(de.max())}
 
 
 def run_fixture(ctx, steps: int):
