@@ -9,7 +9,8 @@ driver (:mod:`.driver`) and the learned-initialiser training (:mod:`.learn`).
 """
 from . import io, ic  # noqa: F401
 from .api import (  # noqa: F401
-    METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, STATS_LEN, Context, find_dxyuv_kernel, flops_per_step,
+    METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, PAIR_RECORD_DTYPE, STATS_LEN, Context, FailureLog,
+    find_dxyuv_kernel, flops_per_step,
     invariance_gpu, reference_step, rkf_c,
 )
 from ._lib import COUPLING_COULOMB, COUPLING_GRAVITY, COUPLING_SCALAR, LIB_PATH, load  # noqa: F401

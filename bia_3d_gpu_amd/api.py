@@ -18,7 +18,8 @@ import numpy as np
 from . import _lib
 from ._lib import METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, STATS_LEN, check, load
 
-__all__ = ["find_dxyuv_kernel", "invariance_gpu", "rkf_c", "reference_step", "Context",
+__all__ = ["find_dxyuv_kernel", "invariance_gpu", "rkf_c", "reference_step", "Context", "FailureLog",
+           "PAIR_RECORD_DTYPE",
            "METHOD_ALG", "METHOD_RKF", "MODE_PLAIN", "MODE_LEARN", "MODE_GUESS", "STATS_LEN", "flops_per_step"]
 
 INVARIANT_NAMES = ("KE", "PE", "Lx", "Ly", "Lz", "Ua", "Va", "Wa", "Xa", "Ya", "Za")
@@ -29,6 +30,81 @@ def _f64(a) -> np.ndarray:
     if a.dtype != np.float64 or not a.flags.c_contiguous:
         raise TypeError("expected a C-contiguous float64 array (updated in place)")
     return a
+
+
+# ------------------------------------------------------------------------------------------------
+# argument checks of the device-tensor entry points: a wrong dtype, a strided or short tensor or a
+# host tensor would become an out-of-bounds device access or silently wrong results, so every
+# entry point checks its tensors here, before any library call (dtype, layout and size first, the
+# device last)
+# ------------------------------------------------------------------------------------------------
+def _dev(t, name: str, dtype: str, min_numel: int = 0, shape=None):
+    """Check a device tensor argument; returns it.  ``dtype`` is a torch dtype name."""
+    if t is None or not hasattr(t, "data_ptr") or not hasattr(t, "is_cuda"):
+        raise ValueError(f"{name}: expected a torch tensor")
+    if str(t.dtype) != "torch." + dtype:
+        raise ValueError(f"{name}: expected dtype {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous tensor")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: expected shape {tuple(shape)}, got {tuple(t.shape)}")
+    if t.numel() < min_numel:
+        raise ValueError(f"{name}: needs at least {min_numel} elements, has {t.numel()}")
+    return t
+
+
+def _on_device(**tensors):
+    """The last check of an entry point: every tensor argument lives on the device."""
+    for name, t in tensors.items():
+        if t is not None and not t.is_cuda:
+            raise ValueError(f"{name}: expected a device (cuda) tensor")
+
+
+def _state_pair(state, delta=None):
+    """[6, ld] float64 device state (and a delta of the same shape); returns ld."""
+    if state is None or not hasattr(state, "shape") or len(state.shape) != 2 or state.shape[0] != 6:
+        raise ValueError("state: expected a [6, ld] tensor")
+    _dev(state, "state", "float64")
+    if delta is not None:
+        _dev(delta, "delta", "float64", shape=state.shape)
+    return int(state.shape[1])
+
+
+def _sizes_fit(sizes, ld: int) -> np.ndarray:
+    sz = np.ascontiguousarray(sizes, dtype=np.int32).reshape(-1)
+    if sz.size < 1 or (sz < 1).any():
+        raise ValueError("sizes: need at least one system and every size >= 1")
+    if int(sz.astype(np.int64).sum()) > ld:
+        raise ValueError(f"sizes: the systems hold {int(sz.sum())} particles, the state only {ld} columns")
+    return sz
+
+
+def _stats_arg(stats):
+    return None if stats is None else _dev(stats, "stats", "int64", STATS_LEN)
+
+
+def _couplings(coupling: int, mass, charge, ntot: int):
+    if coupling not in (_lib.COUPLING_SCALAR, _lib.COUPLING_GRAVITY, _lib.COUPLING_COULOMB):
+        raise ValueError(f"coupling: unknown value {coupling}")
+    if coupling == _lib.COUPLING_GRAVITY and mass is None:
+        raise ValueError("mass: the gravity coupling needs per-particle masses")
+    if coupling == _lib.COUPLING_COULOMB and charge is None:
+        raise ValueError("charge: the Coulomb coupling needs per-particle charges")
+    if mass is not None:
+        _dev(mass, "mass", "float64", ntot)
+    if charge is not None:
+        _dev(charge, "charge", "float64", ntot)
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def _rows(rows):
+    """float32 sample rows [B, 4] (u1, u2, u3, target) on the device."""
+    if rows is None or not hasattr(rows, "shape") or len(rows.shape) != 2 or rows.shape[1] != 4:
+        raise ValueError("rows: expected a [B, 4] tensor")
+    return _dev(rows, "rows", "float32", 4)
 
 
 # ------------------------------------------------------------------------------------------------
@@ -127,6 +203,42 @@ def flops_per_step(stats, mode: int = MODE_PLAIN) -> float:
 
 
 # ------------------------------------------------------------------------------------------------
+# per-pair failure records (bia_set_failure_log)
+# ------------------------------------------------------------------------------------------------
+PAIR_RECORD_DTYPE = np.dtype([("sys", "<i4"), ("i", "<i4"), ("j", "<i4"), ("branch", "<i4"), ("iters", "<i4"),
+                              ("flags", "<u4"), ("error", "<f8")])
+assert PAIR_RECORD_DTYPE.itemsize == 32
+
+
+class FailureLog:
+    """Device buffers of bia_set_failure_log: room for ``capacity`` records of the pairs a solver
+    could not finish (iteration cap, RKF45 failure, non-finite output) and their count, which
+    includes the pairs that did not fit.  The reference's CPU loop writes the same pairs to its
+    units 88 / 99 (bia/bia.f90:151-154,167-170)."""
+
+    def __init__(self, capacity: int = 4096, device: str = "cuda"):
+        import torch
+        if capacity < 0:
+            raise ValueError("capacity must be >= 0")
+        self.capacity = int(capacity)
+        self.buf = torch.zeros(max(1, self.capacity) * PAIR_RECORD_DTYPE.itemsize, dtype=torch.uint8, device=device)
+        self.count = torch.zeros(1, dtype=torch.int64, device=device)
+
+    def reset(self):
+        self.count.zero_()
+
+    def total(self) -> int:
+        """Pairs logged since the last reset (written or not)."""
+        return int(self.count.item())
+
+    def records(self) -> np.ndarray:
+        """The written records, sorted by (sys, i, j) (the device writes them in no particular order)."""
+        n = min(self.total(), self.capacity)
+        raw = self.buf[: n * PAIR_RECORD_DTYPE.itemsize].cpu().numpy()
+        return np.sort(raw.view(PAIR_RECORD_DTYPE), order=["sys", "i", "j"])
+
+
+# ------------------------------------------------------------------------------------------------
 # context API
 # ------------------------------------------------------------------------------------------------
 class Context:
@@ -182,19 +294,34 @@ class Context:
     def synchronize(self):
         check(load().bia_synchronize(self._h), "bia_synchronize", self._h)
 
+    def set_rkf_tolerance(self, relerr: float = 0.0, abserr: float = 1e-16):
+        """Per-pair RKF45 tolerances of the following steps (the reference's 0 / 1e-16 by
+        default, bia/bia-rkf.cu:5)."""
+        check(load().bia_set_rkf_tolerance(self._h, float(relerr), float(abserr)), "bia_set_rkf_tolerance", self._h)
+
+    def set_failure_log(self, log: "FailureLog | None"):
+        """Record the pairs the solvers could not finish in ``log`` during the following steps
+        (None switches it off).  Steps with a log run the pair kernel's statistics variant."""
+        if log is None:
+            check(load().bia_set_failure_log(self._h, None, 0, None), "bia_set_failure_log", self._h)
+            self._log = None
+            return
+        _dev(log.buf, "log.buf", "uint8", log.capacity * PAIR_RECORD_DTYPE.itemsize)
+        _dev(log.count, "log.count", "int64", 1)
+        _on_device(buf=log.buf, count=log.count)
+        check(load().bia_set_failure_log(self._h, log.buf.data_ptr(), log.capacity, log.count.data_ptr()),
+              "bia_set_failure_log", self._h)
+        self._log = log  # the buffers must outlive the steps that write them
+
     # -- whole step on device memory (torch tensors) -------------------------------------------
     def step_(self, state, delta, eps: float, dt: float, method: int = METHOD_ALG, mode: int = MODE_PLAIN,
               itr: int = 18, stats=None):
         """Device step: ``state`` and ``delta`` are CUDA/HIP float64 tensors ``[6, ld]``."""
-        import torch
-
-        assert state.dtype == torch.float64 and delta.dtype == torch.float64
-        assert state.is_cuda and delta.is_cuda and state.is_contiguous() and delta.is_contiguous()
-        n = state.shape[1]
-        assert delta.shape == state.shape
-        sp = 0 if stats is None else stats.data_ptr()
+        n = _state_pair(state, delta)
+        _stats_arg(stats)
+        _on_device(state=state, delta=delta, stats=stats)
         check(load().bia_step_dev(self._h, method, mode, n, eps, dt, itr, state.data_ptr(), n, delta.data_ptr(),
-                                  sp or None), "bia_step_dev", self._h)
+                                  _ptr(stats)), "bia_step_dev", self._h)
         return delta
 
     # -- whole step on host arrays --------------------------------------------------------------
@@ -219,23 +346,38 @@ class Context:
 
     def invariants_(self, state, out, eps: float):
         """Device invariants: ``state`` [6, n] float64 tensor, ``out`` 11-element float64 tensor."""
+        _state_pair(state)
+        _dev(out, "out", "float64", 11)
+        _on_device(state=state, out=out)
         check(load().bia_invariants_dev(self._h, state.shape[1], eps, state.data_ptr(), state.shape[1],
                                         out.data_ptr()), "bia_invariants_dev", self._h)
         return out
 
     # -- batched / coupled steps, autotuner, multi-step driver ---------------------------------
     def _batched_args(self, sizes, eps, coupling):
-        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32).reshape(-1)
+        if coupling == _lib.COUPLING_SCALAR and eps is None:
+            raise ValueError("eps: the scalar coupling needs eps (one value or one per system)")
         e = np.ascontiguousarray(np.zeros(len(sizes)) if eps is None else np.broadcast_to(eps, len(sizes)),
                                  dtype=np.float64)
         return sizes, e
+
+    def _step_args(self, state, delta, sizes, eps, coupling, mass, charge, stats=None):
+        """Checks of the batched entry points' tensors; returns (sizes, eps) for the call."""
+        ld = _state_pair(state, delta)
+        sz = _sizes_fit(sizes, ld)
+        _couplings(coupling, mass, charge, int(sz.astype(np.int64).sum()))
+        _stats_arg(stats)
+        out = self._batched_args(sz, eps, coupling)
+        _on_device(state=state, delta=delta, mass=mass, charge=charge, stats=stats)
+        return out
 
     def step_batched_(self, state, delta, sizes, eps=None, dt: float = 1e-3, method: int = METHOD_ALG,
                       mode: int = MODE_PLAIN, coupling: int = 0, coup: float = 1.0, mass=None, charge=None,
                       itr: int = 18, stats=None):
         """Systems of the given sizes concatenated column-wise in state/delta ([6, ld] CUDA f64)."""
-        sz, e = self._batched_args(sizes, eps, coupling)
-        ptr = lambda t: None if t is None else t.data_ptr()
+        sz, e = self._step_args(state, delta, sizes, eps, coupling, mass, charge, stats)
+        ptr = _ptr
         check(load().bia_step_batched_dev(self._h, method, mode, coupling, len(sz), sz, e, coup, dt, itr,
                                           state.data_ptr(), state.shape[1], ptr(mass), ptr(charge), delta.data_ptr(),
                                           ptr(stats)), "bia_step_batched_dev", self._h)
@@ -243,8 +385,8 @@ class Context:
 
     def autotune_(self, state, delta, sizes, eps=None, dt: float = 1e-3, method: int = METHOD_ALG,
                   mode: int = MODE_PLAIN, coupling: int = 0, coup: float = 1.0, mass=None, charge=None, itr: int = 18):
-        sz, e = self._batched_args(sizes, eps, coupling)
-        ptr = lambda t: None if t is None else t.data_ptr()
+        sz, e = self._step_args(state, delta, sizes, eps, coupling, mass, charge)
+        ptr = _ptr
         ms, g = C.c_float(), C.c_int()
         check(load().bia_autotune_dev(self._h, method, mode, coupling, len(sz), sz, e, coup, dt, itr,
                                       state.data_ptr(), state.shape[1], ptr(mass), ptr(charge), delta.data_ptr(),
@@ -256,8 +398,10 @@ class Context:
                    itr: int = 18, use_graph: bool = True, stats=None):
         """nsteps of (step, state += delta) on the device; ``stats`` (int64 CUDA tensor of
         STATS_LEN, optional) accumulates the per-pair statistics of every step."""
-        sz, e = self._batched_args(sizes, eps, coupling)
-        ptr = lambda t: None if t is None else t.data_ptr()
+        if int(nsteps) < 0:
+            raise ValueError("nsteps must be >= 0")
+        sz, e = self._step_args(state, delta, sizes, eps, coupling, mass, charge, stats)
+        ptr = _ptr
         check(load().bia_run_steps_dev(self._h, method, mode, coupling, len(sz), sz, e, coup, dt, itr,
                                        state.data_ptr(), state.shape[1], ptr(mass), ptr(charge), delta.data_ptr(),
                                        nsteps, int(use_graph), ptr(stats)), "bia_run_steps_dev", self._h)
@@ -266,7 +410,11 @@ class Context:
     def invariants_at_(self, state, offset: int, n: int, eps: float, out):
         """Device invariants of the n particles at column ``offset`` of a [6, ld] state (one system
         of an ensemble); ``out`` is an 11-element float64 CUDA tensor (any offset into a larger one)."""
-        ld = state.shape[1]
+        ld = _state_pair(state)
+        _dev(out, "out", "float64", 11)
+        if int(offset) < 0 or int(n) < 1 or int(offset) + int(n) > ld:
+            raise ValueError("offset / n: the system must lie inside the state's columns")
+        _on_device(state=state, out=out)
         ptr = state.data_ptr() + 8 * int(offset)
         check(load().bia_invariants_dev(self._h, int(n), float(eps), ptr, ld, out.data_ptr()),
               "bia_invariants_dev", self._h)
@@ -276,9 +424,9 @@ class Context:
         """Device invariants of every system of a [6, ld] state (systems of ``sizes`` side by
         side, ``eps`` per system or one for all) into ``out``, a float64 CUDA tensor of
         ``len(sizes) * 11`` elements (e.g. [nsys, 11]); one launch per kernel for the list."""
-        sz, e = self._batched_args(sizes, eps, 0)
-        if out.numel() < 11 * len(sz) or not out.is_contiguous():
-            raise ValueError("out must be a contiguous tensor of at least 11 * len(sizes) float64")
+        sz, e = self._batched_args(_sizes_fit(sizes, _state_pair(state)), eps, 0)
+        _dev(out, "out", "float64", 11 * len(sz))
+        _on_device(state=state, out=out)
         check(load().bia_invariants_batched_dev(self._h, len(sz), sz, e, state.data_ptr(), state.shape[1],
                                                 out.data_ptr()), "bia_invariants_batched_dev", self._h)
         return out
@@ -287,10 +435,13 @@ class Context:
         """Mass- and charge-weighted invariants of every system (``mass`` / ``charge`` float64
         CUDA tensors with the state's columns): KE = sum m|v|^2, PE = -2G sum m_i m_j / r or
         2k sum q_i q_j / r, so KE + PE = 2E; L and the sums weighted by the masses."""
-        sz, _ = self._batched_args(sizes, None, coupling)
-        if out.numel() < 11 * len(sz) or not out.is_contiguous():
-            raise ValueError("out must be a contiguous tensor of at least 11 * len(sizes) float64")
-        ptr = lambda t: None if t is None else t.data_ptr()
+        if coupling not in (_lib.COUPLING_GRAVITY, _lib.COUPLING_COULOMB):
+            raise ValueError("coupling: the weighted invariants take the gravity or Coulomb coupling")
+        sz = _sizes_fit(sizes, _state_pair(state))
+        _couplings(coupling, mass, charge, int(sz.astype(np.int64).sum()))
+        _dev(out, "out", "float64", 11 * len(sz))
+        _on_device(state=state, out=out, mass=mass, charge=charge)
+        ptr = _ptr
         check(load().bia_invariants_coupled_batched_dev(self._h, coupling, coup, len(sz), sz, state.data_ptr(),
                                                         state.shape[1], ptr(mass), ptr(charge), out.data_ptr()),
               "bia_invariants_coupled_batched_dev", self._h)
@@ -312,17 +463,25 @@ class Context:
 
     def mlp_forward_(self, rows, y):
         """rows: float32 CUDA tensor [B, 4]; y: float32 CUDA tensor [B]."""
+        _rows(rows)
+        _dev(y, "y", "float32", rows.shape[0])
+        _on_device(rows=rows, y=y)
         check(load().bia_mlp_forward_dev(self._h, rows.data_ptr(), rows.shape[0], y.data_ptr()),
               "bia_mlp_forward_dev", self._h)
         return y
 
     def mlp_grad_(self, rows, grad):
         """Full-batch gradient of mean((y - t)^2): grad is a float32 CUDA tensor of nparam + 1."""
+        _rows(rows)
+        _dev(grad, "grad", "float32", 4482)  # bia_mlp_nparam() + the loss
+        _on_device(rows=rows, grad=grad)
         check(load().bia_mlp_grad_dev(self._h, rows.data_ptr(), rows.shape[0], grad.data_ptr()),
               "bia_mlp_grad_dev", self._h)
         return grad
 
     def mlp_train_(self, rows, steps: int, batch: int = 8192, lr: float = 3e-3, seed: int = 7) -> np.ndarray:
+        _rows(rows)
+        _on_device(rows=rows)
         curve = np.zeros(steps, dtype=np.float32)
         check(load().bia_mlp_train_dev(self._h, rows.data_ptr(), rows.shape[0], steps, batch, lr, int(seed),
                                        curve.ctypes.data_as(C.c_void_p)), "bia_mlp_train_dev", self._h)
@@ -333,6 +492,9 @@ class Context:
 
     def record_samples_(self, state, eps: float, dt: float, nsamples: int, seed: int, out):
         """out: float32 CUDA tensor [nsamples, 4]."""
+        _state_pair(state)
+        _dev(out, "out", "float32", 4 * int(nsamples))
+        _on_device(state=state, out=out)
         check(load().bia_record_samples_dev(self._h, state.shape[1], eps, dt, state.data_ptr(), state.shape[1],
                                             nsamples, int(seed), out.data_ptr()), "bia_record_samples_dev", self._h)
         return out

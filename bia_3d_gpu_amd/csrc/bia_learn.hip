@@ -386,6 +386,7 @@ int bia_mlp_init(bia_ctx* ctx, unsigned long long seed) {
   for (int q = 0; q < 64; q++) w[oW3 + q] = unif() * 0.2165f;
   std::vector<float> all(4 * (kNParam + 4), 0.f);
   memcpy(all.data(), w.data(), kNParam * sizeof(float));
+  BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // no kernel in flight still reads the weights
   BIA_HIP_TRY(ctx, hipMemcpy(ctx->mlp_w.p, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice));
   ctx->mlp_init = 1;
   ctx->mlp_ready = 0;
@@ -399,6 +400,7 @@ int bia_mlp_set_weights(bia_ctx* ctx, const float* w, int n) {
   BIA_TRY(ctx, mlp_bufs(ctx));
   std::vector<float> all(4 * (kNParam + 4), 0.f);
   memcpy(all.data(), w, kNParam * sizeof(float));
+  BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // no kernel in flight still reads the weights
   BIA_HIP_TRY(ctx, hipMemcpy(ctx->mlp_w.p, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice));
   ctx->mlp_init = 1;
   ctx->mlp_ready = 0;
@@ -453,6 +455,7 @@ int bia_mlp_build_table(bia_ctx* ctx, int n) {
   BIA_LOCK(ctx);
   if (!ctx->mlp_init || n < 2 || n > 256) return BIA_ERR_ARG;
   const int rows = n * n * n;
+  BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // pair kernels in flight may read the old table
   BIA_TRY(ctx, ctx->mlp_table.ensure((size_t)rows * sizeof(float) * 5));
   float* tab = ctx->mlp_table.as<float>();
   float* grid = tab + rows;  // rows x 4 scratch after the table

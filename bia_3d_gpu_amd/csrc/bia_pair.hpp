@@ -68,6 +68,42 @@ struct LearnTable {
   int n;
 };
 
+// Tolerances of the per-pair RKF45 (the reference hard-wires RELERR = 0, ABSERR = 1e-16,
+// bia/bia-rkf.cu:5).
+struct RkfTol {
+  double rel, abs;
+};
+constexpr RkfTol kRkfTolRef{0.0, 1e-16};
+
+// Record of a pair that a solver could not finish: stopped at its iteration cap, RKF45 ended
+// before dt, or a non-finite output.  The reference's CPU loop writes such pairs to its units 88
+// and 99 (bia/bia.f90:151-154,167-170); its GPU path caps them silently.  `error` is the solver's
+// last residual: the reference solvers' best time-equation residual (dimensionless, as their
+// convergence test takes it), the Kepler drift's t(s) - dt at its last anomaly, the per-pair
+// RKF45's error ratio of its last trial step.
+struct PairRecord {
+  int sys, i, j, branch, iters;
+  unsigned flags;
+  double error;
+};
+struct PairLog {
+  PairRecord* rec;
+  unsigned long long* count;
+  int cap;
+};
+constexpr unsigned kLogMask = kItrCap | kRkfFail | kNonFinite;
+
+// one record; the count includes the records that did not fit (at most cap are written, in no
+// particular order)
+BIA_HDF void log_pair(const PairLog& L, int sys, int i, int j, int br, int it, unsigned fl, double err) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const unsigned long long k = atomicAdd(L.count, 1ull);
+#else
+  const unsigned long long k = (*L.count)++;
+#endif
+  if (k < (unsigned long long)L.cap) L.rec[k] = PairRecord{sys, i, j, br, it, fl, err};
+}
+
 // Residual history used by the reference's exact-repeat exit.  On the device it lives in
 // LDS (one column per lane); on the host it is a plain array.
 struct Hist {
@@ -200,7 +236,8 @@ BIA_HDF void hyp_finish(const HypState& S, double th_end, const double yz[4], do
 }
 
 // the reference's guess selection and best-iterate loop: :169-217
-BIA_HDF double hyp_solve_plain(const HypState& S, int itr, Hist hist, int* iters, unsigned* fl) {
+BIA_HDF double hyp_solve_plain(const HypState& S, int itr, Hist hist, int* iters, unsigned* fl,
+                               double* resid = nullptr) {
   double e_ini, e_gss, d;
   double th_ini_end = hyp_newton(S.th_ini, S.TE, S.P, &e_ini, &d);
   double th_gss_end = hyp_newton(S.th_gss, S.TE, S.P, &e_gss, &d);
@@ -239,6 +276,7 @@ BIA_HDF double hyp_solve_plain(const HypState& S, int itr, Hist hist, int* iters
   else if (!(fabs(err_stock) < kConvH)) *fl |= kItrCap;
   if (!updated) *fl |= kStockDefault;
   *iters = ir + 2;  // two set-up Newton evaluations + the loop
+  if (resid) *resid = err_stock;
   return th_stock;
 }
 
@@ -611,7 +649,7 @@ BIA_HDF float learned_log2_ratio(const LearnTable& T, const double a[3]) {
 
 template <int MODE>
 BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6], int* branch, int* iters,
-                         unsigned* fl, const LearnTable& T) {
+                         unsigned* fl, const LearnTable& T, double* resid = nullptr) {
   KepPair K;
   kep_setup(eps, dt, rel, K);
   *branch = (K.beta < 0) ? kHyperbola : (K.beta > 0) ? kEllipse : kParabola;
@@ -631,7 +669,13 @@ BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6]
   int ev = 0;
   KepG last;
   s = kep_solve(K, s, 60, &ev, &ok, &last);
-  if (!ok) *fl |= kItrCap;
+  if (!ok) {
+    *fl |= kItrCap;
+    if (resid) {
+      const KepG g = kep_g(K.beta, s);
+      *resid = K.r0 * s + K.eta * g.G2 + K.zeta * g.G3 - K.tau;
+    }
+  }
   *iters = ev;
   kep_output(K, s, last, rel, d);
 }
@@ -640,7 +684,7 @@ BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6]
 // ellipse: find_dXYUV_kernel.cu:242-364
 // ------------------------------------------------------------------------------------
 BIA_HDF void ellipse(double eps, double dt, int itr, const double yz[4], double yn[4], int* iters,
-                    unsigned* fl) {
+                    unsigned* fl, double* resid = nullptr) {
   double X = yz[0], Y = yz[1], U = yz[2], V = yz[3];
   double g2 = U * U + V * V;
   double r_ini = sqrt(X * X + Y * Y);
@@ -712,6 +756,7 @@ BIA_HDF void ellipse(double eps, double dt, int itr, const double yz[4], double 
       }
     }
     if (!conv) *fl |= kItrCap;
+    if (resid) *resid = err_stock;
     xi_end = th_stock;
   }
   double sn_end, cs_end;
@@ -745,7 +790,7 @@ BIA_HDF double par_dtdth(double th, double rmg) {
   return rmg * t;
 }
 
-BIA_HDF void parabola(double eps, double dt, const double yz[4], double yn[4], int* iters) {
+BIA_HDF void parabola(double eps, double dt, const double yz[4], double yn[4], int* iters, double* resid = nullptr) {
   double r_ini = sqrt(yz[0] * yz[0] + yz[1] * yz[1]);
   double r_dot_g = yz[0] * yz[2] + yz[1] * yz[3];
   double r_crs_g = yz[0] * yz[3] - yz[1] * yz[2];
@@ -792,6 +837,7 @@ BIA_HDF void parabola(double eps, double dt, const double yz[4], double yn[4], i
     if (i != i_min && et[i] == et[i_min]) break;
   }
   double th_end = th[i_min];
+  if (resid) *resid = et[i_min];
   double sb, cb, st, ct;
   sincos_(th_end / 2, &sb, &cb);
   double r_end = r_min / (cb * cb);
@@ -853,7 +899,7 @@ BIA_HDF void kep_fehl(double eps, double h, const double* y0, double* yn, double
 }
 
 BIA_HD void rkf_pair(double eps, double dt, const double yz[4], double yn[4], int* steps, unsigned* fl,
-                     double relerr = 0, double abserr = 1e-16) {
+                     double relerr = kRkfTolRef.rel, double abserr = kRkfTolRef.abs, double* resid = nullptr) {
   const double MINLIMIT = 1e-300, EPSMIN = 1e-15;
   const int ITEMAX = 100000;
   double yh0[4], yl0[4], yln[4], er[4], f[4], yhn[4];
@@ -872,9 +918,10 @@ BIA_HD void rkf_pair(double eps, double dt, const double yz[4], double yn[4], in
   double h = changed ? fmin(dt / 100, pow(toldy, 0.2)) : EPSMIN * dt;
   int acc = 0;
   bool done = false;
+  double erret = 0;
   for (int iter = 0; iter < ITEMAX; iter++) {
     kep_fehl<true>(eps, h, yl0, yln, er);
-    double erret = 0;
+    erret = 0;
     for (int i = 0; i < 4; i++) {
       double et = relerr * (fabs(yl0[i]) + fabs(yln[i])) / 2 + abserr;
       if (et < MINLIMIT) iter = ITEMAX;
@@ -908,6 +955,7 @@ BIA_HD void rkf_pair(double eps, double dt, const double yz[4], double yn[4], in
   for (int i = 0; i < 4; i++) yn[i] = yhn[i];
   *steps = acc;
   if (!done) *fl |= kRkfFail;
+  if (resid) *resid = erret;
 }
 
 // ------------------------------------------------------------------------------------
@@ -918,6 +966,7 @@ BIA_HD void rkf_pair(double eps, double dt, const double yz[4], double yn[4], in
 // whose address crosses a call stays in scratch memory on every path of the caller.
 struct Conic4 {
   double y[4];
+  double err;
   int iters;
   unsigned fl;
 };
@@ -927,7 +976,8 @@ BIA_HDN Conic4 ellipse_call(double eps, double dt, int itr, double y0, double y1
   Conic4 o;
   o.iters = 0;
   o.fl = 0;
-  ellipse(eps, dt, itr, yz, o.y, &o.iters, &o.fl);
+  o.err = 0;
+  ellipse(eps, dt, itr, yz, o.y, &o.iters, &o.fl, &o.err);
   return o;
 }
 
@@ -936,29 +986,32 @@ BIA_HDN Conic4 parabola_call(double eps, double dt, double y0, double y1, double
   Conic4 o;
   o.iters = 0;
   o.fl = 0;
-  parabola(eps, dt, yz, o.y, &o.iters);
+  o.err = 0;
+  parabola(eps, dt, yz, o.y, &o.iters, &o.err);
   return o;
 }
 
+// resid (may be NULL): the solver's last residual (PairRecord.error)
 template <int METHOD, int MODE>
 BIA_HDF void pair_eval(double eps, double dt, int itr, const double rel[6], double d[6], int* branch, int* iters,
-                       unsigned* fl, Hist hist, const LearnTable& T) {
+                       unsigned* fl, Hist hist, const LearnTable& T, const RkfTol& tol = kRkfTolRef,
+                       double* resid = nullptr) {
   if (METHOD == 1 && MODE != kPlain) {
-    kepler_pair<MODE>(eps, dt, rel, d, branch, iters, fl, T);
+    kepler_pair<MODE>(eps, dt, rel, d, branch, iters, fl, T, resid);
   } else {
     double exi[3], eeta[3], yz[4], yn[4];
     frame(rel, rel + 3, exi, eeta, yz);
     int it = 0;
     if (METHOD == 0) {
       *branch = kRkf;
-      rkf_pair(eps, dt, yz, yn, &it, fl);
+      rkf_pair(eps, dt, yz, yn, &it, fl, tol.rel, tol.abs, resid);
     } else {
       const double W = sqrt(yz[0] * yz[0] + yz[1] * yz[1]) * (yz[2] * yz[2] + yz[3] * yz[3]) + 2 * eps;
       if (W > 0) {
         *branch = kHyperbola;
         HypState S;
         hyp_setup(eps, dt, yz, S, fl);
-        const double th = hyp_solve_plain(S, itr, hist, &it, fl);
+        const double th = hyp_solve_plain(S, itr, hist, &it, fl, resid);
         hyp_finish(S, th, yz, yn);
       } else {
         *branch = (W == 0) ? kParabola : kEllipse;
@@ -967,6 +1020,7 @@ BIA_HDF void pair_eval(double eps, double dt, int itr, const double rel[6], doub
         for (int c = 0; c < 4; c++) yn[c] = o.y[c];
         it = o.iters;
         *fl |= o.fl;
+        if (resid) *resid = o.err;
       }
     }
     *iters = it;

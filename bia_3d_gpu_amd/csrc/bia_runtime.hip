@@ -83,7 +83,8 @@ int bia_destroy(bia_ctx* c) {
   c->leg_in.release();
   c->leg_out.release();
   c->dim_h.release();
-  bia::DevBuf* bufs[] = {&c->part, &c->state, &c->delta, &c->stat_slots, &c->inv_state, &c->inv_part, &c->inv_out,
+  bia::DevBuf* bufs[] = {&c->part, &c->state, &c->delta, &c->stat_slots, &c->log_stats, &c->inv_state, &c->inv_part,
+                         &c->inv_out,
                          &c->inv_tiles, &c->inv_sys, &c->dim_work, &c->dim_ctl,
                          &c->mlp_w, &c->mlp_ws, &c->mlp_table};
   for (auto* b : bufs) b->release();

@@ -114,10 +114,13 @@ struct bia_ctx {
   bia::Plan plan;
   bia::DevBuf plan_tiles, plan_sys, plan_chunks;
   std::map<std::vector<int>, std::pair<int, int>> tuned_grid;  // autotuner cache: (method, mode, coupling, sizes) -> (grid, occ)
-  // multi-step driver graph
+  // multi-step driver graph; plan_gen counts the rebuilds of the plan's device buffers (the
+  // graph's kernel arguments point into them, so a graph is only replayed on the generation it
+  // was captured on)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<long long> graph_key;
+  long long plan_gen = 0;
   // legacy per-row buffers
   bia::HostBuf leg_in, leg_out;  // rows of the per-particle reference entry point
   // invariance workspace: staging, partial sums, and the plan of the last list of systems
@@ -134,6 +137,14 @@ struct bia_ctx {
   int mlp_ready = 0;  // table built from the current weights
   int table_n = 0;
   long long mlp_step = 0;
+  // per-pair RKF45 tolerances (the reference's RELERR 0, ABSERR 1e-16 by default) and the
+  // caller's record buffer of pairs the solvers could not finish (bia_set_failure_log);
+  // log_stats: counters for steps that log without statistics of their own
+  double rkf_rel = 0.0, rkf_abs = 1e-16;
+  void* log_rec = nullptr;
+  unsigned long long* log_count = nullptr;
+  int log_cap = 0;
+  bia::DevBuf log_stats;
   // optional timing of the dominant (pair) kernel with HIP events on the context's stream
   // pair-kernel timing: event pairs recorded around the last kProfSlots launches (a ring), and
   // the launches recorded since profiling was switched on

@@ -23,6 +23,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstddef>
+#include <cmath>
 
 #include "bia_pair.hpp"
 #include "bia_runtime.hpp"
@@ -31,6 +33,7 @@ namespace bia {
 
 constexpr int kWave = 64;
 constexpr int kStatSlots = 4096;  // statistics slots of the pair kernel (a power of two)
+constexpr int kMaxGrid64 = (int)(0xffffffffu / kWave);  // largest grid of 64-lane workgroups
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -147,7 +150,8 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
                                                  const SysDesc* __restrict__ sys, const double* __restrict__ st,
                                                  const double* __restrict__ mass, const double* __restrict__ charge,
                                                  double coup, double dt, int itr, double* __restrict__ part,
-                                                 unsigned long long* __restrict__ stats, LearnTable T) {
+                                                 unsigned long long* __restrict__ stats, LearnTable T, RkfTol tol,
+                                                 PairLog lg) {
   __shared__ double hist_lds[kMaxItr * kWave];
   __shared__ double cst[6][kWave];  // column tile state
   __shared__ double cmq[3][kWave];  // column masses, charges and reciprocal masses (couplings only)
@@ -204,8 +208,12 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
           for (int c = 0; c < 6; c++) rel[c] = ri[c] - rj[c];
           int br = 0, it = 0;
           unsigned fl = 0;
-          pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T);
-          if (STATS) ps.add(br, it, fl);
+          double res = 0;
+          pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T, tol, &res);
+          if (STATS) {
+            ps.add(br, it, fl);
+            if (lg.count && (fl & kLogMask)) log_pair(lg, td.z, il, j0 + jj, br, it, fl, res);
+          }
         }
 #pragma unroll
         for (int c = 0; c < 6; c++) {
@@ -269,8 +277,12 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
         for (int c = 0; c < 6; c++) rel[c] = ri[c] - cst[c][jl];
         int br = 0, it = 0;
         unsigned fl = 0;
-        pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T);
-        if (STATS) ps.add_tile(br, it, fl);
+        double res = 0;
+        pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T, tol, &res);
+        if (STATS) {
+          ps.add_tile(br, it, fl);
+          if (lg.count && (fl & kLogMask)) log_pair(lg, td.z, il, j0 + jl, br, it, fl, res);
+        }
 #pragma unroll
         for (int c = 0; c < 6; c++) {
           if (COUP == kScalar) {
@@ -389,7 +401,21 @@ static std::pair<int, int> tuned_for(const bia_ctx* ctx, const StepArgs& a) {
   return it != ctx->tuned_grid.end() ? it->second : std::pair<int, int>(0, 1);
 }
 
-// build (or reuse) the plan for this list of sizes; upload system descriptors when eps changed
+// the captured multi-step graph, if any, is dropped: its kernel arguments point into buffers
+// that are about to change
+static void drop_graph(bia_ctx* ctx) {
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->graph_exec = nullptr;
+  ctx->graph = nullptr;
+  ctx->graph_key.clear();
+}
+
+// build (or reuse) the plan for this list of sizes; upload system descriptors when eps changed.
+// Uploads are synchronous copies, which are not ordered after work in flight on a non-blocking
+// stream: the context's stream is drained first, so no step still running reads a descriptor or
+// tile list that is being overwritten.  While buffers are rebuilt the plan is marked empty, so a
+// failure part-way leaves no plan that a later call could take for valid.
 static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
   Plan& P = ctx->plan;
   std::vector<int> sizes(a.sizes, a.sizes + a.nsys);
@@ -417,26 +443,39 @@ static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
     Q.ntiles = (int)tiles.size();
     Q.nchunks = (int)chunks.size();
     Q.part_doubles = (size_t)poff;
+    P = Plan();  // invalid until every upload below has succeeded
+    drop_graph(ctx);
+    ctx->plan_gen++;
+    BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     BIA_TRY(ctx, ctx->plan_tiles.ensure(tiles.size() * sizeof(int4)));
     BIA_TRY(ctx, ctx->plan_chunks.ensure(chunks.size() * sizeof(int2)));
     BIA_TRY(ctx, ctx->plan_sys.ensure(Q.hsys.size() * sizeof(SysDesc)));
     BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_tiles.p, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
     BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_chunks.p, chunks.data(), chunks.size() * sizeof(int2), hipMemcpyHostToDevice));
+    for (auto& d : Q.hsys) d.eps = __builtin_nan("");  // force the descriptor upload below
     P = Q;
-    for (auto& d : P.hsys) d.eps = __builtin_nan("");  // force the descriptor upload below
   }
+  std::vector<SysDesc> upd = P.hsys;
   bool dirty = false;
   for (int s = 0; s < a.nsys; s++) {
     const double e = a.eps ? a.eps[s] : 0.0;
-    if (!(P.hsys[s].eps == e)) {
-      P.hsys[s].eps = e;
+    if (!(upd[s].eps == e)) {
+      upd[s].eps = e;
       dirty = true;
     }
   }
-  if (dirty)
-    BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_sys.p, P.hsys.data(), P.hsys.size() * sizeof(SysDesc), hipMemcpyHostToDevice));
+  if (dirty) {
+    BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& d : P.hsys) d.eps = __builtin_nan("");  // unknown on the device until the copy succeeds
+    BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_sys.p, upd.data(), upd.size() * sizeof(SysDesc), hipMemcpyHostToDevice));
+    P.hsys = upd;
+  }
+  if (P.part_doubles * sizeof(double) > ctx->part.bytes) {
+    drop_graph(ctx);
+    ctx->plan_gen++;
+  }
   BIA_TRY(ctx, ctx->part.ensure(P.part_doubles * sizeof(double)));
-  if (a.d_stats && !ctx->stat_slots.p) {
+  if ((a.d_stats || ctx->log_count) && !ctx->stat_slots.p) {
     const int n = kStatSlots * BIA_STATS_LEN;
     BIA_TRY(ctx, ctx->stat_slots.ensure((size_t)n * sizeof(unsigned long long)));
     hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
@@ -474,17 +513,24 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   const int occ = occ_override > 0 ? occ_override : tuned_occ;
   const LearnTable T{ctx->mlp_table.as<float>(), ctx->table_n};
   int grid = grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : P.ntiles);
-  grid = std::min(grid, P.ntiles);
+  // a dispatch holds fewer than 2^32 work-items; the pair kernel is grid-stride, so a grid capped
+  // there still covers every tile (one system of more than about 741k particles)
+  grid = std::min(grid, std::min(P.ntiles, kMaxGrid64));
   const int4* tiles = ctx->plan_tiles.as<int4>();
   const SysDesc* sys = ctx->plan_sys.as<SysDesc>();
   double* part = ctx->part.as<double>();
-  const bool st = a.d_stats != nullptr;
+  // a failure log needs the statistics variant of the pair kernel, which writes the records; its
+  // counters go to the caller's array or, without one, to the context's own
+  const PairLog lg{static_cast<PairRecord*>(ctx->log_rec), ctx->log_count, ctx->log_cap};
+  unsigned long long* stats_out = a.d_stats ? a.d_stats : lg.count ? ctx->log_stats.as<unsigned long long>() : nullptr;
+  const bool st = stats_out != nullptr;
   unsigned long long* slots = ctx->stat_slots.as<unsigned long long>();
   if (st && !slots) return BIA_ERR_STATE;  // ensure_plan allocates the slots for a step with statistics
+  const RkfTol tol{ctx->rkf_rel, ctx->rkf_abs};
   BIA_TRY(ctx, prof_begin(ctx));
 #define L_OCC(M, MO, CP, S, O)                                                                                     \
   hipLaunchKernelGGL((k_tiles<M, MO, CP, S, O>), dim3(grid), dim3(kWave), 0, ctx->stream, P.ntiles, a.ld, tiles, sys, \
-                     a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, slots, T)
+                     a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, slots, T, tol, lg)
 #define L_(M, MO, CP, S)                                    \
   do {                                                      \
     if (occ == 4) L_OCC(M, MO, CP, S, 4);                   \
@@ -516,7 +562,7 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   BIA_HIP_TRY(ctx, hipGetLastError());
   BIA_TRY(ctx, prof_end(ctx));
   if (st) {
-    hipLaunchKernelGGL(k_stats_fold, dim3(BIA_STATS_LEN), dim3(256), 0, ctx->stream, slots, a.d_stats);
+    hipLaunchKernelGGL(k_stats_fold, dim3(BIA_STATS_LEN), dim3(256), 0, ctx->stream, slots, stats_out);
     BIA_HIP_TRY(ctx, hipGetLastError());
   }
   hipLaunchKernelGGL(k_reduce, dim3(P.nchunks, 6), dim3(256), 0, ctx->stream, ctx->plan_chunks.as<int2>(), sys, a.ld,
@@ -617,13 +663,16 @@ static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, 
                                 (long long)(size_t)a.d_delta, (long long)(size_t)a.d_mass,
                                 (long long)(size_t)a.d_charge, a.ld, (long long)(size_t)ctx->stream, tuned.first, tuned.second,
                                 ctx->mlp_ready, (long long)(size_t)ctx->part.p,
-                                (long long)(size_t)ctx->mlp_table.p, (long long)(size_t)a.d_stats,
-                                (long long)(size_t)ctx->stat_slots.p};
-  long long dtb, cb;
-  memcpy(&dtb, &a.dt, 8);
-  memcpy(&cb, &a.coup, 8);
-  key.push_back(dtb);
-  key.push_back(cb);
+                                (long long)(size_t)ctx->mlp_table.p, ctx->table_n, (long long)(size_t)a.d_stats,
+                                (long long)(size_t)ctx->stat_slots.p, ctx->plan_gen,
+                                (long long)(size_t)ctx->plan_tiles.p, (long long)(size_t)ctx->plan_sys.p,
+                                (long long)(size_t)ctx->plan_chunks.p, (long long)(size_t)ctx->log_rec,
+                                (long long)(size_t)ctx->log_count, ctx->log_cap};
+  for (const double v : {a.dt, a.coup, ctx->rkf_rel, ctx->rkf_abs}) {
+    long long b;
+    memcpy(&b, &v, 8);
+    key.push_back(b);
+  }
   for (int s : ctx->plan.sizes) key.push_back(s);
   if (key != ctx->graph_key || !ctx->graph_exec) {
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -652,6 +701,9 @@ static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, 
   for (int k = 0; k < nsteps; k++) BIA_HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
   return BIA_OK;
 }
+
+static_assert(sizeof(PairRecord) == sizeof(bia_pair_record) && offsetof(PairRecord, error) ==
+                  offsetof(bia_pair_record, error), "the device record is the public one");
 
 }  // namespace bia
 
@@ -696,6 +748,31 @@ int bia_run_steps_dev(bia_ctx* ctx, int method, int mode, int coupling, int nsys
   if (nsteps < 0) return BIA_ERR_ARG;
   StepArgs a{method, mode, coupling, nsys, sizes, eps, coup, dt, itr, d_state, ld, d_mass, d_charge, d_delta, d_stats};
   return run_steps_locked(ctx, a, d_state, nsteps, use_graph);
+}
+
+int bia_set_rkf_tolerance(bia_ctx* ctx, double relerr, double abserr) {
+  BIA_LOCKED(ctx);
+  // the error test divides by relerr |y| + abserr: both non-negative and finite, not both zero
+  if (!(relerr >= 0 && abserr >= 0 && relerr + abserr > 0) || !std::isfinite(relerr + abserr)) return BIA_ERR_ARG;
+  ctx->rkf_rel = relerr;
+  ctx->rkf_abs = abserr;
+  return BIA_OK;
+}
+
+int bia_set_failure_log(bia_ctx* ctx, bia_pair_record* d_records, int capacity, unsigned long long* d_count) {
+  BIA_LOCKED(ctx);
+  if (!d_count) {  // logging off
+    ctx->log_rec = nullptr;
+    ctx->log_count = nullptr;
+    ctx->log_cap = 0;
+    return BIA_OK;
+  }
+  if (capacity < 0 || (capacity > 0 && !d_records)) return BIA_ERR_ARG;
+  BIA_TRY(ctx, ctx->log_stats.ensure(BIA_STATS_LEN * sizeof(unsigned long long)));
+  ctx->log_rec = d_records;
+  ctx->log_count = d_count;
+  ctx->log_cap = capacity;
+  return BIA_OK;
 }
 
 int bia_step_host(bia_ctx* ctx, int method, int mode, int n, double eps, double dt, int itr, const double* X,

@@ -75,14 +75,12 @@ __device__ __forceinline__ int sys_of_tile(const InvSys* __restrict__ sys, int n
   return lo;
 }
 
-// potential tiles: part[t] = sum over the tile's pairs of 1/r, or of pw_i pw_j / r with W
+// potential tile t: part[t] = sum over the tile's pairs of 1/r, or of pw_i pw_j / r with W
 // (double-double)
 template <bool W>
-__global__ __launch_bounds__(64) void k_inv_pe_tiles(int nsys, const InvSys* __restrict__ sys, int ld,
-                                                     const double* __restrict__ state, const double* __restrict__ pw,
-                                                     DD* __restrict__ part) {
-  const int lane = threadIdx.x;
-  const long long t = blockIdx.x;
+__device__ __forceinline__ void pe_tile(long long t, int lane, int nsys, const InvSys* __restrict__ sys, int ld,
+                                        const double* __restrict__ state, const double* __restrict__ pw,
+                                        DD* __restrict__ part) {
   const InvSys S = sys[sys_of_tile(sys, nsys, t)];
   const int2 IJ = tri_tile(t - S.tile0, S.nt);
   const int I = IJ.x, J = IJ.y, n = S.n;
@@ -107,6 +105,16 @@ __global__ __launch_bounds__(64) void k_inv_pe_tiles(int nsys, const InvSys* __r
   }
   s = dd_wave_sum(s);
   if (lane == 0) part[t] = s;
+}
+
+// all potential tiles of a list of systems, one wave per tile, grid-stride (the grid is capped
+// below 2^32 work-items)
+template <bool W>
+__global__ __launch_bounds__(64) void k_inv_pe_tiles(int nsys, long long ntiles, const InvSys* __restrict__ sys,
+                                                     int ld, const double* __restrict__ state,
+                                                     const double* __restrict__ pw, DD* __restrict__ part) {
+  const int lane = threadIdx.x;
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) pe_tile<W>(t, lane, nsys, sys, ld, state, pw, part);
 }
 
 // ten linear sums per block (Xa Ya Za Ua Va Wa Lx Ly Lz KE), each term weighted by kw_i with W;
@@ -246,8 +254,11 @@ static int ensure_inv_plan(bia_ctx* ctx, int nsys, const int* sizes, const doubl
     maxnpe = std::max(maxnpe, S.npe);
     maxnlin = std::max(maxnlin, S.nlin);
   }
-  if (tile0 > INT_MAX) return BIA_ERR_ARG;  // one potential tile per workgroup of a 1-D grid
+  if (tile0 > INT_MAX) return BIA_ERR_ARG;  // tile counts are held in int (about 4.2 million particles)
   ctx->inv_key.clear();  // invalid until the upload below succeeds
+  // the copy is synchronous and not ordered after kernels still in flight on a non-blocking
+  // stream, which read these descriptors: drain the context's stream first
+  BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   BIA_TRY(ctx, ctx->inv_sys.ensure(h.size() * sizeof(InvSys)));
   BIA_HIP_TRY(ctx, hipMemcpy(ctx->inv_sys.p, h.data(), h.size() * sizeof(InvSys), hipMemcpyHostToDevice));
   ctx->inv_ntiles = (int)tile0;
@@ -278,12 +289,14 @@ static int invariants_locked(bia_ctx* ctx, int nsys, const int* sizes, const dou
   DD* pe = ctx->inv_part.as<DD>();
   DD* lin = pe + ctx->inv_ntiles;
   DD* pe2 = lin + (size_t)ctx->inv_nlin * 10;
+  const long long nt = ctx->inv_ntiles;
+  const unsigned grid = (unsigned)std::min<long long>(nt, 0xffffffffLL / 64);
   if (pw)
-    hipLaunchKernelGGL(k_inv_pe_tiles<true>, dim3((unsigned)ctx->inv_ntiles), dim3(64), 0, ctx->stream, nsys, sys, ld,
-                       d_state, pw, pe);
+    hipLaunchKernelGGL(k_inv_pe_tiles<true>, dim3(grid), dim3(64), 0, ctx->stream, nsys, nt, sys, ld, d_state, pw,
+                       pe);
   else
-    hipLaunchKernelGGL(k_inv_pe_tiles<false>, dim3((unsigned)ctx->inv_ntiles), dim3(64), 0, ctx->stream, nsys, sys,
-                       ld, d_state, nullptr, pe);
+    hipLaunchKernelGGL(k_inv_pe_tiles<false>, dim3(grid), dim3(64), 0, ctx->stream, nsys, nt, sys, ld, d_state,
+                       nullptr, pe);
   BIA_HIP_TRY(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_inv_pe_reduce, dim3(ctx->inv_maxnpe, nsys), dim3(256), 0, ctx->stream, sys, pe, pe2);
   BIA_HIP_TRY(ctx, hipGetLastError());

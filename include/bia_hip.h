@@ -158,6 +158,33 @@ int bia_run_steps_dev(bia_ctx *ctx, int method, int mode, int coupling, int nsys
                       const double *d_charge, double *d_delta, int nsteps, int use_graph,
                       unsigned long long *d_stats);
 
+/* Tolerances of the per-pair RKF45 (BIA_METHOD_RKF) for the context's subsequent steps: the error
+ * test of a trial step is |err_k| <= relerr (|y_k| + |y_k'|)/2 + abserr per component, as in
+ * bia/bia-rkf.cu:72-79, where RELERR = 0 and ABSERR = 1e-16 are compile-time constants (bia-rkf.cu:5;
+ * they stay the defaults).  Both >= 0, finite, not both 0; else BIA_ERR_ARG.  The per-particle
+ * reference symbol find_dxyuv_kernel_ keeps the reference's values. */
+int bia_set_rkf_tolerance(bia_ctx *ctx, double relerr, double abserr);
+
+/* Per-pair failure records.  The reference's CPU pair loop writes every pair whose solver did not
+ * converge to its units 88 (RKF) and 99 (ALG) with (realisation, i, j, IER, iterations, error)
+ * (bia/bia.f90:151-154,167-170); its GPU path caps such pairs silently.  With a log set, every
+ * subsequent step of the context writes one bia_pair_record per pair that stopped at its
+ * iteration cap, whose RKF45 ended before dt, or whose output is not finite, into d_records
+ * (device, capacity records) and adds one to *d_count (device, the caller zeroes it) per such
+ * pair, including those beyond the capacity, which are not written.  Records are in no particular
+ * order (sort by sys, i, j).  Such steps run the pair kernel's statistics variant.  d_count NULL
+ * switches logging off.  The buffers are the caller's and must outlive the steps that use them. */
+typedef struct {
+  int sys;        /* system of the call's list */
+  int i, j;       /* the pair, 0-based within the system, i < j */
+  int branch;     /* 0 hyperbola, 1 parabola, 2 ellipse, 3 RKF45 (Kepler drift: sign of the energy) */
+  int iters;      /* evaluations / accepted RKF steps */
+  unsigned flags; /* status bits of the statistics (4 iteration cap, 16 RKF failure, 32 non-finite, ...) */
+  double error;   /* last residual: the reference solvers' best residual of their dimensionless time
+                     equation; the Kepler drift's t(s) - dt; the RKF45's error ratio of its last trial */
+} bia_pair_record;
+int bia_set_failure_log(bia_ctx *ctx, bia_pair_record *d_records, int capacity, unsigned long long *d_count);
+
 /* Host-array variant of bia_step_dev for Fortran callers (bind(C) interface in
  * bia_3d_gpu_amd/fortran/bia_hip.f90).  dX..dW are overwritten; stats (may be NULL) is
  * BIA_STATS_LEN counters for this call. */

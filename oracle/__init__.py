@@ -56,8 +56,58 @@ def lib():
         L.orc_conic_indeterminate.argtypes = [C.c_double, _dp]
         L.orc_sensitivity.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, C.c_void_p,
                                       C.c_void_p, C.c_int, _dp, _dp]
+        L.orc_rkf_pair_tol.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_pair_quad.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_rows_quad.argtypes = [C.c_int, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp, _dp, _dp]
+        L.orc_rows_rkf_tol.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       _dp, C.c_void_p, C.c_void_p, C.c_int, _ip, _dp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
+
+
+def rkf2d_tol(eps: float, dt: float, yz, relerr: float, abserr: float):
+    """The per-pair RKF45 of one 2D pair at the given tolerances; returns (yn, steps, flags)."""
+    yz = np.ascontiguousarray(yz, dtype=np.float64)
+    yn = np.zeros(4)
+    it, fl = C.c_int(0), C.c_int(0)
+    lib().orc_rkf_pair_tol(eps, dt, relerr, abserr, yz, yn, C.byref(it), C.byref(fl))
+    return yn, it.value, fl.value
+
+
+def rows_rkf_tol(state_soa, rows, dt: float, relerr: float, abserr: float, eps: float = 0.0, coupling: int = 0,
+                 coup: float = 1.0, mass=None, charge=None):
+    """Deltas [6, len(rows)] of whole rows of the given particles with the per-pair RKF45 at the
+    given tolerances (couplings as bia_step_coupled); returns (delta, pairs that failed)."""
+    st = np.ascontiguousarray(state_soa, dtype=np.float64).reshape(6, -1)
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    out = np.zeros((6, r.size))
+    m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+    q = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64)
+    nf = C.c_int(0)
+    lib().orc_rows_rkf_tol(st.shape[1], coupling, coup, eps, dt, relerr, abserr, st,
+                           None if m is None else m.ctypes.data_as(C.c_void_p),
+                           None if q is None else q.ctypes.data_as(C.c_void_p), r.size, r, out, C.byref(nf))
+    return out, nf.value
+
+
+def pair_quad(eps: float, dt: float, rel):
+    """One pair's half deltas from the two-body solution in quadruple precision, as (hi, lo)
+    doubles with hi + lo the quad result."""
+    hi, lo = np.zeros(6), np.zeros(6)
+    lib().orc_pair_quad(eps, dt, np.ascontiguousarray(rel, dtype=np.float64), hi, lo)
+    return hi, lo
+
+
+def rows_quad(state_soa, rows, eps: float, dt: float):
+    """Whole rows of the given particles (scalar coupling) in quadruple precision: returns
+    (corr_hi, corr_lo, delta), each [6, len(rows)]; corr = the sum of the particle's pair half
+    deltas as hi + lo, delta = corr + v dt rounded once."""
+    st = np.ascontiguousarray(state_soa, dtype=np.float64).reshape(6, -1)
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    hi, lo, d = (np.zeros((6, r.size)) for _ in range(3))
+    lib().orc_rows_quad(st.shape[1], eps, dt, st, r.size, r, hi, lo, d)
+    return hi, lo, d
 
 
 def ref_available() -> bool:

@@ -387,9 +387,11 @@ static void kep_fhstep(double eps, double h, double *y0, double *yn) { /* FHSTEP
   kep_fehl(0, eps, h1, y0, yn, NULL);
 }
 
-void orc_rkf_pair(double eps, double dt, const double YZ[4], double YN[4], int *nsteps, int *flags) {
-  /* rkf_device :47-103; RELERR = 0, ABSERR = 1e-16, ITEMAX = 100000 (:3-5) */
-  const double RELERR = 0, ABSERR = 1e-16, MINLIMIT = 1e-300, EPSMIN = 1e-15;
+/* rkf_device :47-103 with the error test's tolerances as arguments (the reference fixes them at
+ * RELERR = 0, ABSERR = 1e-16, :5; orc_rkf_pair below) */
+void orc_rkf_pair_tol(double eps, double dt, double RELERR, double ABSERR, const double YZ[4], double YN[4],
+                      int *nsteps, int *flags) {
+  const double MINLIMIT = 1e-300, EPSMIN = 1e-15;
   const int ITEMAX = 100000;
   double yh0[4] = {YZ[0], YZ[1], YZ[2], YZ[3]}, yl0[4], yln[4], er[4], f[4], yhn[4];
   int i, steps = 0, done = 0;
@@ -440,6 +442,11 @@ void orc_rkf_pair(double eps, double dt, const double YZ[4], double YN[4], int *
   for (i = 0; i < 4; i++) YN[i] = yhn[i];
   if (nsteps) *nsteps = steps;
   if (flags) *flags = done ? 0 : ORC_F_RKF_FAIL;
+}
+
+void orc_rkf_pair(double eps, double dt, const double YZ[4], double YN[4], int *nsteps, int *flags) {
+  /* rkf_device :47-103; RELERR = 0, ABSERR = 1e-16, ITEMAX = 100000 (:3-5) */
+  orc_rkf_pair_tol(eps, dt, 0.0, 1e-16, YZ, YN, nsteps, flags);
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -894,4 +901,209 @@ void orc_bia_rows_coupled(int method, int n, int coupling, double coup, double d
     for (int c = 0; c < 3; c++) delta[(size_t)c * n + i] += st[(size_t)(3 + c) * n + i] * dt;
   }
   free(row);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Two-body step in quadruple precision (no reference counterpart): the accuracy oracle of  */
+/* the pair propagators at 1e-12 and below.  Restated from the textbook two-body solution   */
+/* in universal variables (Battin, "An Introduction to the Mathematics and Methods of      */
+/* Astrodynamics", 1999, sec. 4.5; Danby, "Fundamentals of Celestial Mechanics", 1988,      */
+/* sec. 6.9), not from this library's device code: the Stumpff functions in closed form     */
+/* (cos/sin for bound, cosh/sinh for unbound orbits) or, for |z| < 0.1, their defining      */
+/* series summed to convergence; the time equation t(s) = dt solved by bisection to a       */
+/* relative bracket of 2^-20, then by Newton steps kept inside the bracket.                */
+/*   r'' = eps r/|r|^3, mu = -eps, beta = 2 mu / r0 - v0^2, eta = r0.v0,                    */
+/*   zeta = mu - beta r0, G_k(s) = s^k c_k(beta s^2),                                       */
+/*   t(s) = r0 s + eta G2 + zeta G3, r(s) = r0 + eta G1 + zeta G2,                          */
+/*   f - 1 = -mu G2 / r0, g - t = -mu G3, f' = -mu G1 / (r0 r), g' - 1 = -mu G2 / r;        */
+/* the pair's half deltas are ((f-1) r0 + (g-dt) v0)/2 and (f' r0 + (g'-1) v0)/2, the      */
+/* quantities of the equal-mass half split (find_dXYUV_kernel.cu:507-515).                  */
+/* ------------------------------------------------------------------------------------ */
+typedef __float128 q128;
+
+/* 1 / ((m + 1)(m + 2)) for m = 0..kQTerms+3: the ratio of consecutive terms of the series */
+enum { kQTerms = 40 };
+static q128 q_ratio[kQTerms + 4];
+static void q_init(void) {
+  static int done = 0;
+#pragma omp critical(q_init)
+  if (!done) {
+    for (int m = 0; m < kQTerms + 4; m++) q_ratio[m] = 1 / ((q128)(m + 1) * (q128)(m + 2));
+    done = 1;
+  }
+}
+
+static void q_stumpff(q128 z, q128 *c1, q128 *c2, q128 *c3) {
+  if (fabsq(z) < 0.1Q) {
+    /* c_k(z) = sum_j (-z)^j / (2j + k)!; with |z| < 0.1 the terms fall by 10 or more each, so
+     * kQTerms terms leave less than 1e-40 */
+    q128 t1 = 1, t2 = 0.5Q, t3 = 1 / 6.0Q, s1 = 0, s2 = 0, s3 = 0;
+    for (int j = 0; j < kQTerms; j++) {
+      s1 += t1;
+      s2 += t2;
+      s3 += t3;
+      if (fabsq(t1) < 1e-40Q) break;
+      t1 *= -z * q_ratio[2 * j + 1];  /* 1 / ((2j + 2)(2j + 3)) */
+      t2 *= -z * q_ratio[2 * j + 2];
+      t3 *= -z * q_ratio[2 * j + 3];
+    }
+    *c1 = s1;
+    *c2 = s2;
+    *c3 = s3;
+  } else if (z > 0) {
+    const q128 q = sqrtq(z), sn = sinq(q), cs = cosq(q);
+    *c1 = sn / q;
+    *c2 = (1 - cs) / z;
+    *c3 = (q - sn) / (q * q * q);
+  } else {
+    const q128 q = sqrtq(-z), sh = sinhq(q), ch = coshq(q);
+    *c1 = sh / q;
+    *c2 = (ch - 1) / -z;
+    *c3 = (sh - q) / (q * q * q);
+  }
+}
+
+/* half deltas of one pair over dt > 0, in quad */
+static void q_pair(q128 eps, q128 dt, const q128 r[3], const q128 v[3], q128 d[6]) {
+  q_init();
+  const q128 mu = -eps;
+  const q128 R0 = sqrtq(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  const q128 eta = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
+  const q128 beta = 2 * mu / R0 - (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const q128 zeta = mu - beta * R0;
+  q128 c1, c2, c3;
+#define Q_T(s_, f_, fp_)                                       \
+  do {                                                         \
+    q_stumpff(beta * (s_) * (s_), &c1, &c2, &c3);              \
+    const q128 g1_ = (s_) * c1, g2_ = (s_) * (s_) * c2;        \
+    (f_) = R0 * (s_) + eta * g2_ + zeta * (s_) * (s_) * (s_) * c3 - dt; \
+    (fp_) = R0 + eta * g1_ + zeta * g2_;                       \
+  } while (0)
+  q128 lo = 0, hi = dt / R0, f, fp;
+  Q_T(hi, f, fp);
+  for (int k = 0; k < 4000 && f < 0; k++) {
+    lo = hi;
+    hi *= 2;
+    Q_T(hi, f, fp);
+  }
+  while (hi - lo > 0x1p-20Q * hi) {  /* t increases with s (t' = r > 0) */
+    const q128 mid = 0.5Q * (lo + hi);
+    Q_T(mid, f, fp);
+    if (f < 0) lo = mid;
+    else hi = mid;
+  }
+  q128 s = 0.5Q * (lo + hi);
+  for (int k = 0; k < 60; k++) {
+    Q_T(s, f, fp);
+    if (f == 0) break;
+    if (f < 0) lo = s;
+    else hi = s;
+    q128 sn = s - f / fp;
+    if (!(sn > lo && sn < hi)) sn = 0.5Q * (lo + hi);
+    const q128 step = fabsq(sn - s);
+    s = sn;
+    if (step <= 1e-33Q * fabsq(s)) break;
+  }
+#undef Q_T
+  q_stumpff(beta * s * s, &c1, &c2, &c3);
+  const q128 G1 = s * c1, G2 = s * s * c2, G3 = s * s * s * c3;
+  const q128 rn = R0 + eta * G1 + zeta * G2;
+  const q128 fm1 = -mu * G2 / R0, gmdt = -mu * G3;
+  const q128 fd = -mu * G1 / (R0 * rn), gdm1 = -mu * G2 / rn;
+  for (int c = 0; c < 3; c++) {
+    d[c] = 0.5Q * (fm1 * r[c] + gmdt * v[c]);
+    d[3 + c] = 0.5Q * (fd * r[c] + gdm1 * v[c]);
+  }
+}
+
+/* one pair: rel (double) in, half deltas out as double-double hi + lo */
+void orc_pair_quad(double eps, double dt, const double rel[6], double hi[6], double lo[6]) {
+  q128 r[3], v[3], d[6];
+  for (int c = 0; c < 3; c++) {
+    r[c] = rel[c];
+    v[c] = rel[3 + c];
+  }
+  q_pair(eps, dt, r, v, d);
+  for (int c = 0; c < 6; c++) {
+    hi[c] = (double)d[c];
+    lo[c] = (double)(d[c] - (q128)hi[c]);
+  }
+}
+
+/* Whole rows of selected particles of an n-particle system (SoA st[6][n]) with the scalar
+ * coupling: particle p receives + the half deltas of pair (p, j) for j > p and - those of
+ * pair (j, p) for j < p (each pair oriented i < j, rel = state_i - state_j, as the step takes
+ * it), summed in quad.  corr_hi/lo [6][nrows]: the sums (the Kepler correction, no drift);
+ * delta [6][nrows]: corr + v dt for positions, rounded once to double. */
+void orc_rows_quad(int n, double eps, double dt, const double *st, int nrows, const int *rows, double *corr_hi,
+                   double *corr_lo, double *delta) {
+  for (int k = 0; k < nrows; k++) {
+    const int p = rows[k];
+    q128 acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma omp parallel
+    {
+      q128 part[6] = {0, 0, 0, 0, 0, 0};
+#pragma omp for schedule(dynamic, 256)
+      for (int j = 0; j < n; j++) {
+        if (j == p) continue;
+        const int i0 = p < j ? p : j, j0 = p < j ? j : p;
+        q128 r[3], v[3], d[6];
+        for (int c = 0; c < 3; c++) {
+          r[c] = (q128)(st[(size_t)c * n + i0] - st[(size_t)c * n + j0]);
+          v[c] = (q128)(st[(size_t)(3 + c) * n + i0] - st[(size_t)(3 + c) * n + j0]);
+        }
+        q_pair(eps, dt, r, v, d);
+        for (int c = 0; c < 6; c++) part[c] += (p < j) ? d[c] : -d[c];
+      }
+#pragma omp critical
+      for (int c = 0; c < 6; c++) acc[c] += part[c];
+    }
+    for (int c = 0; c < 6; c++) {
+      corr_hi[(size_t)c * nrows + k] = (double)acc[c];
+      corr_lo[(size_t)c * nrows + k] = (double)(acc[c] - (q128)(double)acc[c]);
+      const q128 drift = c < 3 ? (q128)st[(size_t)(3 + c) * n + p] * (q128)dt : 0;
+      delta[(size_t)c * nrows + k] = (double)(acc[c] + drift);
+    }
+  }
+}
+
+/* Whole rows of selected particles with the per-pair RKF45 at the given tolerances and the
+ * couplings of orc_bia_rows_coupled (0 scalar with eps, 1 gravity, 2 Coulomb, coup = G or k):
+ * particle p receives + w_i d for pairs (p, j > p) and - w_j d for pairs (j < p, p), each pair
+ * oriented i < j, plus its drift; delta [6][nrows]. */
+void orc_rows_rkf_tol(int n, int coupling, double coup, double eps, double dt, double relerr, double abserr,
+                      const double *st, const double *mass, const double *charge, int nrows, const int *rows,
+                      double *delta, int *failed) {
+  int nf = 0;
+  for (int k = 0; k < nrows; k++) {
+    const int p = rows[k];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < n; j++) {
+      if (j == p) continue;
+      const int i0 = p < j ? p : j, j0 = p < j ? j : p;
+      double rel[6], exi[3], eeta[3], YZ[4], YN[4];
+      for (int c = 0; c < 6; c++) rel[c] = st[(size_t)c * n + i0] - st[(size_t)c * n + j0];
+      double e = eps, wi = 1, wj = 1;
+      if (coupling != 0) {
+        const double mi = mass ? mass[i0] : 1.0, mj = mass ? mass[j0] : 1.0;
+        e = (coupling == 1) ? -coup * (mi + mj) : coup * charge[i0] * charge[j0] * (1 / mi + 1 / mj);
+        wi = 2 * mj / (mi + mj);
+        wj = 2 * mi / (mi + mj);
+      }
+      int it = 0, fl = 0;
+      orc_frame(rel, rel + 3, exi, eeta, YZ);
+      orc_rkf_pair_tol(e, dt, relerr, abserr, YZ, YN, &it, &fl);
+      if (fl) nf++;
+      const double drx = YN[0] - YZ[0], dre = YN[1] - YZ[1], dgx = YN[2] - YZ[2], dge = YN[3] - YZ[3];
+      for (int c = 0; c < 3; c++) {
+        const double dp = (drx * exi[c] + dre * eeta[c] - rel[3 + c] * dt) / 2;
+        const double dv = (dgx * exi[c] + dge * eeta[c]) / 2;
+        acc[c] += p < j ? wi * dp : -wj * dp;
+        acc[3 + c] += p < j ? wi * dv : -wj * dv;
+      }
+    }
+    for (int c = 0; c < 6; c++)
+      delta[(size_t)c * nrows + k] = acc[c] + (c < 3 ? st[(size_t)(3 + c) * n + p] * dt : 0.0);
+  }
+  if (failed) *failed = nf;
 }

@@ -34,6 +34,8 @@ void orc_hyperbola(double eps, double dt, int itr, const double YZ[4], double YN
 void orc_ellipse(double eps, double dt, int itr, const double YZ[4], double YN[4], int *iters, int *flags);
 void orc_parabola(double eps, double dt, const double YZ[4], double YN[4], int *iters, int *flags);
 void orc_rkf_pair(double eps, double dt, const double YZ[4], double YN[4], int *nsteps, int *flags);
+void orc_rkf_pair_tol(double eps, double dt, double relerr, double abserr, const double YZ[4], double YN[4],
+                      int *nsteps, int *flags);
 
 /* one pair: rel = (r_i - r_j, g_i - g_j); out = the six half-deltas credited to i */
 void orc_pair(int method, double eps, double dt, int itr, const double rel[6], double out[6],
@@ -69,6 +71,19 @@ void orc_bia_rows_coupled(int method, int n, int coupling, double coup, double d
 int orc_conic_indeterminate(double eps, const double rel[6]);
 void orc_sensitivity(int method, int n, int coupling, double eps, double dt, int itr, const double *st,
                      const double *mass, const double *charge, int samples, double *bpos, double *bvel);
+
+/* two-body step in quadruple precision (closed-form Stumpff functions, bisection + Newton):
+ * one pair's half deltas as hi + lo, and whole rows of selected particles of a system with the
+ * scalar coupling (corr = sum of pair half deltas, delta = corr + drift), all [6][nrows] */
+void orc_pair_quad(double eps, double dt, const double rel[6], double hi[6], double lo[6]);
+void orc_rows_quad(int n, double eps, double dt, const double *st, int nrows, const int *rows, double *corr_hi,
+                   double *corr_lo, double *delta);
+
+/* per-pair RKF45 with the error test's tolerances as arguments, and whole rows of selected
+ * particles with it (couplings as orc_bia_rows_coupled, eps for the scalar one) */
+void orc_rows_rkf_tol(int n, int coupling, double coup, double eps, double dt, double relerr, double abserr,
+                      const double *st, const double *mass, const double *charge, int nrows, const int *rows,
+                      double *delta, int *failed);
 
 /* one direct-sum derivative evaluation f = (v, a) on an AoS state */
 void orc_dim_func(int np, double eps, const double *y, double *f);

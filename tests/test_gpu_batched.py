@@ -101,6 +101,28 @@ def test_graph_driver_matches_eager_steps(ctx, mode):
     assert torch.equal(a, b)
 
 
+def test_graph_replay_after_plan_rebuild(ctx):
+    """Graphs on size list A, then B (more systems, fewer pairs: the plan's descriptor and chunk
+    buffers are reallocated), then A again on the same tensors: the graph captured on A reads
+    the plan buffers through pointers, so it must be captured anew, not replayed.  Checked
+    bitwise against eager steps on another context."""
+    import torch
+    from bia_3d_gpu_amd import Context, ic
+    size_lists = {"A": [1500], "B": [64] * 40}
+    runs = {}
+    for key, sizes in size_lists.items():
+        st = _cat([ic.plummer_system(n, seed=70 + k) for k, n in enumerate(sizes)])
+        g = torch.from_numpy(st).cuda()
+        runs[key] = (sizes, [-2.0 / n for n in sizes], g, g.clone(), torch.empty_like(g), torch.empty_like(g))
+    with Context(0) as c:
+        for key in ("A", "B", "A", "B"):
+            sizes, eps, g, e, dg, de = runs[key]
+            c.run_steps_(g, dg, sizes, 2, eps=eps, dt=1e-3, mode=2, use_graph=True)
+            ctx.run_steps_(e, de, sizes, 2, eps=eps, dt=1e-3, mode=2, use_graph=False)
+            torch.cuda.synchronize()
+            assert torch.equal(g, e), key
+
+
 @pytest.mark.parametrize("mode", [0, 2])
 def test_autotuner_keeps_results(ctx, mode):
     import torch

@@ -105,27 +105,56 @@ def test_plummer_ensemble_512_for_200_steps(ctx):
     assert np.median(de[-1]) < 1e-7 and de.max() < 1e-4
 
 
-def test_benchmark_system_65536_against_rkf45(ctx):
+def _closest_pairs(st, k):
+    """The k closest pairs of a system (its hardest binaries), as index pairs."""
+    from scipy.spatial import cKDTree
+    dist, idx = cKDTree(st[:3].T).query(st[:3].T, k=2)
+    order = np.argsort(dist[:, 1])
+    out, seen = [], set()
+    for i in order:
+        j = int(idx[i, 1])
+        if i not in seen and j not in seen:
+            out.append((int(i), j))
+            seen.update((int(i), j))
+        if len(out) == k:
+            break
+    return out
+
+
+def test_benchmark_system_65536_against_rkf45_and_quad(ctx):
     """The benchmark system itself (Plummer N = 65536 with 20 % of the particles in hard binaries,
-    eps = -2/N, dt = 1e-3; 2.1e9 pairs), one step: the Kepler drift against the per-pair RKF45 of
-    the same pairs (the reference's FEHL integrator and step control, an independent solution).
-    No particle's delta is off by more than 1e-6 of the largest delta of its component, and
-    positions agree normwise to 1e-11 — the floor of the RKF45 here: it advances the whole relative
-    state and subtracts the straight-line motion, so each weak distant pair (|r| ~ 9, half delta
-    ~ 2e-10) carries ~1.5e-15 of rounding, which 65535 pairs add up in a particle's position delta;
-    the Kepler drift forms that departure directly and is within 1e-26 of the 40-digit solution
-    on those pairs.  The reference solvers (plain mode) are reported beside it: they stop at their
-    iteration cap on some binary pairs and are off there."""
+    eps = -2/N, dt = 1e-3; 2.1e9 pairs), one step in every mode.
+
+    1. Against the per-pair RKF45 of the same pairs (the reference's FEHL integrator and step
+       control, an independent solution) on every particle: no particle's delta is off by more
+       than 1e-6 of the largest delta of its component; positions agree normwise to 1e-11, the
+       RKF45's own floor here (it advances the whole relative state and subtracts the straight-line
+       motion, so each weak distant pair carries ~1.5e-15 of rounding, which 65535 pairs add up).
+    2. Against the two-body solution in quadruple precision (oracle orc_rows_quad) on 16 whole
+       rows: both members of the three hardest binaries, the particles where the Kepler drift and
+       the reference solvers disagree most with RKF45, the first and the last particle and two
+       more.  The Kepler drift, with and without the learned start, is within 1e-12 normwise in
+       positions and velocities (max |x - x_quad| / max |x_quad| over the system, SURVEY's
+       definition) and within 1e-11 of each row's Kepler correction (the sum of its pair deltas,
+       the part of the delta that is not drift).  The reference solvers (plain mode) are
+       reported: they stop at their iteration cap on some binary pairs and are off there."""
     import torch
-    from bia_3d_gpu_amd import METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_PLAIN, STATS_LEN, ic
+    from bia_3d_gpu_amd import METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, STATS_LEN, ic, learn
+    import oracle as O
     n = 65536
     st = ic.plummer_with_binaries(n, 65536)
     eps, dt = -2.0 / n, 1e-3
     s = torch.from_numpy(st).cuda()
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    # a network for the learned mode, trained on the recorder's usual systems (small run)
+    raw = learn.record(ctx, learn.training_systems(29), per_system=100_000, seed=29)
+    rows_t = learn.balance(raw, 500_000, seed=29)
+    ctx.mlp_init(29)
+    ctx.mlp_train_(rows_t, steps=600, batch=8192, lr=3e-3, seed=29)
+    ctx.mlp_build_table(learn.TABLE_N)
     out, stats = {}, {}
-    for name, method, mode in (("kepler", METHOD_ALG, MODE_GUESS), ("plain", METHOD_ALG, MODE_PLAIN),
-                               ("rkf", METHOD_RKF, MODE_PLAIN)):
+    for name, method, mode in (("guess", METHOD_ALG, MODE_GUESS), ("learned", METHOD_ALG, MODE_LEARN),
+                               ("plain", METHOD_ALG, MODE_PLAIN), ("rkf", METHOD_RKF, MODE_PLAIN)):
         d = torch.empty_like(s)
         c = torch.zeros(STATS_LEN, dtype=torch.int64, device="cuda")
         ctx.step_(s, d, eps, dt, method=method, mode=mode, stats=c)
@@ -134,17 +163,122 @@ def test_benchmark_system_65536_against_rkf45(ctx):
     ref = out["rkf"]
     assert stats["rkf"][12] == 0  # no RKF failure on this system
     scale = np.abs(ref).max(axis=1, keepdims=True)
-    res = {}
-    for name in ("kepler", "plain"):
-        off = (np.abs(out[name] - ref) / scale).max(axis=0)
+    res, off = {}, {}
+    for name in ("guess", "learned", "plain"):
+        off[name] = (np.abs(out[name] - ref) / scale).max(axis=0)
         a, b = st + out[name], st + ref
-        res[name] = (int((off > 1e-6).sum()), int((off > 1e-8).sum()),
+        res[name] = (int((off[name] > 1e-6).sum()), int((off[name] > 1e-8).sum()),
                      np.abs(a[:3] - b[:3]).max() / np.abs(b[:3]).max(), np.abs(a[3:] - b[3:]).max() / np.abs(b[3:]).max())
         print(f"{name} vs RKF45: particles off by more than 1e-6 / 1e-8 of the largest delta {res[name][0]} / "
               f"{res[name][1]}; normwise positions {res[name][2]:.2e}, velocities {res[name][3]:.2e}; "
-              f"iteration caps {stats[name][10]}")
-    assert res["kepler"][0] == 0 and res["kepler"][2] < 1e-11 and res["kepler"][3] < 1e-12
-    assert stats["kepler"][10] == 0 and stats["kepler"][13] == 0
+              f"iteration caps {stats[name][10]}, learned starts {stats[name][14]}")
+    for name in ("guess", "learned"):
+        assert res[name][0] == 0 and res[name][2] < 1e-11 and res[name][3] < 1e-12
+        assert stats[name][10] == 0 and stats[name][13] == 0
+    assert stats["learned"][14] > 0  # the learned start was used
+    # 16 whole rows against the quad two-body solution
+    rows = [p for pr in _closest_pairs(st, 3) for p in pr]
+    for name in ("guess", "plain"):
+        for p in np.argsort(-off[name]):
+            if len(rows) >= (9 if name == "guess" else 12):
+                break
+            if int(p) not in rows:
+                rows.append(int(p))
+    rng = np.random.default_rng(16)
+    for p in [0, n - 1] + [int(x) for x in rng.integers(1, n - 1, 8)]:
+        if len(rows) < 16 and p not in rows:
+            rows.append(p)
+    chi, clo, dq = O.rows_quad(st, rows, eps, dt)
+    xq = st[:, rows] + dq
+    xmax = np.abs(st[:3] + ref[:3]).max()
+    vmax = np.abs(st[3:] + ref[3:]).max()
+    corr = np.abs(chi).max(axis=0)  # largest component of each row's pair sum
+    quad = {}
+    for name in ("guess", "learned", "plain", "rkf"):
+        x = st[:, rows] + out[name][:, rows]
+        ep = np.abs(x[:3] - xq[:3]).max() / xmax
+        ev = np.abs(x[3:] - xq[3:]).max() / vmax
+        dd = np.abs(out[name][:, rows] - dq)
+        ec = (dd[3:].max(axis=0) / np.abs(chi[3:]).max(axis=0)).max()
+        ecp = (dd[:3].max(axis=0) / np.abs(chi[:3]).max(axis=0)).max()
+        quad[name] = (ep, ev, ecp, ec)
+        print(f"{name} vs quad on {len(rows)} rows: normwise positions {ep:.2e}, velocities {ev:.2e}; "
+              f"relative to each row's Kepler correction: positions {ecp:.2e}, velocities {ec:.2e}")
+    for name in ("guess", "learned"):
+        ep, ev, ecp, ec = quad[name]
+        assert ep < 1e-12 and ev < 1e-12, (name, ep, ev)
+        assert ecp < 1e-11 and ec < 1e-11, (name, ecp, ec)
+
+
+def _dim_to(ctx, st, eps_dim, t_end, chunks):
+    """The DIM RKF45 (abserr 1e-16, the reference's controller) from 0 to t_end in `chunks` calls
+    (the controller's 10000-step budget is per call; h carries over); returns (state, accepted)."""
+    y = np.ascontiguousarray(st.T).copy()
+    h, init, acc = 0.0, 1, 0
+    for k in range(chunks):
+        r = ctx.dim_rkf45(y, eps_dim, t_end * k / chunks, t_end * (k + 1) / chunks, h=h, init=init)
+        assert r["ier"] == 0, (k, r["ier"])
+        y, h, init, acc = r["y"].reshape(-1, 6), r["h"], 0, acc + r["naccept"]
+    return np.ascontiguousarray(y.T), acc
+
+
+def test_config1_uniform_ball_1000_steps_against_dim(ctx):
+    """Config 1: N = 1332 uniform in the unit ball, Gaussian velocities rescaled to virial ratio
+    0.5, G = 1, m = 1/N (eps = -2/N for BIA, -1/N for the DIM, as bia.f90:37 and dim_3D.f90:9 pair
+    them), seed 1332.  Plain BIA-ALG (the reference's solvers) for 1000 graph-replayed steps of
+    dt = 1e-3, as the reference's own BIA-vs-DIM report does for its run (bia.f90:217-278):
+    * the first step against the oracle and against the reference's CPU ALG loop (amdflang build
+      of bia.f90's routines), within the per-particle condition bound;
+    * the state at t = 1 against the DIM RKF45 to t = 1 on the GPU: the error of the median
+      particle falls by about 4 when dt is halved (2000 steps of 5e-4; the BIA splitting error is
+      third order per step, second order over a fixed time), so it is the method's, not a solver's;
+    * |dE/E0| of BIA and of the DIM at t = 1 from the library's invariants (KE + PE = 2E/m)."""
+    import torch
+    import oracle as O
+    from bia_3d_gpu_amd import MODE_GUESS, MODE_PLAIN, ic
+    n = 1332
+    st = ic.uniform_ball_virial(n, seed=1332)
+    eps, eps_dim, dt = -2.0 / n, -1.0 / n, 1e-3
+    # first step
+    d1 = ctx.step_host(st, eps, dt, mode=MODE_PLAIN)
+    b = O.sensitivity(st, eps, dt)
+    ok, rp, rv = O.within_condition_bound(st, d1, O.bia_step(st, eps, dt), eps, bounds=b)
+    print(f"first step vs oracle: error / allowed {rp:.2f}, {rv:.2f}")
+    assert ok
+    if O.ref_available():
+        ok, rp, rv = O.within_condition_bound(st, d1, O.ref_cpu_rows(st, eps, dt), eps, bounds=b)
+        print(f"first step vs the reference CPU ALG: error / allowed {rp:.2f}, {rv:.2f}")
+        assert ok
+    # t = 1 by BIA at dt and dt/2, and by the DIM
+    final = {}
+    for mode, name in ((MODE_PLAIN, "plain"), (MODE_GUESS, "guess")):
+        for k, h in ((1000, dt), (2000, dt / 2)):
+            s = torch.from_numpy(st).cuda()
+            d = torch.empty_like(s)
+            ctx.run_steps_(s, d, [n], k, eps=eps, dt=h, mode=mode, use_graph=True)
+            torch.cuda.synchronize()
+            final[(name, k)] = s.cpu().numpy()
+    dim, nacc = _dim_to(ctx, st, eps_dim, 1.0, 20)
+    e0 = ctx.invariants(st, eps)
+    ed = ctx.invariants(dim, eps)
+    E = lambda inv: inv["KE"] + inv["PE"]
+    print(f"DIM to t = 1: {nacc} accepted steps, |dE/E0| {abs(E(ed) - E(e0)) / abs(E(e0)):.2e}")
+    med = {}
+    for (name, k), x in final.items():
+        dp = np.linalg.norm(x[:3] - dim[:3], axis=0)
+        dv = np.linalg.norm(x[3:] - dim[3:], axis=0)
+        eb = ctx.invariants(x, eps)
+        med[(name, k)] = np.median(dp)
+        print(f"{name} BIA, {k} steps: position error median {np.median(dp):.2e}, max {dp.max():.2e}; velocity "
+              f"median {np.median(dv):.2e}, max {dv.max():.2e}; |dE/E0| {abs(E(eb) - E(e0)) / abs(E(e0)):.2e}")
+        assert np.isfinite(x).all()
+    for name in ("plain", "guess"):
+        ratio = med[(name, 1000)] / med[(name, 2000)]
+        print(f"{name}: median error ratio dt / (dt/2) = {ratio:.2f}")
+        assert 2.5 < ratio < 8, ratio
+        assert med[(name, 1000)] < 1e-5
+    # the two solvers of the same method agree far below the method's error
+    assert np.abs(final[("plain", 1000)] - final[("guess", 1000)]).max() < 1e-3 * med[("plain", 1000)] + 1e-12
 
 
 _ENSEMBLE = {}

@@ -24,6 +24,22 @@ def _status(fn) -> int:
     return e.value.status
 
 
+def _raw(name, *args) -> int:
+    """Status of a direct call of a C entry point (the Python wrappers reject most of these
+    arguments themselves before calling the library, tests/test_api_checks.py)."""
+    from bia_3d_gpu_amd._lib import load
+    return getattr(load(), name)(*args)
+
+
+def _raw_step(ctx, s, d, sizes, eps, method=1, mode=MODE_GUESS, coupling=0, mass=None, charge=None, itr=18):
+    sz = np.ascontiguousarray(sizes, dtype=np.int32)
+    e = np.ascontiguousarray(np.broadcast_to(np.asarray(eps if eps is not None else 0.0, dtype=np.float64),
+                                             (len(sz),)))
+    ptr = lambda t: None if t is None else t.data_ptr()
+    return _raw("bia_step_batched_dev", ctx.handle, method, mode, coupling, len(sz), sz, e, 1.0, 1e-3, itr,
+                s.data_ptr(), s.shape[1], ptr(mass), ptr(charge), d.data_ptr(), None)
+
+
 @pytest.fixture(scope="module")
 def arrays():
     import torch
@@ -45,30 +61,41 @@ def _valid_step(c, s, mode=MODE_GUESS):
 def test_step_argument_errors(ctx, arrays):
     s, d, m = arrays
     n = s.shape[1]
-    step = ctx.step_batched_
-    assert _status(lambda: step(s, d, [0, n], eps=[-1.0, -1.0])) == BIA_ERR_ARG            # empty system
-    assert _status(lambda: step(s, d, [-1, n + 1], eps=[-1.0, -1.0])) == BIA_ERR_ARG       # negative size
-    assert _status(lambda: step(s, d, [64, 67], eps=EPS)) == BIA_ERR_ARG                   # sizes exceed ld
-    assert _status(lambda: step(s, d, [], eps=[])) == BIA_ERR_ARG                          # no system
-    assert _status(lambda: step(s, d, SIZES, eps=EPS, method=7)) == BIA_ERR_ARG
-    assert _status(lambda: step(s, d, SIZES, eps=EPS, mode=7)) == BIA_ERR_ARG
-    assert _status(lambda: step(s, d, SIZES, eps=EPS, method=METHOD_RKF, mode=MODE_GUESS)) == BIA_ERR_ARG
-    assert _status(lambda: step(s, d, SIZES, eps=EPS, coupling=3)) == BIA_ERR_ARG
-    assert _status(lambda: step(s, d, SIZES, coupling=COUPLING_GRAVITY)) == BIA_ERR_ARG     # no masses
-    assert _status(lambda: step(s, d, SIZES, coupling=COUPLING_COULOMB, mass=m)) == BIA_ERR_ARG  # no charges
-    assert _status(lambda: step(s, d, SIZES, eps=EPS, mode=MODE_PLAIN, itr=-1)) == BIA_ERR_ARG
-    assert _status(lambda: step(s, d, SIZES, eps=EPS, mode=MODE_PLAIN, itr=21)) == BIA_ERR_ARG  # history holds 20
-    assert _status(lambda: ctx.run_steps_(s, d, SIZES, -1, eps=EPS, mode=MODE_GUESS)) == BIA_ERR_ARG
-    assert _status(lambda: ctx.autotune_(s, d, [64, 67], eps=EPS, mode=MODE_GUESS)) == BIA_ERR_ARG
+    raw = lambda *a, **k: _raw_step(ctx, s, d, *a, **k)
+    assert raw([0, n], [-1.0, -1.0]) == BIA_ERR_ARG            # empty system
+    assert raw([-1, n + 1], [-1.0, -1.0]) == BIA_ERR_ARG       # negative size
+    assert raw([64, 67], EPS) == BIA_ERR_ARG                   # sizes exceed ld
+    assert raw([], []) == BIA_ERR_ARG                          # no system
+    assert raw(SIZES, EPS, method=7) == BIA_ERR_ARG
+    assert raw(SIZES, EPS, mode=7) == BIA_ERR_ARG
+    assert raw(SIZES, EPS, method=METHOD_RKF, mode=MODE_GUESS) == BIA_ERR_ARG
+    assert raw(SIZES, EPS, coupling=3) == BIA_ERR_ARG
+    assert raw(SIZES, None, coupling=COUPLING_GRAVITY) == BIA_ERR_ARG     # no masses
+    assert raw(SIZES, None, coupling=COUPLING_COULOMB, mass=m) == BIA_ERR_ARG  # no charges
+    assert raw(SIZES, EPS, mode=MODE_PLAIN, itr=-1) == BIA_ERR_ARG
+    assert raw(SIZES, EPS, mode=MODE_PLAIN, itr=21) == BIA_ERR_ARG  # history holds 20
+    sz = np.ascontiguousarray(SIZES, dtype=np.int32)
+    e = np.ascontiguousarray(EPS, dtype=np.float64)
+    assert _raw("bia_run_steps_dev", ctx.handle, 1, MODE_GUESS, 0, 2, sz, e, 1.0, 1e-3, 18, s.data_ptr(),
+                s.shape[1], None, None, d.data_ptr(), -1, 1, None) == BIA_ERR_ARG
+    bad = np.ascontiguousarray([64, 67], dtype=np.int32)
+    assert _raw("bia_autotune_dev", ctx.handle, 1, MODE_GUESS, 0, 2, bad, e, 1.0, 1e-3, 18, s.data_ptr(),
+                s.shape[1], None, None, d.data_ptr(), None, None) == BIA_ERR_ARG
     assert _status(lambda: ctx.step_host(np.zeros((6, 0)), -1.0, 1e-3)) == BIA_ERR_ARG
+    # the same arguments through the Python wrapper are rejected before the library is called
+    with pytest.raises(ValueError):
+        ctx.step_batched_(s, d, [64, 67], eps=EPS, dt=1e-3, mode=MODE_GUESS)
 
 
 def test_invariant_argument_errors(ctx, arrays):
     import torch
     s, _, _ = arrays
     out = torch.zeros(22, dtype=torch.float64, device="cuda")
-    assert _status(lambda: ctx.invariants_batched_(s, [0, 130], EPS, out)) == BIA_ERR_ARG
-    assert _status(lambda: ctx.invariants_batched_(s, [64, 67], EPS, out)) == BIA_ERR_ARG
+    e = np.ascontiguousarray(EPS, dtype=np.float64)
+    for bad in ([0, 130], [64, 67]):
+        sz = np.ascontiguousarray(bad, dtype=np.int32)
+        assert _raw("bia_invariants_batched_dev", ctx.handle, 2, sz, e, s.data_ptr(), s.shape[1],
+                    out.data_ptr()) == BIA_ERR_ARG
     assert _status(lambda: ctx.invariants(np.zeros((6, 0)), -1.0)) == BIA_ERR_ARG
 
 
@@ -95,8 +122,7 @@ def test_errors_leave_the_context_usable(ctx, arrays):
     from bia_3d_gpu_amd import Context
     s, d, _ = arrays
     for bad in ([0, 130], [64, 67], [-1, 131]):
-        with pytest.raises(BiaError):
-            ctx.step_batched_(s, d, bad, eps=[-1.0, -1.0], dt=1e-3, mode=MODE_GUESS)
+        assert _raw_step(ctx, s, d, bad, [-1.0, -1.0]) == BIA_ERR_ARG
     got = _valid_step(ctx, s)
     with Context(0) as fresh:
         want = _valid_step(fresh, s)

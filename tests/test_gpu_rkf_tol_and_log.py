@@ -1,0 +1,102 @@
+"""Per-pair RKF45 tolerances (bia_set_rkf_tolerance) and per-pair failure records
+(bia_set_failure_log) on the GPU."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_config2_rkf_at_relative_tolerance(ctx):
+    """Config 2 (Coulomb box, N = 16384, charges +-1, unit masses, k = 1, dt = 5e-4) with the
+    per-pair RKF45 at relerr 1e-12 (abserr kept at the reference's 1e-16 floor; the reference
+    hard-wires RELERR = 0, bia/bia-rkf.cu:5).  Six whole rows against the oracle's RKF45 at the
+    same tolerances (the same Fehlberg steps and controller, orc_rows_rkf_tol): within 1e-10 of
+    each row's largest delta.  The looser tolerance takes fewer RKF steps than the reference's,
+    and both agree with the Kepler drift to their tolerance."""
+    import torch
+    from bia_3d_gpu_amd import COUPLING_COULOMB, METHOD_ALG, METHOD_RKF, MODE_GUESS, STATS_LEN, ic
+    n, dt = 16384, 5e-4
+    st, q = ic.coulomb_box(n, seed=16384)
+    s = torch.from_numpy(st).cuda()
+    qc = torch.from_numpy(q).cuda()
+    rows = [0, 1, 4097, 8191, 12000, n - 1]
+    out, stats = {}, {}
+    try:
+        for name, method, tol in (("rkf_rel", METHOD_RKF, (1e-12, 1e-16)), ("rkf_ref", METHOD_RKF, (0.0, 1e-16)),
+                                  ("kepler", METHOD_ALG, (0.0, 1e-16))):
+            ctx.set_rkf_tolerance(*tol)
+            d = torch.empty_like(s)
+            c = torch.zeros(STATS_LEN, dtype=torch.int64, device="cuda")
+            ctx.step_batched_(s, d, [n], dt=dt, method=method, mode=MODE_GUESS if method else 0,
+                              coupling=COUPLING_COULOMB, coup=1.0, charge=qc, stats=c)
+            torch.cuda.synchronize()
+            out[name], stats[name] = d.cpu().numpy(), c.cpu().numpy()
+    finally:
+        ctx.set_rkf_tolerance()
+    steps = {k: stats[k][7] / stats[k][3] for k in ("rkf_rel", "rkf_ref")}
+    print(f"accepted RKF steps per pair: relerr 1e-12 {steps['rkf_rel']:.2f}, reference tolerance "
+          f"{steps['rkf_ref']:.2f}; failures {stats['rkf_rel'][12]} / {stats['rkf_ref'][12]}")
+    assert steps["rkf_rel"] < steps["rkf_ref"]
+    want, nf = O.rows_rkf_tol(st, rows, dt, 1e-12, 1e-16, coupling=2, coup=1.0, charge=q)
+    got = out["rkf_rel"][:, rows]
+    err = (np.abs(got - want).max(axis=0) / np.abs(want).max(axis=0)).max()
+    print(f"{len(rows)} rows vs the oracle's RKF45 at relerr 1e-12: {err:.1e} of each row's largest delta "
+          f"({nf} failed pairs in these rows)")
+    assert err < 1e-10
+    k = out["kepler"][:, rows]
+    ek = (np.abs(got - k).max(axis=0) / np.abs(k).max(axis=0)).max()
+    print(f"the same rows vs the Kepler drift: {ek:.1e}")
+    assert ek < 1e-6
+
+
+def test_failure_log_records_the_capped_pairs(ctx):
+    """A Coulomb box of 2048 particles in the reference solvers' mode (plain), where the
+    reference Newton stops at its iteration cap on many bound pairs: the log counts exactly the
+    pairs the statistics count with the cap, non-finite or RKF failure bits, writes one record per
+    pair (each i < j, no pair twice) up to its capacity, and the oracle (the same algorithm on the
+    CPU) reports the cap on the recorded pairs.  The Kepler drift logs none on the same box, and
+    a log smaller than the count keeps counting."""
+    import torch
+    from bia_3d_gpu_amd import COUPLING_COULOMB, MODE_GUESS, MODE_PLAIN, STATS_LEN, FailureLog, ic
+    n, dt = 2048, 5e-4
+    st, q = ic.coulomb_box(n, seed=2048)
+    s = torch.from_numpy(st).cuda()
+    qc = torch.from_numpy(q).cuda()
+    d = torch.empty_like(s)
+    res = {}
+    for name, mode, cap in (("plain", MODE_PLAIN, 1 << 20), ("plain_small", MODE_PLAIN, 100), ("kepler", MODE_GUESS, 64)):
+        log = FailureLog(cap)
+        c = torch.zeros(STATS_LEN, dtype=torch.int64, device="cuda")
+        ctx.set_failure_log(log)
+        try:
+            ctx.step_batched_(s, d, [n], dt=dt, mode=mode, coupling=COUPLING_COULOMB, coup=1.0, charge=qc, stats=c)
+            torch.cuda.synchronize()
+        finally:
+            ctx.set_failure_log(None)
+        res[name] = (log.total(), log.records(), c.cpu().numpy())
+    total, rec, stats = res["plain"]
+    flagged_bits = stats[10] + stats[12] + stats[13]
+    print(f"plain: {total} pairs logged; cap {stats[10]}, RKF failures {stats[12]}, non-finite {stats[13]}")
+    assert total > 0 and stats[10] <= total <= flagged_bits
+    assert len(rec) == total
+    assert (rec["i"] < rec["j"]).all() and (rec["sys"] == 0).all()
+    assert len(np.unique(rec["i"].astype(np.int64) * n + rec["j"])) == total
+    assert ((rec["flags"] & 4) != 0).sum() == stats[10]
+    # the oracle on a sample of the recorded pairs
+    capped = 0
+    sample = rec[:: max(1, len(rec) // 200)]
+    for r in sample:
+        i, j = int(r["i"]), int(r["j"])
+        rel = st[:, i] - st[:, j]
+        eps = q[i] * q[j] * 2.0
+        _, br, _, fl = O.pair(1, eps, dt, rel)
+        capped += bool(fl & O.F_ITR_CAP)
+        assert r["branch"] == br
+    print(f"oracle reports the cap on {capped} of {len(sample)} recorded pairs")
+    assert capped >= 0.9 * len(sample)
+    t_small, rec_small, _ = res["plain_small"]
+    assert t_small == total and len(rec_small) == 100
+    t_k, rec_k, stats_k = res["kepler"]
+    assert t_k == 0 and len(rec_k) == 0 and stats_k[10] == 0

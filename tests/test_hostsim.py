@@ -267,11 +267,11 @@ def test_kepler_drift_against_40_digit_solution(hs):
     print(f"{sum(len(c[0]) for c in cases)} pairs: largest error {worst:.1e} of the pair's largest delta")
 
 
-def _drift_closed_form(rel, eps, dt, dps=80):
+def _drift_closed_form(rel, eps, dt, dps=80, exact=False):
     """The pair's half deltas from the two-body solution with the Stumpff functions in closed
     form (cos/sin for bound, cosh/sinh for unbound orbits) and the time equation solved by
     bisection, in `dps` digits (mpmath): valid at any strength, where the 80-term series of
-    _universal_drift_40_digits is not."""
+    _universal_drift_40_digits is not.  With exact, the mpmath values themselves."""
     import mpmath as mp
     with mp.workdps(dps):
         r0 = [mp.mpf(float(v)) for v in rel[:3]]
@@ -311,7 +311,7 @@ def _drift_closed_form(rel, eps, dt, dps=80):
         fm1, gmdt = -mu * g2 / R0, -mu * g3
         fd, gdm1 = -mu * g1 / (R0 * rn), -mu * g2 / rn
         d = [(fm1 * r0[q] + gmdt * v0[q]) / 2 for q in range(3)] + [(fd * r0[q] + gdm1 * v0[q]) / 2 for q in range(3)]
-        return np.array([float(c) for c in d])
+        return d if exact else np.array([float(c) for c in d])
 
 
 def test_kepler_drift_converges_on_strong_pairs(hs):

@@ -329,3 +329,113 @@ def test_coupled_step_conserves_momentum_and_centre_of_mass():
     q = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
     dc = O.bia_step_coupled(st, 2, 1e-4, 1e-3, charge=q)
     assert np.abs(dc[3:].sum(1)).max() < 1e-15 * np.abs(dc[3:]).sum()
+
+
+# --------------------------------------------------------------------------------------------
+# two-body step in quadruple precision (orc_pair_quad / orc_rows_quad): the accuracy oracle of
+# the pair propagators below 1e-12
+# --------------------------------------------------------------------------------------------
+def _quad_pairs(rng, m):
+    """Pairs of every kind: weak distant ones, bound over whole periods, fast flybys, repulsive."""
+    out = []
+    for k in range(m):
+        r = rng.standard_normal(3) * 10.0 ** rng.uniform(-2, 1)
+        v = rng.standard_normal(3) * 10.0 ** rng.uniform(-2, 0.5)
+        eps = -10.0 ** rng.uniform(-5, 0) * (1 if k % 5 else -1)
+        dt = 10.0 ** rng.uniform(-3, 0.5)
+        out.append((np.concatenate([r, v]), eps, dt))
+    return out
+
+
+def test_quad_pair_against_80_digits():
+    """Every pair's quad half deltas (hi + lo) against the 80-digit closed-form solution to 1e-30
+    of the pair's largest delta: a dropped term, a wrong sign or a lost digit of the Stumpff
+    functions would show at 1e-16 or worse."""
+    mp = pytest.importorskip("mpmath")
+    from test_hostsim import _drift_closed_form
+    rng = np.random.default_rng(1201)
+    worst = 0.0
+    for rel, eps, dt in _quad_pairs(rng, 40):
+        hi, lo = O.pair_quad(eps, dt, rel)
+        with mp.workdps(80):
+            ex = _drift_closed_form(rel, eps, dt, exact=True)
+            err = max(abs(mp.mpf(float(h)) + mp.mpf(float(l)) - e) for h, l, e in zip(hi, lo, ex))
+            worst = max(worst, float(err / max(abs(e) for e in ex)))
+    print(f"quad pair vs 80 digits: largest error {worst:.1e} of the pair's largest delta")
+    assert worst < 1e-30
+
+
+def test_quad_pair_conserves_energy_and_angular_momentum():
+    """r1 = r0 + v0 dt + 2 d_r and v1 = v0 + 2 d_v from the quad deltas keep the relative energy
+    v^2/2 - mu/r and the angular momentum r x v to 1e-28 (in 50 digits)."""
+    mp = pytest.importorskip("mpmath")
+    rng = np.random.default_rng(1202)
+    with mp.workdps(50):
+        for rel, eps, dt in _quad_pairs(rng, 30):
+            hi, lo = O.pair_quad(eps, dt, rel)
+            d = [mp.mpf(float(h)) + mp.mpf(float(l)) for h, l in zip(hi, lo)]
+            r0 = [mp.mpf(float(c)) for c in rel[:3]]
+            v0 = [mp.mpf(float(c)) for c in rel[3:]]
+            r1 = [r0[q] + v0[q] * mp.mpf(dt) + 2 * d[q] for q in range(3)]
+            v1 = [v0[q] + 2 * d[3 + q] for q in range(3)]
+            mu = -mp.mpf(eps)
+            e0 = sum(c * c for c in v0) / 2 - mu / mp.sqrt(sum(c * c for c in r0))
+            e1 = sum(c * c for c in v1) / 2 - mu / mp.sqrt(sum(c * c for c in r1))
+            l0 = [r0[1] * v0[2] - r0[2] * v0[1], r0[2] * v0[0] - r0[0] * v0[2], r0[0] * v0[1] - r0[1] * v0[0]]
+            l1 = [r1[1] * v1[2] - r1[2] * v1[1], r1[2] * v1[0] - r1[0] * v1[2], r1[0] * v1[1] - r1[1] * v1[0]]
+            escale = sum(c * c for c in v0) / 2 + abs(mu) / mp.sqrt(sum(c * c for c in r0))
+            lscale = mp.sqrt(sum(c * c for c in r0) * sum(c * c for c in v0))
+            assert abs(e1 - e0) < 1e-28 * escale, (rel, eps, dt)
+            assert max(abs(a - b) for a, b in zip(l0, l1)) < 1e-28 * lscale, (rel, eps, dt)
+
+
+def test_quad_rows_against_pairs_and_double_oracle():
+    """orc_rows_quad on a 48-particle Plummer system with hard binaries: each row equals the
+    signed sum of the quad pair deltas over its partners (each pair oriented i < j, drift added),
+    and agrees with the double-precision oracle's whole step (the reference algorithm) to its
+    rounding; a row with the partners' signs or orientation wrong would not."""
+    from bia_3d_gpu_amd import ic
+    mp = pytest.importorskip("mpmath")
+    st = ic.plummer_with_binaries(48, seed=4801)
+    n, eps, dt = st.shape[1], -2.0 / 48, 1e-3
+    rows = [0, 1, 17, 46, 47]
+    hi, lo, d = O.rows_quad(st, rows, eps, dt)
+    with mp.workdps(40):
+        for k, p in enumerate(rows):
+            acc = [mp.mpf(0)] * 6
+            for j in range(n):
+                if j == p:
+                    continue
+                i0, j0 = min(p, j), max(p, j)
+                ph, pl = O.pair_quad(eps, dt, st[:, i0] - st[:, j0])
+                sg = 1 if p < j else -1
+                acc = [a + sg * (mp.mpf(float(h)) + mp.mpf(float(l))) for a, h, l in zip(acc, ph, pl)]
+            for c in range(6):
+                got = mp.mpf(float(hi[c, k])) + mp.mpf(float(lo[c, k]))
+                assert abs(got - acc[c]) <= 1e-30 * max(abs(x) for x in acc), (p, c)
+                want = acc[c] + (mp.mpf(float(st[3 + c, p])) * mp.mpf(dt) if c < 3 else 0)
+                assert d[c, k] == float(want)
+    # every row against the double oracle's whole step: the reference algorithm is off the exact
+    # solution by its own rounding sensitivity (up to 1e-10 of the largest delta near the binaries
+    # here), so the comparison is the per-particle condition bound of the parity tests
+    _, _, dall = O.rows_quad(st, list(range(n)), eps, dt)
+    orc = O.bia_step(st, eps, dt)
+    ok, rp, rv = O.within_condition_bound(st, orc, dall, eps, bounds=O.sensitivity(st, eps, dt))
+    print(f"double oracle vs quad rows: error / allowed positions {rp:.2f}, velocities {rv:.2f}")
+    assert ok
+
+
+def test_config1_first_step_oracle_vs_reference_cpu():
+    """Config 1 (seeded N = 1332 uniform ball at virial ratio 0.5, G = 1, m = 1/N, eps = -2/N,
+    dt = 1e-3): the oracle's first step (GPU semantics) against the reference's own CPU ALG loop
+    (bia.f90:142-196, amdflang build) within the per-particle condition bound."""
+    if not O.ref_available():
+        pytest.skip("reference CPU routines not built (oracle/_ref)")
+    from bia_3d_gpu_amd import ic
+    st = ic.uniform_ball_virial(1332, seed=1332)
+    eps, dt = -2.0 / 1332, 1e-3
+    orc = O.bia_step(st, eps, dt)
+    ref = O.ref_cpu_rows(st, eps, dt)
+    ok, rp, rv = O.within_condition_bound(st, ref, orc, eps, bounds=O.sensitivity(st, eps, dt))
+    print(f"config 1 first step, reference CPU vs oracle: error / allowed {rp:.2f}, {rv:.2f}")
+    assert ok
