@@ -7,9 +7,9 @@ dt = 1e-3, eps = -2Gm, seed 65536.  One "step" = one BIA step of every unordered
 
 Modes measured on the same input: plain (the reference's conic solvers, guesses and iteration
 control), guess (universal-variable Kepler drift, every pair from its series start, no network)
-and learned (the same drift; pairs that are not weak over dt start from the network's table).
-The network is trained on the GPU first, from samples the solver records on other systems
-(training time reported separately).
+and learned (the same drift; a pair whose series start is far off takes its second point from the
+network's table).  The network is trained on the GPU first, from samples the solver records on
+other systems (training time reported separately).
 
 Printed: one JSON line (rank 0).  ``value`` is the pair-interactions/s of the learned mode
 (guess if training is skipped with --no-learn).  ``roofline`` prices the dominant kernel (the
@@ -20,9 +20,15 @@ gives the reference algorithm's flops for the same pairs (the plain mode's count
 duration, a rate of reference-equivalent work that can exceed the FP64 peak.  ``energy`` gives
 |dE/E0| over the timed steps of each mode from the library's double-double invariants.
 ``cpu_baseline`` times the reference's own Fortran CPU pair loop (oracle/_ref) on bounded row
-samples, on one core and on one process per host core (at most 16).  ``agreement_vs_rkf45``
-compares every mode's first step with an untimed per-pair RKF45 step of the same system (the
-reference's own integrator, an independent solution of the same pairs).
+samples, on one core and on one process per host core (at most 16), and its RKF path on one core
+(CPU model, compiler and flags given).  ``agreement_vs_rkf45`` compares every mode's first step
+with an untimed per-pair RKF45 step of the same system (the reference's own integrator, an
+independent solution of the same pairs).  ``reductions`` gives the HBM rate of the step's
+tile-partial reduction kernel (its own HIP events) and of the invariants.  Side measurements:
+``learned_vs_guess`` (headline, longer steps, the reference fixture), ``reference_fixture``
+(with each mode's normwise agreement with the reference's own .BIA output), ``config1`` (1000
+steps against the DIM), ``energy_order`` (|dE/E0| over a fixed time at dt, dt/2, dt/4), and the
+two ensembles.
 """
 from __future__ import annotations
 
@@ -88,16 +94,34 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kms = ctx.kernel_ms()  # the pair kernel of each timed step (HIP events on the library's stream)
+    rms = ctx.reduce_kernel_ms()  # the tile-partial reduction of the same steps
     ctx.set_profiling(False)
     ms = e0.elapsed_time(e1) / steps
     inv1 = torch.zeros(11, dtype=torch.float64, device=dev)
     ctx.invariants_(s, inv1, eps)
+    # the invariants' kernels (on the same stream), timed over five calls
+    i0, i1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    i0.record(stream)
+    for _ in range(5):
+        ctx.invariants_(s, inv1, eps)
+    i1.record(stream)
+    torch.cuda.synchronize()
+    inv_ms = i0.elapsed_time(i1) / 5
     a, b = inv0.cpu().numpy(), inv1.cpu().numpy()
     e_0, e_1 = a[0] + a[1], b[0] + b[1]  # KE + PE = 2E/m
     kms = float(np.mean(kms))
+    n = s.shape[1]
+    nt = (n + 63) // 64
+    # the reduction reads every tile partial once (6 components x nt tiles x 64 nt slots) and the
+    # velocities, and writes the deltas
+    red_bytes = 8 * 6 * nt * nt * 64 + 24 * n + 48 * n
+    red_ms = float(np.mean(rms)) if rms else float("nan")
     return dict(ms_per_step=ms, wall_s=wall, kernel_ms=kms, stats=st.tolist(), flops=flops, tuned=tuned,
                 tflops=flops / (kms / 1e3) / 1e12, dE_rel=abs(e_1 - e_0) / abs(e_0), dE_steps=steps,
-                first_delta=first_delta)
+                first_delta=first_delta,
+                reduction={"kernel_ms": red_ms, "bytes_per_launch": red_bytes,
+                           "GB_per_s": red_bytes / (red_ms * 1e-3) / 1e9},
+                invariants={"ms": inv_ms, "state_bytes": 48 * n, "GB_per_s_of_state": 48 * n / (inv_ms * 1e-3) / 1e9})
 
 
 def agreement(st_host, d, ref):
@@ -184,17 +208,19 @@ def run_mass_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 4096):
             "dE_rel_median": float(np.median(de)), "dE_rel_max": float(de.max())}
 
 
-def run_fixture(ctx, steps: int):
+def run_fixture(ctx, steps: int, learned: bool):
     """Side measurement on the reference's own N = 1332 input (tests/golden/0001332_10^0.bbb), the
     only case with published reference timings (one BIA step, dt = 1, eps = -6e-7, TITAN V: ALG
     0.193 s and per-pair RKF 0.293 s of GPU solver time; the DIM RKF45 run to t = 1: 2.946 s
-    total).  Times the plain ALG mode (the reference's algorithm), the guess mode, the per-pair
-    RKF method and the DIM run."""
+    total).  Times the plain ALG mode (the reference's algorithm), the guess and learned modes, the
+    per-pair RKF method and the DIM run, and gives each step's normwise agreement with the
+    reference's own output on this input (OUT/0001332_10^0.BIA, its CUDA ALG run)."""
     import torch
 
-    from bia_3d_gpu_amd import MODE_GUESS, MODE_PLAIN, io
+    from bia_3d_gpu_amd import MODE_GUESS, MODE_LEARN, MODE_PLAIN, io
 
     st = io.read_bbb(os.path.join(ROOT, "tests", "golden", "0001332_10^0.bbb"))[0]
+    (_, bia_ref), = io.read_timed(os.path.join(ROOT, "tests", "golden", "0001332_10^0.BIA"))
     n = st.shape[1]
     pairs = n * (n - 1) / 2
     out = {"n": n, "pairs_per_step": pairs, "reference_published": {"bia_alg_gpu_solver_s": 0.193,
@@ -202,20 +228,147 @@ def run_fixture(ctx, steps: int):
                                                                      "dim_total_s": 2.946}}
     s = torch.from_numpy(np.ascontiguousarray(st)).to("cuda:0")
     d = torch.empty_like(s)
-    for name, method, mode, k in (("plain", 1, MODE_PLAIN, steps), ("guess", 1, MODE_GUESS, steps),
-                                  ("rkf", 0, MODE_PLAIN, 2)):
-        ctx.step_(s, d, -6e-7, 1.0, method=method, mode=mode)
+    runs = [("plain", 1, MODE_PLAIN, steps), ("guess", 1, MODE_GUESS, steps), ("rkf", 0, MODE_PLAIN, 2)]
+    if learned:
+        runs.insert(2, ("learned", 1, MODE_LEARN, steps))
+    stats = torch.zeros(16, dtype=torch.int64, device="cuda:0")
+    for name, method, mode, k in runs:
+        stats.zero_()
+        ctx.step_(s, d, -6e-7, 1.0, method=method, mode=mode, stats=stats)
         torch.cuda.synchronize()
+        c = stats.cpu().numpy()
+        # normwise agreement with the reference's own output for this input (OUT/0001332_10^0.BIA)
+        agree = agreement(st, d.cpu().numpy(), bia_ref)
+        ctx.set_profiling(True)
         t0 = time.perf_counter()
         for _ in range(k):
             ctx.step_(s, d, -6e-7, 1.0, method=method, mode=mode)
         torch.cuda.synchronize()
         sec = (time.perf_counter() - t0) / k
-        out[name] = {"s_per_step": sec, "pairs_per_s": pairs / sec}
+        kms = float(np.mean(ctx.kernel_ms()))
+        ctx.set_profiling(False)
+        out[name] = {"s_per_step": sec, "pairs_per_s": pairs / sec, "kernel_ms": kms,
+                     "evaluations_per_pair": float(c[4:8].sum() / max(1, c[0:4].sum())), "learned_starts": int(c[14]),
+                     "vs_reference_BIA": {"positions": agree["positions"], "velocities": agree["velocities"]}}
+    if learned:
+        out["learned_vs_guess"] = {"kernel_ms_ratio": out["learned"]["kernel_ms"] / out["guess"]["kernel_ms"],
+                                   "s_per_step_ratio": out["learned"]["s_per_step"] / out["guess"]["s_per_step"]}
     y = np.ascontiguousarray(st.T).reshape(-1)
     t0 = time.perf_counter()
     r = ctx.dim_rkf45(y, -3e-7, 0.0, 1.0)
     out["dim"] = {"s": time.perf_counter() - t0, "accepted_steps": int(r["naccept"]), "ier": int(r["ier"])}
+    return out
+
+
+def run_config1(ctx, steps: int = 1000):
+    """Side measurement, config 1: N = 1332 uniform in the unit ball, virial ratio 0.5, G = 1,
+    m = 1/N, seed 1332; `steps` graph-replayed BIA steps of dt = 1e-3 in the plain (reference
+    solvers) and guess modes, against the DIM RKF45 (abserr 1e-16, the reference's controller) to
+    t = steps dt on the GPU: ms per step, the median and worst particle's position error, |dE/E0|
+    from the invariants.  The reference's ellipse Newton fails on near-radial bound pairs of this
+    system (from step 20), so the plain trajectory ends far from the DIM."""
+    import torch
+
+    from bia_3d_gpu_amd import MODE_GUESS, MODE_PLAIN, ic
+
+    n, dt = 1332, 1e-3
+    st = ic.uniform_ball_virial(n, seed=1332)
+    eps = -2.0 / n
+    E = lambda inv: inv["KE"] + inv["PE"]
+    e0 = E(ctx.invariants(st, eps))
+    y = np.ascontiguousarray(st.T).copy()
+    h, init, acc = 0.0, 1, 0
+    t0 = time.perf_counter()
+    t_end, chunks = steps * dt, 20
+    for k in range(chunks):  # the controller's 10000-step budget is per call; h carries over
+        r = ctx.dim_rkf45(y, -1.0 / n, t_end * k / chunks, t_end * (k + 1) / chunks, h=h, init=init)
+        y, h, init, acc = r["y"].reshape(-1, 6), r["h"], 0, acc + r["naccept"]
+    dim_s = time.perf_counter() - t0
+    dim = np.ascontiguousarray(y.T)
+    out = {"n": n, "dt": dt, "steps": steps, "dim": {"s": dim_s, "accepted_steps": acc,
+                                                      "dE_rel": abs(E(ctx.invariants(dim, eps)) - e0) / abs(e0)}}
+    for name, mode in (("plain", MODE_PLAIN), ("guess", MODE_GUESS)):
+        s = torch.from_numpy(st).to("cuda:0")
+        d = torch.empty_like(s)
+        ctx.run_steps_(s, d, [n], 1, eps=eps, dt=dt, mode=mode, use_graph=True)  # capture
+        s.copy_(torch.from_numpy(st))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.run_steps_(s, d, [n], steps, eps=eps, dt=dt, mode=mode, use_graph=True)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        x = s.cpu().numpy()
+        dp = np.linalg.norm(x[:3] - dim[:3], axis=0)
+        out[name] = {"ms_per_step": sec * 1e3, "pairs_per_s": n * (n - 1) / 2 / sec,
+                     "position_error_median": float(np.median(dp)), "position_error_max": float(dp.max()),
+                     "dE_rel": abs(E(ctx.invariants(x, eps)) - e0) / abs(e0)}
+    return out
+
+
+def run_energy_order(ctx, st, eps, dt: float, learned: bool, nsteps: int = 4):
+    """Side measurement: |dE/E0| of the benchmark system over the fixed time T = nsteps dt with
+    steps dt, dt/2 and dt/4 (guess and learned modes) and with the per-pair RKF45 at dt, from the
+    library's double-double invariants (KE + PE = 2E/m); the observed order of the energy error,
+    log2(e(h) / e(h/2))."""
+    import torch
+
+    from bia_3d_gpu_amd import METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN
+
+    n = st.shape[1]
+    inv = torch.zeros(11, dtype=torch.float64, device="cuda:0")
+    s0 = torch.from_numpy(st).to("cuda:0")
+    ctx.invariants_(s0, inv, eps)
+    e0 = float(inv[0] + inv[1])
+    out = {"T": nsteps * dt}
+    runs = [("guess", 1, MODE_GUESS)] + ([("learned", 1, MODE_LEARN)] if learned else [])
+    for name, method, mode in runs:
+        errs = []
+        for k in (1, 2, 4):
+            s = s0.clone()
+            d = torch.empty_like(s)
+            ctx.run_steps_(s, d, [n], nsteps * k, eps=eps, dt=dt / k, mode=mode, use_graph=True)
+            ctx.invariants_(s, inv, eps)
+            errs.append(abs(float(inv[0] + inv[1]) - e0) / abs(e0))
+        out[name] = {"dE_rel": dict(zip(("dt", "dt/2", "dt/4"), errs)),
+                     "order": [float(np.log2(errs[0] / errs[1])), float(np.log2(errs[1] / errs[2]))]}
+    s = s0.clone()
+    d = torch.empty_like(s)
+    ctx.run_steps_(s, d, [n], nsteps, eps=eps, dt=dt, method=METHOD_RKF, mode=MODE_PLAIN, use_graph=False)
+    ctx.invariants_(s, inv, eps)
+    out["rkf45"] = {"dE_rel": {"dt": abs(float(inv[0] + inv[1]) - e0) / abs(e0)}}
+    return out
+
+
+def run_learned_gain(ctx, st, eps, dts=(1e-2, 1e-1), steps: int = 5):
+    """Side measurement: the guess and learned modes on the benchmark system at longer steps,
+    where more pairs are strong over dt and the learned table is consulted: pair-kernel ms,
+    Kepler-equation evaluations per pair and learned starts."""
+    import torch
+
+    from bia_3d_gpu_amd import MODE_GUESS, MODE_LEARN
+
+    n = st.shape[1]
+    s = torch.from_numpy(st).to("cuda:0")
+    d = torch.empty_like(s)
+    c = torch.zeros(16, dtype=torch.int64, device="cuda:0")
+    out = {}
+    for dt in dts:
+        row = {}
+        for name, mode in (("guess", MODE_GUESS), ("learned", MODE_LEARN)):
+            ctx.autotune_(s, d, [n], eps=eps, dt=dt, mode=mode)
+            c.zero_()
+            ctx.step_(s, d, eps, dt, mode=mode, stats=c)
+            torch.cuda.synchronize()
+            cs = c.cpu().numpy()
+            ctx.set_profiling(True)
+            for _ in range(steps):
+                ctx.step_(s, d, eps, dt, mode=mode)
+            kms = float(np.median(ctx.kernel_ms()))
+            ctx.set_profiling(False)
+            row[name] = {"kernel_ms": kms, "evaluations_per_pair": float(cs[4:8].sum() / cs[0:4].sum()),
+                         "learned_starts": int(cs[14])}
+        row["learned_vs_guess_kernel"] = row["learned"]["kernel_ms"] / row["guess"]["kernel_ms"]
+        out[f"dt={dt:g}"] = row
     return out
 
 
@@ -233,8 +386,10 @@ def measured_traffic(mode: str, n: int):
 
 
 def _cpu_rows(args):
-    """One CPU worker: the reference's Fortran pair loop on the given row ranges, until budget_s."""
-    st_host, eps, dt, itr, ranges, budget_s = args
+    """One CPU worker: the reference's Fortran pair loop (method 1 ALG, 0 RKF) on the given row
+    ranges, until budget_s."""
+    st_host, eps, dt, itr, ranges, budget_s = args[:6]
+    method = args[6] if len(args) > 6 else 1
     import oracle as O
 
     n = st_host.shape[1]
@@ -245,12 +400,34 @@ def _cpu_rows(args):
             break
         t0 = time.perf_counter()
         if kind == "reference":
-            O.ref_cpu_rows(st_host, eps, dt, 1, itr, rows=(r0, r1))
+            O.ref_cpu_rows(st_host, eps, dt, method, itr, rows=(r0, r1))
         else:
-            O.bia_step(st_host, eps, dt, 1, itr, rows=(r0, r1))
+            O.bia_step(st_host, eps, dt, method, itr, rows=(r0, r1))
         t += time.perf_counter() - t0
         pairs += sum(n - 1 - i for i in range(r0, r1))
     return pairs, t, kind
+
+
+def _host_info():
+    """CPU model, and the compiler and flags of the reference CPU build (oracle/Makefile)."""
+    import subprocess
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        comp = subprocess.run(["amdflang", "--version"], capture_output=True, text=True, timeout=30).stdout.splitlines()[0]
+    except Exception:
+        comp = "amdflang"
+    flags = None
+    for line in open(os.path.join(ROOT, "oracle", "Makefile")):
+        if line.startswith("FFLAGS"):
+            flags = line.split(":=", 1)[1].strip()
+    return model, comp, flags
 
 
 def cpu_baseline(st_host, eps, dt, itr, budget_s: float):
@@ -271,11 +448,17 @@ def cpu_baseline(st_host, eps, dt, itr, budget_s: float):
         out = pool.map(_cpu_rows, work)
     wall = time.perf_counter() - t0
     pairs = sum(o[0] for o in out)
+    # the reference's CPU RKF path (bia.f90:147-154, find_dXYUV_RKF.f90) on one core, rows of two
+    rk_ranges = [(r, min(n, r + 2)) for r in range(0, n, max(2, n // 64))]
+    pr, tr, _ = _cpu_rows((st_host, eps, dt, itr, rk_ranges, budget_s / 4, 0))
     what = "amdflang -O3 build of the reference Fortran CPU routines" if kind == "reference" else "C oracle"
+    model, comp, flags = _host_info()
     return dict(value=pairs / wall, unit="pair-interactions/s", cores=cores, kind=kind,
-                single_core_value=p1 / t1,
-                sample=f"row chunks of the same N={n} system, one step, {what}: {p1} pairs on one core in "
-                       f"{t1:.1f} s; {pairs} pairs on {cores} processes in {wall:.1f} s wall")
+                single_core_value=p1 / t1, rkf_single_core_value=pr / tr, cpu_model=model, compiler=comp,
+                flags=flags,
+                sample=f"row chunks of the same N={n} system, one step, {what}: ALG {p1} pairs on one core in "
+                       f"{t1:.1f} s; ALG {pairs} pairs on {cores} processes in {wall:.1f} s wall; RKF {pr} pairs "
+                       f"on one core in {tr:.1f} s")
 
 
 def main():
@@ -318,7 +501,8 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(st, eps, args.dt, args.itr, args.cpu_budget)
     if cpu:
         log(f"cpu baseline: {cpu['value']:.3e} pairs/s on {cpu['cores']} cores "
-            f"({cpu['single_core_value']:.3e} on one)")
+            f"({cpu['single_core_value']:.3e} on one; RKF {cpu['rkf_single_core_value']:.3e} on one; "
+            f"{cpu['cpu_model']}, {cpu['compiler']} {cpu['flags']})")
     assert torch.cuda.is_available()
     ctx = Context(0)
     modes = set(args.modes.split(","))
@@ -368,11 +552,29 @@ def main():
         mass_ensemble["mode"] = "learned" if emode == MODE_LEARN else "guess"
         log(f"mass-spectrum ensemble (64 systems, {mass_ensemble['particles']} particles, gravity coupling): "
             f"{mass_ensemble['ms_per_step']:.2f} ms/step, {mass_ensemble['pairs_per_s']:.3e} pairs/s")
-    fixture = None if args.no_fixture else run_fixture(ctx, 20)
+    fixture = None if args.no_fixture else run_fixture(ctx, 20, "learned" in res)
     if fixture:
         log(f"reference fixture N=1332: plain {fixture['plain']['s_per_step'] * 1e3:.3f} ms/step, guess "
             f"{fixture['guess']['s_per_step'] * 1e3:.3f} ms/step, RKF {fixture['rkf']['s_per_step'] * 1e3:.1f} "
             f"ms/step, DIM {fixture['dim']['s']:.3f} s ({fixture['dim']['accepted_steps']} steps)")
+    config1 = None if args.no_fixture else run_config1(ctx)
+    if config1:
+        log(f"config 1 (N=1332 ball, 1000 steps of 1e-3 vs DIM, {config1['dim']['accepted_steps']} DIM steps in "
+            f"{config1['dim']['s']:.1f} s): plain {config1['plain']['ms_per_step']:.3f} ms/step, median error "
+            f"{config1['plain']['position_error_median']:.2e}, |dE/E0| {config1['plain']['dE_rel']:.2e}; guess "
+            f"{config1['guess']['ms_per_step']:.3f} ms/step, median error {config1['guess']['position_error_median']:.2e}, "
+            f"|dE/E0| {config1['guess']['dE_rel']:.2e}")
+    energy_order = None if args.no_ensemble else run_energy_order(ctx, st, eps, args.dt, "learned" in res)
+    if energy_order:
+        log(f"energy over T={energy_order['T']}: " + "; ".join(
+            f"{k} {v['dE_rel']}" + (f" order {v['order']}" if 'order' in v else "") for k, v in energy_order.items()
+            if k != "T"))
+    learned_gain = run_learned_gain(ctx, st, eps) if "learned" in res and not args.no_ensemble else None
+    if learned_gain:
+        for k, v in learned_gain.items():
+            log(f"learned vs guess at {k}: kernel {v['learned']['kernel_ms']:.2f} vs {v['guess']['kernel_ms']:.2f} ms, "
+                f"evaluations/pair {v['learned']['evaluations_per_pair']:.5f} vs {v['guess']['evaluations_per_pair']:.5f}, "
+                f"{v['learned']['learned_starts']} learned starts")
     plain = res.get("plain")
     # every mode's first step against an independent solution of the same pairs: the per-pair
     # RKF45 (the reference's FEHL integrator and step control), one untimed step at full size
@@ -391,7 +593,7 @@ def main():
     head = res[head_name]
     # roofline of the dominant kernel (the pair kernel of the reported mode), its own algorithm's flops
     achieved = head["tflops"]
-    roof = {"bound": "mfma", "pipe": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+    roof = {"bound": "fp64-valu", "pipe": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(head_name, n),
             "flops_per_launch": head["flops"], "kernel_ms": head["kernel_ms"], "kernel": f"pair kernel, {head_name} mode"}
     if plain is not None and head_name != "plain":
@@ -413,7 +615,16 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "modes": {k: {q: v[q] for q in ("pairs_per_s", "ms_per_step", "kernel_ms", "tflops", "dE_rel", "dE_steps",
-                                         "stats", "tuned")} for k, v in res.items()},
+                                         "stats", "tuned", "reduction", "invariants")} for k, v in res.items()},
+        "learned_vs_guess": None if not {"learned", "guess"} <= set(res) else {
+            "ms_per_step_ratio": res["learned"]["ms_per_step"] / res["guess"]["ms_per_step"],
+            "kernel_ms_ratio": res["learned"]["kernel_ms"] / res["guess"]["kernel_ms"],
+            "learned_starts": int(res["learned"]["stats"][14]),
+            "longer_steps": learned_gain,
+            "reference_fixture": None if not fixture else fixture.get("learned_vs_guess")},
+        "reductions": {"tile_partials": head["reduction"], "invariants": head["invariants"]},
+        "config1": config1,
+        "energy_order": energy_order,
         "speedup_vs_plain": None if plain is None else {k: v["pairs_per_s"] / plain["pairs_per_s"]
                                                         for k, v in res.items()},
         "agreement_vs_rkf45": parity,

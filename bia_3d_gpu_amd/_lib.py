@@ -45,6 +45,7 @@ SIGNATURES = {
     "bia_set_profiling": (C.c_int, [_vp, C.c_int]),
     "bia_last_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "bia_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_float), C.c_int, _ip]),
+    "bia_reduce_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_float), C.c_int, _ip]),
     "bia_step_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _vp, C.c_int,
                                _vp, _vp]),
     "bia_step_host": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,

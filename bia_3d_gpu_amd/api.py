@@ -291,6 +291,14 @@ class Context:
         check(load().bia_kernel_ms(self._h, buf, int(max_launches), C.byref(n)), "bia_kernel_ms", self._h)
         return [buf[i] for i in range(n.value)]
 
+    def reduce_kernel_ms(self, max_launches: int = 64) -> list[float]:
+        """Durations of the tile-partial reduction kernel of the same steps, oldest first."""
+        buf = (C.c_float * max_launches)()
+        n = C.c_int()
+        check(load().bia_reduce_kernel_ms(self._h, buf, int(max_launches), C.byref(n)), "bia_reduce_kernel_ms",
+              self._h)
+        return [buf[i] for i in range(n.value)]
+
     def synchronize(self):
         check(load().bia_synchronize(self._h), "bia_synchronize", self._h)
 

@@ -312,7 +312,7 @@ __global__ __launch_bounds__(64) void k_record(int n, int ld, double eps, double
     bool okc = false;
     const double s0 = kep_series_start(K, a);
     KepG last;
-    const double sv = kep_solve(K, s0, 200, &ev, &okc, &last);
+    const double sv = kep_solve<kGuess>(K, s0, 200, &ev, &okc, &last);
     const double t = log(sv / s0) / kep_target_scale(a);
     if (!okc || !(fabs(t) <= kMaxTarget)) return;
     o[0] = (float)feat_squash(a[0]);

@@ -491,6 +491,45 @@ BIA_HDF double kep_halley(const KepPair& K, const KepG& g, double* F, double* es
   return d;
 }
 
+// learned start: trilinear lookup of the target log(s / s_start) / min(|a|, 1)^3 in a table
+// over the squashed features u_k = sign(a_k) log2(1 + |a_k| / kFeatScale) / log2(1 + kFeatMax /
+// kFeatScale), clamped to [-1, 1].  The logarithm is taken in single precision (one hardware
+// instruction): the features index a table of 64 nodes per axis.
+constexpr double kFeatScale = 1e-2;
+constexpr double kFeatMax = 1e2;
+constexpr float kFeatNorm = 13.287856641840545f;  // log2(1 + kFeatMax / kFeatScale)
+constexpr double kWeak = 1e-2;  // the recorder samples pairs with some |a_k| at or above this
+
+BIA_HDF float feat_squash(double a) {
+  const float u = fminf(log2f(1.0f + (float)(fabs(a) * (1 / kFeatScale))) * (1.0f / kFeatNorm), 1.0f);
+  return copysignf(u, (float)a);
+}
+
+// log2(s / s_start) proposed by the table, in single precision (the start only has to be close
+// to the root): trilinear lookup of the target at the squashed features, times min(|a|, 1)^3 and
+// log2(e)
+BIA_HDF float learned_log2_ratio(const LearnTable& T, const double a[3]) {
+  const int n = T.n;
+  float f[3];
+  int i[3];
+  for (int k = 0; k < 3; k++) {
+    const float x = fminf(fmaxf((feat_squash(a[k]) + 1.0f) * 0.5f * (float)(n - 1), 0.0f), (float)(n - 1));
+    const int ik = (int)x;
+    i[k] = ik < n - 2 ? ik : n - 2;  // x = n - 1 interpolates to the last node with weight 1
+    f[k] = x - (float)i[k];
+  }
+  const float* p = T.t + ((size_t)i[2] * n + i[1]) * n + i[0];
+  const size_t s1 = n, s2 = (size_t)n * n;
+  const float c00 = (1 - f[0]) * p[0] + f[0] * p[1];
+  const float c01 = (1 - f[0]) * p[s1] + f[0] * p[s1 + 1];
+  const float c10 = (1 - f[0]) * p[s2] + f[0] * p[s2 + 1];
+  const float c11 = (1 - f[0]) * p[s2 + s1] + f[0] * p[s2 + s1 + 1];
+  const float c0 = (1 - f[1]) * c00 + f[1] * c01;
+  const float c1 = (1 - f[1]) * c10 + f[1] * c11;
+  const float m = fminf(sqrtf((float)(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])), 1.0f);
+  return ((1 - f[2]) * c0 + f[2] * c1) * (m * m * m) * 1.44269504f;
+}
+
 // Halley iteration on t(s) = tau inside a bracket; returns s, *evals = evaluations of t, and in
 // *last the G functions at the last point where t was evaluated.  The first step is taken apart
 // from the loop: nearly every pair converges in it, and it needs no bracket bookkeeping.
@@ -502,7 +541,14 @@ BIA_HDF double kep_halley(const KepPair& K, const KepG& g, double* F, double* es
 // the root they are finite): the sign of inf - inf or of the overflowed sum says nothing.
 BIA_HDF bool above(const KepPair& K, double f) { return isfinite(f) ? f >= 0 : K.tau > 0; }
 
-BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool* ok, KepG* last) {
+// With MODE == kLearn, a pair whose first Halley correction from the series start is over a tenth of
+// s (the start was far off) takes the learned table's start as its second point instead of the
+// Halley step, when that start lies inside the bracket; every other pair runs exactly as in the
+// guess mode, and the series start stays the fallback.  *fl gets kLearned when the table's start
+// is taken.
+template <int MODE>
+BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool* ok, KepG* last,
+                         const LearnTable* T = nullptr, unsigned* fl = nullptr) {
   constexpr double kInf = __builtin_huge_val();
   double lo, hi;
   float dx, dxold = __builtin_huge_valf();  // the last two step lengths (only compared)
@@ -525,8 +571,21 @@ BIA_HDF double kep_solve(const KepPair& K, double s, int kmax, int* evals, bool*
     else lo = s;
     const bool take = valid && sn > lo && sn < hi;
     dx = take ? (float)fabs(d) : __builtin_huge_valf();
+    const double s0 = s;
     s = take ? sn : (hi == kInf) ? 2 * s + K.ir0 * fabs(K.tau)
                   : (lo == -kInf) ? 2 * s - K.ir0 * fabs(K.tau) : 0.5 * (lo + hi);
+    if (MODE == kLearn && T && !(fabs(d) <= 0.1 * fabs(s0))) {
+      // the network is trained on targets |log(s / s_start)| <= 4 (a ratio within e^4); the
+      // proposal is held to a ratio of 2^6 so that no table can start the solver far away
+      double a[3];
+      kep_features(K, a);
+      const double sl = s0 * (double)exp2f(fminf(fmaxf(learned_log2_ratio(*T, a), -6.0f), 6.0f));
+      if (sl > lo && sl < hi) {
+        dx = (float)fabs(sl - s0);
+        s = sl;
+        *fl |= kLearned;
+      }
+    }
   }
   int k = 1;
   while (k < kmax) {
@@ -608,45 +667,6 @@ BIA_HDF void kep_output(const KepPair& K, double s, const KepG& e, const double 
   }
 }
 
-// learned start: trilinear lookup of the target log(s / s_start) / min(|a|, 1)^3 in a table
-// over the squashed features u_k = sign(a_k) log2(1 + |a_k| / kFeatScale) / log2(1 + kFeatMax /
-// kFeatScale), clamped to [-1, 1].  The logarithm is taken in single precision (one hardware
-// instruction): the features index a table of 64 nodes per axis.
-constexpr double kFeatScale = 1e-2;
-constexpr double kFeatMax = 1e2;
-constexpr float kFeatNorm = 13.287856641840545f;  // log2(1 + kFeatMax / kFeatScale)
-constexpr double kWeak = 1e-2;  // all |a_k| below this: the series start (one evaluation)
-
-BIA_HDF float feat_squash(double a) {
-  const float u = fminf(log2f(1.0f + (float)(fabs(a) * (1 / kFeatScale))) * (1.0f / kFeatNorm), 1.0f);
-  return copysignf(u, (float)a);
-}
-
-// log2(s / s_start) proposed by the table, in single precision (the start only has to be close
-// to the root): trilinear lookup of the target at the squashed features, times min(|a|, 1)^3 and
-// log2(e)
-BIA_HDF float learned_log2_ratio(const LearnTable& T, const double a[3]) {
-  const int n = T.n;
-  float f[3];
-  int i[3];
-  for (int k = 0; k < 3; k++) {
-    const float x = fminf(fmaxf((feat_squash(a[k]) + 1.0f) * 0.5f * (float)(n - 1), 0.0f), (float)(n - 1));
-    const int ik = (int)x;
-    i[k] = ik < n - 2 ? ik : n - 2;  // x = n - 1 interpolates to the last node with weight 1
-    f[k] = x - (float)i[k];
-  }
-  const float* p = T.t + ((size_t)i[2] * n + i[1]) * n + i[0];
-  const size_t s1 = n, s2 = (size_t)n * n;
-  const float c00 = (1 - f[0]) * p[0] + f[0] * p[1];
-  const float c01 = (1 - f[0]) * p[s1] + f[0] * p[s1 + 1];
-  const float c10 = (1 - f[0]) * p[s2] + f[0] * p[s2 + 1];
-  const float c11 = (1 - f[0]) * p[s2 + s1] + f[0] * p[s2 + s1 + 1];
-  const float c0 = (1 - f[1]) * c00 + f[1] * c01;
-  const float c1 = (1 - f[1]) * c10 + f[1] * c11;
-  const float m = fminf(sqrtf((float)(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])), 1.0f);
-  return ((1 - f[2]) * c0 + f[2] * c1) * (m * m * m) * 1.44269504f;
-}
-
 template <int MODE>
 BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6], int* branch, int* iters,
                          unsigned* fl, const LearnTable& T, double* resid = nullptr) {
@@ -656,19 +676,10 @@ BIA_HDF void kepler_pair(double eps, double dt, const double rel[6], double d[6]
   double a[3];
   kep_features(K, a);
   double s = kep_series_start(K, a);
-  if (MODE == kLearn && fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2]))) >= kWeak) {
-    // the network is trained on targets |log(s / s_start)| <= 4 (a ratio within e^4); the
-    // proposal is held to a ratio of 2^6 so that no table can start the solver far away
-    const double sl = s * (double)exp2f(fminf(fmaxf(learned_log2_ratio(T, a), -6.0f), 6.0f));
-    if (isfinite(sl) && sl * s > 0) {
-      s = sl;
-      *fl |= kLearned;
-    }
-  }
   bool ok;
   int ev = 0;
   KepG last;
-  s = kep_solve(K, s, 60, &ev, &ok, &last);
+  s = kep_solve<MODE>(K, s, 60, &ev, &ok, &last, &T, fl);
   if (!ok) {
     *fl |= kItrCap;
     if (resid) {

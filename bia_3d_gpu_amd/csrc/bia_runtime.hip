@@ -28,16 +28,29 @@ void legacy_fail(const char* where, int code, bia_ctx* ctx) {
   abort();
 }
 
-int prof_begin(bia_ctx* ctx) {
+int prof_begin(bia_ctx* ctx, int ring) {
   if (!ctx->profiling) return BIA_OK;
-  BIA_HIP_TRY(ctx, hipEventRecord(ctx->ev_begin[ctx->ev_count % kProfSlots], ctx->stream));
+  BIA_HIP_TRY(ctx, hipEventRecord(ctx->ev_begin[ring][ctx->ev_count[ring] % kProfSlots], ctx->stream));
   return BIA_OK;
 }
 
-int prof_end(bia_ctx* ctx) {
+int prof_end(bia_ctx* ctx, int ring) {
   if (!ctx->profiling) return BIA_OK;
-  BIA_HIP_TRY(ctx, hipEventRecord(ctx->ev_end[ctx->ev_count % kProfSlots], ctx->stream));
-  ctx->ev_count++;
+  BIA_HIP_TRY(ctx, hipEventRecord(ctx->ev_end[ring][ctx->ev_count[ring] % kProfSlots], ctx->stream));
+  ctx->ev_count[ring]++;
+  return BIA_OK;
+}
+
+// durations of the last min(max, kProfSlots) launches recorded in a ring, oldest first
+static int ring_ms(bia_ctx* c, int ring, float* ms, int max, int* n) {
+  const long long avail = std::min<long long>(c->ev_count[ring], kProfSlots);
+  const int cnt = (int)std::min<long long>(avail, max);
+  for (int i = 0; i < cnt; i++) {
+    const int k = (int)((c->ev_count[ring] - cnt + i) % kProfSlots);
+    BIA_HIP_TRY(c, hipEventSynchronize(c->ev_end[ring][k]));
+    BIA_HIP_TRY(c, hipEventElapsedTime(ms + i, c->ev_begin[ring][k], c->ev_end[ring][k]));
+  }
+  *n = cnt;
   return BIA_OK;
 }
 
@@ -94,10 +107,11 @@ int bia_destroy(bia_ctx* c) {
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->graph) (void)hipGraphDestroy(c->graph);
-  for (int k = 0; k < bia::kProfSlots; k++) {
-    if (c->ev_begin[k]) (void)hipEventDestroy(c->ev_begin[k]);
-    if (c->ev_end[k]) (void)hipEventDestroy(c->ev_end[k]);
-  }
+  for (int r = 0; r < 2; r++)
+    for (int k = 0; k < bia::kProfSlots; k++) {
+      if (c->ev_begin[r][k]) (void)hipEventDestroy(c->ev_begin[r][k]);
+      if (c->ev_end[r][k]) (void)hipEventDestroy(c->ev_end[r][k]);
+    }
   delete c;
   return BIA_OK;
 }
@@ -113,39 +127,38 @@ int bia_set_profiling(bia_ctx* c, int on) {
   if (!c) return BIA_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
   BIA_HIP_TRY(c, hipSetDevice(c->device));
-  if (on && !c->ev_begin[0]) {
-    for (int k = 0; k < bia::kProfSlots; k++) {
-      BIA_HIP_TRY(c, hipEventCreate(&c->ev_begin[k]));
-      BIA_HIP_TRY(c, hipEventCreate(&c->ev_end[k]));
-    }
+  if (on && !c->ev_begin[0][0]) {
+    for (int r = 0; r < 2; r++)
+      for (int k = 0; k < bia::kProfSlots; k++) {
+        BIA_HIP_TRY(c, hipEventCreate(&c->ev_begin[r][k]));
+        BIA_HIP_TRY(c, hipEventCreate(&c->ev_end[r][k]));
+      }
   }
   c->profiling = on ? 1 : 0;
-  c->ev_count = 0;
+  c->ev_count[0] = c->ev_count[1] = 0;
   return BIA_OK;
 }
 
 int bia_last_kernel_ms(bia_ctx* c, float* ms) {
   if (!c || !ms) return BIA_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
-  if (c->ev_count == 0) return BIA_ERR_STATE;
-  const int k = (int)((c->ev_count - 1) % bia::kProfSlots);
-  BIA_HIP_TRY(c, hipEventSynchronize(c->ev_end[k]));
-  BIA_HIP_TRY(c, hipEventElapsedTime(ms, c->ev_begin[k], c->ev_end[k]));
+  if (c->ev_count[0] == 0) return BIA_ERR_STATE;
+  const int k = (int)((c->ev_count[0] - 1) % bia::kProfSlots);
+  BIA_HIP_TRY(c, hipEventSynchronize(c->ev_end[0][k]));
+  BIA_HIP_TRY(c, hipEventElapsedTime(ms, c->ev_begin[0][k], c->ev_end[0][k]));
   return BIA_OK;
 }
 
 int bia_kernel_ms(bia_ctx* c, float* ms, int max, int* n) {
   if (!c || !ms || !n || max < 0) return BIA_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
-  const long long avail = std::min<long long>(c->ev_count, bia::kProfSlots);
-  const int cnt = (int)std::min<long long>(avail, max);
-  for (int i = 0; i < cnt; i++) {  // oldest first
-    const int k = (int)((c->ev_count - cnt + i) % bia::kProfSlots);
-    BIA_HIP_TRY(c, hipEventSynchronize(c->ev_end[k]));
-    BIA_HIP_TRY(c, hipEventElapsedTime(ms + i, c->ev_begin[k], c->ev_end[k]));
-  }
-  *n = cnt;
-  return BIA_OK;
+  return bia::ring_ms(c, 0, ms, max, n);
+}
+
+int bia_reduce_kernel_ms(bia_ctx* c, float* ms, int max, int* n) {
+  if (!c || !ms || !n || max < 0) return BIA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return bia::ring_ms(c, 1, ms, max, n);
 }
 
 int bia_synchronize(bia_ctx* c) {

@@ -149,8 +149,9 @@ struct bia_ctx {
   // pair-kernel timing: event pairs recorded around the last kProfSlots launches (a ring), and
   // the launches recorded since profiling was switched on
   int profiling = 0;
-  hipEvent_t ev_begin[bia::kProfSlots] = {}, ev_end[bia::kProfSlots] = {};
-  long long ev_count = 0;
+  // ring 0: the pair kernel; ring 1: the tile-partial reduction kernel of the same steps
+  hipEvent_t ev_begin[2][bia::kProfSlots] = {}, ev_end[2][bia::kProfSlots] = {};
+  long long ev_count[2] = {0, 0};
   char last_error[256] = {0};
 };
 
@@ -181,8 +182,8 @@ bia_ctx* default_ctx();
 // abort with a message: the legacy symbols have no error return, so failures are loud
 [[noreturn]] void legacy_fail(const char* where, int code, bia_ctx* ctx);
 // record the profiling events around the pair kernel (no-ops unless profiling is on)
-int prof_begin(bia_ctx* ctx);
-int prof_end(bia_ctx* ctx);
+int prof_begin(bia_ctx* ctx, int ring = 0);
+int prof_end(bia_ctx* ctx, int ring = 0);
 // waits for the context's stream when it goes out of scope, so that a call returning early on
 // an error leaves no copy pending from or into host memory that the caller may then release
 struct StreamDrain {

@@ -565,9 +565,11 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
     hipLaunchKernelGGL(k_stats_fold, dim3(BIA_STATS_LEN), dim3(256), 0, ctx->stream, slots, stats_out);
     BIA_HIP_TRY(ctx, hipGetLastError());
   }
+  BIA_TRY(ctx, prof_begin(ctx, 1));
   hipLaunchKernelGGL(k_reduce, dim3(P.nchunks, 6), dim3(256), 0, ctx->stream, ctx->plan_chunks.as<int2>(), sys, a.ld,
                      a.ld, a.dt, a.d_state, part, a.d_delta);
   BIA_HIP_TRY(ctx, hipGetLastError());
+  BIA_TRY(ctx, prof_end(ctx, 1));
   return BIA_OK;
 }
 

@@ -44,7 +44,7 @@ extern "C" {
 
 /* solver modes of the analytic method (BIA_METHOD_ALG) */
 #define BIA_MODE_PLAIN 0  /* the reference's conic solvers, starting guesses and iteration control */
-#define BIA_MODE_LEARN 1  /* universal-variable Kepler drift; pairs not weak over dt start from the learned table */
+#define BIA_MODE_LEARN 1  /* the Kepler drift; a pair whose series start is far off takes its second point from the learned table */
 #define BIA_MODE_GUESS 2  /* the same Kepler drift, every pair from its series start (no network) */
 
 /* pair couplings of the batched entry points */
@@ -121,6 +121,9 @@ int bia_last_kernel_ms(bia_ctx *ctx, float *ms);
  * profiling was switched on, oldest first (waits for them); *n receives how many were written.
  * For timing steps enqueued back to back, with no synchronisation per step. */
 int bia_kernel_ms(bia_ctx *ctx, float *ms, int max, int *n);
+/* The same for the step's second kernel, the fixed-order reduction of the pair kernel's tile
+ * partials into the deltas (an HBM-bound pass; its bytes per launch are the partials' size). */
+int bia_reduce_kernel_ms(bia_ctx *ctx, float *ms, int max, int *n);
 
 /* One whole BIA step of every unordered pair of an n-particle system.  No counterpart in the
  * reference ABI: it replaces the per-particle loop of bia/bia.f90:128-202 (N calls of
@@ -243,7 +246,10 @@ int bia_dim_rkf45_coupled_host(bia_ctx *ctx, int coupling, double coup, int np, 
 /* learned initialiser (no counterpart in the reference)                                     */
 /* ---------------------------------------------------------------------------------------- */
 /* A 3 -> 64 -> 64 -> 1 GELU network proposes the starting universal anomaly of the Kepler drift
- * (BIA_MODE_LEARN) for pairs that are not weak over dt.  Inputs: the squashed dimensionless
+ * (BIA_MODE_LEARN): a pair whose first Halley correction from the rational series start exceeds a
+ * tenth of the anomaly (the start was far off) takes the table's proposal as its second point,
+ * inside the bracket the first evaluation set; every other pair runs as in BIA_MODE_GUESS.
+ * Inputs: the squashed dimensionless
  * numbers u_k = sign(a_k) log2(1 + |a_k| / 1e-2) / log2(1 + 1e4) (single precision) of
  * a1 = eta dt / r0^2, a2 = zeta dt^2 / r0^3,
  * a3 = beta dt^2 / r0^2 (eta = r.v, beta = 2 mu / r0 - v^2, zeta = mu - beta r0, mu = -eps);

@@ -56,8 +56,9 @@ def test_failure_log_records_the_capped_pairs(ctx):
     reference Newton stops at its iteration cap on many bound pairs: the log counts exactly the
     pairs the statistics count with the cap, non-finite or RKF failure bits, writes one record per
     pair (each i < j, no pair twice) up to its capacity, and the oracle (the same algorithm on the
-    CPU) reports the cap on the recorded pairs.  The Kepler drift logs none on the same box, and
-    a log smaller than the count keeps counting."""
+    CPU) reports the cap on most recorded pairs; the others are pairs whose best residual sits
+    within a few rounding units of the tolerance, where one libm rounding decides.  The Kepler
+    drift logs none on the same box, and a log smaller than the count keeps counting."""
     import torch
     from bia_3d_gpu_amd import COUPLING_COULOMB, MODE_GUESS, MODE_PLAIN, STATS_LEN, FailureLog, ic
     n, dt = 2048, 5e-4
@@ -84,18 +85,31 @@ def test_failure_log_records_the_capped_pairs(ctx):
     assert (rec["i"] < rec["j"]).all() and (rec["sys"] == 0).all()
     assert len(np.unique(rec["i"].astype(np.int64) * n + rec["j"])) == total
     assert ((rec["flags"] & 4) != 0).sum() == stats[10]
-    # the oracle on a sample of the recorded pairs
-    capped = 0
+    # a capped pair's best residual is at or above the solver's tolerance (2e-16 for the ellipse,
+    # 2.22e-16 for the hyperbola, dimensionless)
+    capped_rec = rec[(rec["flags"] & 4) != 0]
+    assert (np.abs(capped_rec["error"]) >= 2e-16).all()
+    # the oracle on a sample of the recorded pairs: the same branch; it stops at the cap too, or
+    # the pair is one whose best residual sits within a few rounding units of the tolerance
+    # (most of this box's capped pairs: whether the cap is reached there is decided by one libm
+    # rounding of the residual)
+    capped = near = 0
     sample = rec[:: max(1, len(rec) // 200)]
     for r in sample:
         i, j = int(r["i"]), int(r["j"])
         rel = st[:, i] - st[:, j]
         eps = q[i] * q[j] * 2.0
         _, br, _, fl = O.pair(1, eps, dt, rel)
-        capped += bool(fl & O.F_ITR_CAP)
         assert r["branch"] == br
-    print(f"oracle reports the cap on {capped} of {len(sample)} recorded pairs")
-    assert capped >= 0.9 * len(sample)
+        if fl & O.F_ITR_CAP:
+            capped += 1
+        else:
+            assert abs(r["error"]) < 8 * 2.22e-16, r
+            near += 1
+    print(f"oracle reports the cap on {capped} of {len(sample)} recorded pairs; the other {near} have best "
+          f"residuals within 8 rounding units of the tolerance; recorded residuals "
+          f"{np.abs(rec['error']).min():.2e} .. {np.abs(rec['error']).max():.2e}")
+    assert capped >= len(sample) // 2
     t_small, rec_small, _ = res["plain_small"]
     assert t_small == total and len(rec_small) == 100
     t_k, rec_k, stats_k = res["kepler"]
