@@ -48,6 +48,8 @@ SIGNATURES = {
     "bia_reduce_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_float), C.c_int, _ip]),
     "bia_step_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _vp, C.c_int,
                                _vp, _vp]),
+    "bia_step_noalloc_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _vp,
+                                       C.c_int, _vp]),
     "bia_step_host": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "bia_step_batched_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _ip32, _dp, C.c_double, C.c_double,

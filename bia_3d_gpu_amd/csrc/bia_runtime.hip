@@ -82,6 +82,9 @@ int bia_create(bia_ctx** out, int device) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BIA_ERR_HIP;
   if (device < 0 || device >= ndev) return BIA_ERR_ARG;
   if (hipSetDevice(device) != hipSuccess) return BIA_ERR_HIP;
+  // the no-allocation step's scratch address on this device; a failure is reported by that
+  // entry point (BIA_ERR_STATE), the others do not need it
+  (void)bia::noalloc_scratch_init(device);
   bia_ctx* c = new bia_ctx();
   c->device = device;
   c->stream = nullptr;

@@ -182,6 +182,9 @@ bia_ctx* default_ctx();
 // abort with a message: the legacy symbols have no error return, so failures are loud
 [[noreturn]] void legacy_fail(const char* where, int code, bia_ctx* ctx);
 // record the profiling events around the pair kernel (no-ops unless profiling is on)
+// with `device` current: keep the device address of the no-allocation step's scratch array
+// (bia_step.hip), once per device
+int noalloc_scratch_init(int device);
 int prof_begin(bia_ctx* ctx, int ring = 0);
 int prof_end(bia_ctx* ctx, int ring = 0);
 // waits for the context's stream when it goes out of scope, so that a call returning early on

@@ -505,23 +505,39 @@ static int validate(bia_ctx* ctx, const StepArgs& a) {
   return BIA_OK;
 }
 
-// enqueue pair kernel + reduction on ctx->stream (no synchronisation, no allocation once planned)
-static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int occ_override = 0) {
-  const Plan& P = ctx->plan;
-  const auto tuned = tuned_for(ctx, a);
+// the device plan a launch reads: the context's buffers, or the no-allocation scratch array
+struct PlanView {
+  const int4* tiles;
+  const SysDesc* sys;
+  const int2* chunks;
+  double* part;
+  int ntiles, nchunks;
+  bool context_plan;  // the context's own plan: statistics, failure records and autotuned grids apply
+};
+
+// enqueue pair kernel + reduction on ctx->stream (no synchronisation, no allocation once planned);
+// view NULL: the context's plan
+static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int occ_override = 0,
+                       const PlanView* view = nullptr) {
+  const PlanView V = view ? *view
+                          : PlanView{ctx->plan_tiles.as<int4>(), ctx->plan_sys.as<SysDesc>(), ctx->plan_chunks.as<int2>(),
+                                     ctx->part.as<double>(), ctx->plan.ntiles, ctx->plan.nchunks, true};
+  const auto tuned = V.context_plan ? tuned_for(ctx, a) : std::pair<int, int>(0, 1);
   const int tuned_grid = tuned.first, tuned_occ = tuned.second;
   const int occ = occ_override > 0 ? occ_override : tuned_occ;
   const LearnTable T{ctx->mlp_table.as<float>(), ctx->table_n};
-  int grid = grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : P.ntiles);
+  int grid = grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : V.ntiles);
   // a dispatch holds fewer than 2^32 work-items; the pair kernel is grid-stride, so a grid capped
   // there still covers every tile (one system of more than about 741k particles)
-  grid = std::min(grid, std::min(P.ntiles, kMaxGrid64));
-  const int4* tiles = ctx->plan_tiles.as<int4>();
-  const SysDesc* sys = ctx->plan_sys.as<SysDesc>();
-  double* part = ctx->part.as<double>();
+  grid = std::min(grid, std::min(V.ntiles, kMaxGrid64));
+  const int4* tiles = V.tiles;
+  const SysDesc* sys = V.sys;
+  double* part = V.part;
   // a failure log needs the statistics variant of the pair kernel, which writes the records; its
   // counters go to the caller's array or, without one, to the context's own
-  const PairLog lg{static_cast<PairRecord*>(ctx->log_rec), ctx->log_count, ctx->log_cap};
+  const PairLog lg = V.context_plan ? PairLog{static_cast<PairRecord*>(ctx->log_rec), ctx->log_count, ctx->log_cap}
+                                    : PairLog{nullptr, nullptr, 0};
+  if (!V.context_plan && a.d_stats) return BIA_ERR_ARG;
   unsigned long long* stats_out = a.d_stats ? a.d_stats : lg.count ? ctx->log_stats.as<unsigned long long>() : nullptr;
   const bool st = stats_out != nullptr;
   unsigned long long* slots = ctx->stat_slots.as<unsigned long long>();
@@ -529,7 +545,7 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   const RkfTol tol{ctx->rkf_rel, ctx->rkf_abs};
   BIA_TRY(ctx, prof_begin(ctx));
 #define L_OCC(M, MO, CP, S, O)                                                                                     \
-  hipLaunchKernelGGL((k_tiles<M, MO, CP, S, O>), dim3(grid), dim3(kWave), 0, ctx->stream, P.ntiles, a.ld, tiles, sys, \
+  hipLaunchKernelGGL((k_tiles<M, MO, CP, S, O>), dim3(grid), dim3(kWave), 0, ctx->stream, V.ntiles, a.ld, tiles, sys, \
                      a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, slots, T, tol, lg)
 #define L_(M, MO, CP, S)                                    \
   do {                                                      \
@@ -566,8 +582,8 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
     BIA_HIP_TRY(ctx, hipGetLastError());
   }
   BIA_TRY(ctx, prof_begin(ctx, 1));
-  hipLaunchKernelGGL(k_reduce, dim3(P.nchunks, 6), dim3(256), 0, ctx->stream, ctx->plan_chunks.as<int2>(), sys, a.ld,
-                     a.ld, a.dt, a.d_state, part, a.d_delta);
+  hipLaunchKernelGGL(k_reduce, dim3(V.nchunks, 6), dim3(256), 0, ctx->stream, V.chunks, sys, a.ld, a.ld, a.dt,
+                     a.d_state, part, a.d_delta);
   BIA_HIP_TRY(ctx, hipGetLastError());
   BIA_TRY(ctx, prof_end(ctx, 1));
   return BIA_OK;
@@ -704,6 +720,68 @@ static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, 
   return BIA_OK;
 }
 
+// ------------------------------------------------------------------------------------------------
+// no-allocation step (bia_step_noalloc_dev).  The plan and the tile partials of one system of at
+// most kNoAllocMaxN particles live in the file-scope device array `scratch` (48.2 MiB, held by
+// every GPU the library runs on until the process ends), so the call allocates nothing.  In host
+// code `scratch` names a host-side stand-in, not device memory: its device address on each GPU
+// comes from hipGetSymbolAddress, once per device at bia_create, and only that address is used.
+// ------------------------------------------------------------------------------------------------
+constexpr int kNoAllocMaxN = 8192;
+constexpr int kNoAllocMaxNt = kNoAllocMaxN / kWave;
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kOffTiles = 0;
+constexpr size_t kOffChunks = align256(kOffTiles + (size_t)kNoAllocMaxNt * (kNoAllocMaxNt + 1) / 2 * sizeof(int4));
+constexpr size_t kOffSys = align256(kOffChunks + (size_t)(kNoAllocMaxN + 255) / 256 * sizeof(int2));
+constexpr size_t kOffPart = align256(kOffSys + sizeof(SysDesc));
+constexpr size_t kScratchBytes = kOffPart + (size_t)6 * kNoAllocMaxNt * kNoAllocMaxNt * kWave * sizeof(double);
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kMaxDevices] = {};  // device address of `scratch` on each GPU
+
+// with `device` current: look up and keep this GPU's address of the scratch array (once)
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return BIA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return BIA_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    return BIA_ERR_HIP;
+  }
+  g_scratch[device] = ptr;
+  return BIA_OK;
+}
+
+static void* scratch_on(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[device];
+}
+
+// the plan of one n-particle system, as ensure_plan lists it: tiles (I, J), I <= J, row by row;
+// reduction chunks of 256 particles; the system descriptor.  Rewritten on every call (a fill
+// kernel: a copy or memset node would not be safe to capture in a graph).
+__global__ __launch_bounds__(256) void k_noalloc_plan(int n, int nt, double eps, int4* __restrict__ tiles,
+                                                      int2* __restrict__ chunks, SysDesc* __restrict__ sys) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ntiles = nt * (nt + 1) / 2;
+  if (t < ntiles) {
+    int I = 0, first = 0;  // row I holds tiles [first, first + nt - I)
+    while (first + (nt - I) <= t) {
+      first += nt - I;
+      ++I;
+    }
+    tiles[t] = make_int4(I, I + (t - first), 0, 0);
+  }
+  if (t < (n + 255) / 256) chunks[t] = make_int2(0, 256 * t);
+  if (t == 0) *sys = SysDesc{0, n, nt, 0, 0LL, eps};
+}
+
 static_assert(sizeof(PairRecord) == sizeof(bia_pair_record) && offsetof(PairRecord, error) ==
                   offsetof(bia_pair_record, error), "the device record is the public one");
 
@@ -750,6 +828,35 @@ int bia_run_steps_dev(bia_ctx* ctx, int method, int mode, int coupling, int nsys
   if (nsteps < 0) return BIA_ERR_ARG;
   StepArgs a{method, mode, coupling, nsys, sizes, eps, coup, dt, itr, d_state, ld, d_mass, d_charge, d_delta, d_stats};
   return run_steps_locked(ctx, a, d_state, nsteps, use_graph);
+}
+
+int bia_step_noalloc_dev(bia_ctx* ctx, int method, int mode, int n, double eps, double dt, int itr,
+                         const double* d_state, int ld, double* d_delta) {
+  BIA_LOCKED(ctx);
+  if (n < 1 || n > kNoAllocMaxN) {
+    snprintf(ctx->last_error, sizeof(ctx->last_error), "n = %d: the no-allocation step takes 1 <= n <= %d", n,
+             kNoAllocMaxN);
+    return BIA_ERR_ARG;
+  }
+  StepArgs a{method, mode, BIA_COUPLING_SCALAR, 1, &n, &eps, 0.0, dt, itr, d_state, ld, nullptr, nullptr, d_delta,
+             nullptr};
+  BIA_TRY(ctx, validate(ctx, a));
+  unsigned char* base = static_cast<unsigned char*>(scratch_on(ctx->device));
+  if (!base) {
+    snprintf(ctx->last_error, sizeof(ctx->last_error), "no scratch address for device %d", ctx->device);
+    return BIA_ERR_STATE;
+  }
+  const int nt = (n + kWave - 1) / kWave;
+  const int ntiles = nt * (nt + 1) / 2, nchunks = (n + 255) / 256;
+  int4* tiles = reinterpret_cast<int4*>(base + kOffTiles);
+  int2* chunks = reinterpret_cast<int2*>(base + kOffChunks);
+  SysDesc* sys = reinterpret_cast<SysDesc*>(base + kOffSys);
+  double* part = reinterpret_cast<double*>(base + kOffPart);
+  hipLaunchKernelGGL(k_noalloc_plan, dim3((ntiles + 255) / 256), dim3(256), 0, ctx->stream, n, nt, eps, tiles, chunks,
+                     sys);
+  BIA_HIP_TRY(ctx, hipGetLastError());
+  const PlanView V{tiles, sys, chunks, part, ntiles, nchunks, false};
+  return launch_step(ctx, a, 0, 0, &V);
 }
 
 int bia_set_rkf_tolerance(bia_ctx* ctx, double relerr, double abserr) {

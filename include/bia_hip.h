@@ -135,6 +135,17 @@ int bia_reduce_kernel_ms(bia_ctx *ctx, float *ms, int max, int *n);
 int bia_step_dev(bia_ctx *ctx, int method, int mode, int n, double eps, double dt, int itr,
                  const double *d_state, int ld, double *d_delta, unsigned long long *d_stats);
 
+/* The step of bia_step_dev (same kernels, bitwise the same deltas) that allocates nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).  Its plan and tile
+ * partials live in a library-owned static device array of 48.1 MiB that each GPU the library
+ * runs on holds until the process ends, which limits n to 1 <= n <= 8192 (BIA_ERR_ARG beyond,
+ * d_delta untouched).  No statistics and no failure records; the context's autotuned grids do
+ * not apply.  Every call on one GPU shares that GPU's array (from any context): calls that
+ * overlap on one GPU race, so order them on one stream. */
+int bia_step_noalloc_dev(bia_ctx *ctx, int method, int mode, int n, double eps, double dt, int itr,
+                         const double *d_state, int ld, double *d_delta);
+
 /* Batched step: nsys independent systems of uneven sizes in one persistent launch.  The systems
  * are concatenated column-wise in one SoA state [6][ld] (system s occupies the next sizes[s]
  * columns); d_delta has the same layout.  No counterpart in the reference ABI (its only ensemble
