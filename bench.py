@@ -93,8 +93,10 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kms = ctx.kernel_ms()  # the pair kernel of each timed step (HIP events on the library's stream)
-    rms = ctx.reduce_kernel_ms()  # the tile-partial reduction of the same steps
+    # the pair kernel and the tile-partial reduction of each timed step (HIP events on the
+    # library's stream), summed over the stages of a step
+    kms, red_ms = step_kernel_ms(ctx)
+    nstages, wbytes = ctx.plan_info()
     ctx.set_profiling(False)
     ms = e0.elapsed_time(e1) / steps
     inv1 = torch.zeros(11, dtype=torch.float64, device=dev)
@@ -109,19 +111,28 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
     inv_ms = i0.elapsed_time(i1) / 5
     a, b = inv0.cpu().numpy(), inv1.cpu().numpy()
     e_0, e_1 = a[0] + a[1], b[0] + b[1]  # KE + PE = 2E/m
-    kms = float(np.mean(kms))
     n = s.shape[1]
     nt = (n + 63) // 64
     # the reduction reads every tile partial once (6 components x nt tiles x 64 nt slots) and the
-    # velocities, and writes the deltas
-    red_bytes = 8 * 6 * nt * nt * 64 + 24 * n + 48 * n
-    red_ms = float(np.mean(rms)) if rms else float("nan")
+    # velocities, and writes the deltas (once per stage, but the last, for the running sums)
+    red_bytes = 8 * 6 * nt * nt * 64 + 24 * n + 48 * n + 2 * 48 * n * max(0, nstages - 1)
     return dict(ms_per_step=ms, wall_s=wall, kernel_ms=kms, stats=st.tolist(), flops=flops, tuned=tuned,
                 tflops=flops / (kms / 1e3) / 1e12, dE_rel=abs(e_1 - e_0) / abs(e_0), dE_steps=steps,
                 first_delta=first_delta,
-                reduction={"kernel_ms": red_ms, "bytes_per_launch": red_bytes,
-                           "GB_per_s": red_bytes / (red_ms * 1e-3) / 1e9},
+                reduction={"kernel_ms": red_ms, "bytes_per_step": red_bytes, "stages": nstages,
+                           "workspace_bytes": wbytes, "GB_per_s": red_bytes / (red_ms * 1e-3) / 1e9},
                 invariants={"ms": inv_ms, "state_bytes": 48 * n, "GB_per_s_of_state": 48 * n / (inv_ms * 1e-3) / 1e9})
+
+
+def step_kernel_ms(ctx):
+    """(pair kernel, reduction) milliseconds per step, summed over the step's stages, from the most
+    recent whole steps in the library's timing rings."""
+    kms, rms = ctx.kernel_ms(), ctx.reduce_kernel_ms()
+    nstages, _ = ctx.plan_info()
+    whole = (min(len(kms), len(rms)) // nstages) * nstages
+    if whole == 0:
+        return float("nan"), float("nan")
+    return float(np.sum(kms[-whole:]) / (whole / nstages)), float(np.sum(rms[-whole:]) / (whole / nstages))
 
 
 def agreement(st_host, d, ref):
@@ -245,7 +256,7 @@ def run_fixture(ctx, steps: int, learned: bool):
             ctx.step_(s, d, -6e-7, 1.0, method=method, mode=mode)
         torch.cuda.synchronize()
         sec = (time.perf_counter() - t0) / k
-        kms = float(np.mean(ctx.kernel_ms()))
+        kms = step_kernel_ms(ctx)[0]
         ctx.set_profiling(False)
         out[name] = {"s_per_step": sec, "pairs_per_s": pairs / sec, "kernel_ms": kms,
                      "evaluations_per_pair": float(c[4:8].sum() / max(1, c[0:4].sum())), "learned_starts": int(c[14]),
@@ -363,7 +374,7 @@ def run_learned_gain(ctx, st, eps, dts=(1e-2, 1e-1), steps: int = 5):
             ctx.set_profiling(True)
             for _ in range(steps):
                 ctx.step_(s, d, eps, dt, mode=mode)
-            kms = float(np.median(ctx.kernel_ms()))
+            kms = step_kernel_ms(ctx)[0]
             ctx.set_profiling(False)
             row[name] = {"kernel_ms": kms, "evaluations_per_pair": float(cs[4:8].sum() / cs[0:4].sum()),
                          "learned_starts": int(cs[14])}
@@ -480,6 +491,8 @@ def main():
                     help="comma-separated subset of plain,guess,learned (profiling runs)")
     ap.add_argument("--copy-mb", type=float, default=0.0,
                     help="also run one device copy of this many MB (HBM counter calibration)")
+    ap.add_argument("--workspace-mb", type=float, default=None,
+                    help="tile-partial workspace limit of a step in MB (0 = none; default: the library's 1 GiB)")
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-steps", type=int, default=3000)
     args = ap.parse_args()
@@ -505,6 +518,8 @@ def main():
             f"{cpu['cpu_model']}, {cpu['compiler']} {cpu['flags']})")
     assert torch.cuda.is_available()
     ctx = Context(0)
+    if args.workspace_mb is not None:
+        ctx.set_workspace_limit(int(args.workspace_mb * 1e6))
     modes = set(args.modes.split(","))
     if args.copy_mb > 0:  # a plain copy of known size for scaling HBM byte counters
         nb = int(args.copy_mb * 1e6) // 8

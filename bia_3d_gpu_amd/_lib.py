@@ -58,6 +58,8 @@ SIGNATURES = {
                                    C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.POINTER(C.c_float), _ip]),
     "bia_run_steps_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _ip32, _dp, C.c_double, C.c_double,
                                     C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "bia_set_workspace_limit": (C.c_int, [_vp, C.c_ulonglong]),
+    "bia_plan_info": (C.c_int, [_vp, _ip, C.POINTER(C.c_ulonglong)]),
     "bia_set_rkf_tolerance": (C.c_int, [_vp, C.c_double, C.c_double]),
     "bia_set_failure_log": (C.c_int, [_vp, _vp, C.c_int, _vp]),
     "bia_invariants_dev": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_int, _vp]),

@@ -283,15 +283,15 @@ class Context:
         check(load().bia_last_kernel_ms(self._h, C.byref(ms)), "bia_last_kernel_ms", self._h)
         return ms.value
 
-    def kernel_ms(self, max_launches: int = 64) -> list[float]:
+    def kernel_ms(self, max_launches: int = 256) -> list[float]:
         """Durations of the pair-kernel launches recorded since profiling was switched on (the
-        last 64 at most), oldest first; waits for them."""
+        last 256 at most), oldest first; waits for them."""
         buf = (C.c_float * max_launches)()
         n = C.c_int()
         check(load().bia_kernel_ms(self._h, buf, int(max_launches), C.byref(n)), "bia_kernel_ms", self._h)
         return [buf[i] for i in range(n.value)]
 
-    def reduce_kernel_ms(self, max_launches: int = 64) -> list[float]:
+    def reduce_kernel_ms(self, max_launches: int = 256) -> list[float]:
         """Durations of the tile-partial reduction kernel of the same steps, oldest first."""
         buf = (C.c_float * max_launches)()
         n = C.c_int()
@@ -301,6 +301,17 @@ class Context:
 
     def synchronize(self):
         check(load().bia_synchronize(self._h), "bia_synchronize", self._h)
+
+    def set_workspace_limit(self, nbytes: int = 1 << 30):
+        """Largest tile-partial workspace of a step (bytes, 0 = no limit); a single system beyond it
+        runs in stages of row tiles with the same deltas bitwise."""
+        check(load().bia_set_workspace_limit(self._h, int(nbytes)), "bia_set_workspace_limit", self._h)
+
+    def plan_info(self) -> tuple[int, int]:
+        """(stages, tile-partial workspace bytes) of the last step's plan."""
+        ns, wb = C.c_int(), C.c_ulonglong()
+        check(load().bia_plan_info(self._h, C.byref(ns), C.byref(wb)), "bia_plan_info", self._h)
+        return ns.value, wb.value
 
     def set_rkf_tolerance(self, relerr: float = 0.0, abserr: float = 1e-16):
         """Per-pair RKF45 tolerances of the following steps (the reference's 0 / 1e-16 by

@@ -115,6 +115,146 @@ struct Hist {
 
 BIA_HDF void sincos_(double x, double* s, double* c) { ::sincos(x, s, c); }
 
+struct double2_ {
+  double x, y;
+};
+
+// 1/x from the hardware reciprocal estimate r (relative error e = 1 - x r near 2^-24) and one
+// third-order correction r (1 + e + e^2), which leaves e^3 (within an ulp); 1/0 and non-finite x
+// give a non-finite result.  The host build divides.
+BIA_HDF double fast_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r = __builtin_amdgcn_rcp(x);
+  const double e = fma(-x, r, 1.0);
+  return fma(r, fma(e, e, e), r);
+#else
+  return 1.0 / x;
+#endif
+}
+
+// sin and cos of |x| <= pi in FP64 (J.-M. Muller, "Elementary Functions: Algorithms and
+// Implementation", 3rd ed., 2016: Cody-Waite reduction with a two-part constant, then the Taylor
+// series in Horner form): |x| - pi/2 and pi - |x| are exact where they are taken (Sterbenz) and
+// the constants' low parts restore pi/2 and pi to 2^-106; the reduced argument b lies in
+// [-pi/4, pi/4], where the series through b^19 / 19! and b^20 / 20! leave less than 1e-22.  Within
+// about an ulp of the correctly rounded values (tests/test_hostsim.py).  Other arguments go to
+// the library sincos.
+BIA_HDF void sincos_small(double b, double* s, double* c) {
+  const double z = b * b;
+  // sin b = b (1 - z/3! + z^2/5! - ... - z^9/19!)
+  double p = -1.0 / 121645100408832000.0;  // -1/19!
+  p = p * z + 1.0 / 355687428096000.0;     // 1/17!
+  p = p * z - 1.0 / 1307674368000.0;       // 1/15!
+  p = p * z + 1.0 / 6227020800.0;          // 1/13!
+  p = p * z - 1.0 / 39916800.0;            // 1/11!
+  p = p * z + 1.0 / 362880.0;              // 1/9!
+  p = p * z - 1.0 / 5040.0;                // 1/7!
+  p = p * z + 1.0 / 120.0;                 // 1/5!
+  p = p * z - 1.0 / 6.0;                   // 1/3!
+  *s = b + (b * z) * p;
+  // cos b = 1 - z/2 + z^2 (1/4! - z/6! + ... + z^8/20!)
+  double q = 1.0 / 2432902008176640000.0;  // 1/20!
+  q = q * z - 1.0 / 6402373705728000.0;    // 1/18!
+  q = q * z + 1.0 / 20922789888000.0;      // 1/16!
+  q = q * z - 1.0 / 87178291200.0;         // 1/14!
+  q = q * z + 1.0 / 479001600.0;           // 1/12!
+  q = q * z - 1.0 / 3628800.0;             // 1/10!
+  q = q * z + 1.0 / 40320.0;               // 1/8!
+  q = q * z - 1.0 / 720.0;                 // 1/6!
+  q = q * z + 1.0 / 24.0;                  // 1/4!
+  const double hz = 0.5 * z, w = 1.0 - hz;
+  // 1 - z/2 rounded once, then its rounding error and the series tail (Muller, sec. 11.3)
+  *c = w + (((1.0 - w) - hz) + (z * z) * q);
+}
+
+// the library routines for the arguments outside the ranges above, kept out of line (rare)
+BIA_HDN double2_ sincos_lib(double x) {
+  double2_ r;
+  ::sincos(x, &r.x, &r.y);
+  return r;
+}
+BIA_HDN double log_lib(double x) { return ::log(x); }
+
+BIA_HDF void sincos_pi(double x, double* s, double* c) {
+  constexpr double kPiHi = 3.141592653589793116, kPiLo = 1.2246467991473532e-16;
+  constexpr double kPio2Hi = 1.5707963267948966, kPio2Lo = 6.123233995736766e-17;
+  constexpr double kPio4 = 0.78539816339744830962, k3Pio4 = 2.35619449019234492885;
+  const double a = fabs(x);
+  if (!(a <= kPiHi)) {  // beyond pi, NaN: the library routine
+    const double2_ r = sincos_lib(x);
+    *s = r.x;
+    *c = r.y;
+    return;
+  }
+  // one reduction to |b| <= pi/4: a itself, a - pi/2 or pi - a; the difference with the
+  // constant's high part is exact, and adding its low part rounds once: that rounding, bl, is
+  // recovered exactly (Fast2Sum) and applied to first order, sin(b + bl) = sin b + bl cos b
+  const bool mid = a > kPio4 && a <= k3Pio4, top = a > k3Pio4;
+  const double bh = mid ? a - kPio2Hi : top ? kPiHi - a : a;
+  const double lo = mid ? -kPio2Lo : top ? kPiLo : 0.0;
+  const double b = bh + lo, bl = (bh - b) + lo;
+  double sb, cb;
+  sincos_small(b, &sb, &cb);
+  sb = fma(bl, cb, sb);
+  cb = fma(-bl, sb, cb);
+  // sin a = sin b, cos b, sin b; cos a = cos b, -sin b, -cos b
+  const double sa = mid ? cb : sb, ca = mid ? -sb : top ? -cb : cb;
+  *s = copysign(sa, x);
+  *c = ca;
+}
+
+// natural logarithm of a positive normal finite x in FP64 (Muller, 2016: x = 2^k m with m in
+// [sqrt(1/2), sqrt(2)), ln x = k ln 2 + 2 atanh(u), u = (m - 1)/(m + 1), |u| <= 0.1716, the
+// atanh series through u^23 / 23, leaving less than 1e-18 relative); m - 1 is exact (Sterbenz),
+// the quotient is corrected once with its exact residual, ln 2 is split in two parts so that
+// k ln2_hi is exact.  Within about an ulp of the correctly rounded value (tests/test_hostsim.py).
+// Other arguments go to the library log.
+BIA_HDF double log_pos(double x) {
+  if (!(x >= 2.2250738585072014e-308 && x <= 1.7976931348623157e308)) return log_lib(x);
+  int k;
+  double m = frexp(x, &k);  // m in [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m *= 2;
+    --k;
+  }
+  // u = f / (m + 1), f = m - 1 exact; d = m + 1 rounded and its rounding dd = m - (d - 1) exact
+  // (d - 1 and the difference are exact by Sterbenz); one correction of the quotient with the
+  // exact residual f - u0 d - u0 dd leaves u within about half an ulp
+  const double f = m - 1.0, d = m + 1.0, dd = m - (d - 1.0);
+  const double r = fast_rcp(d), u0 = f * r;
+  const double u = u0 + (fma(-u0, d, f) - u0 * dd) * r;
+  const double v = u * u;
+  double p = 1.0 / 23;
+  p = p * v + 1.0 / 21;
+  p = p * v + 1.0 / 19;
+  p = p * v + 1.0 / 17;
+  p = p * v + 1.0 / 15;
+  p = p * v + 1.0 / 13;
+  p = p * v + 1.0 / 11;
+  p = p * v + 1.0 / 9;
+  p = p * v + 1.0 / 7;
+  p = p * v + 1.0 / 5;
+  p = p * v + 1.0 / 3;
+  // ln 2 = kLn2Hi + kLn2Lo to 2e-31; kLn2Hi has 42 significant bits, so k kLn2Hi is exact
+  constexpr double kLn2Hi = 0x1.62e42fefa3800p-1, kLn2Lo = 5.497923018708371e-14;
+  const double dk = (double)k;
+  // 2u + 2u v p, then the exponent's part
+  return dk * kLn2Hi + (dk * kLn2Lo + (2 * u + (2 * u * v) * p));
+}
+
+// 1/sqrt(x) from the hardware estimate y (e = 1 - x y^2 near 2^-24) and one third-order
+// correction y (1 + e/2 + 3e^2/8), which leaves 5e^3/16 (within an ulp); the host build uses the
+// library square root
+BIA_HDF double fast_rsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-(x * y), y, 1.0);
+  return fma(y, e * fma(e, 0.375, 0.5), y);
+#else
+  return 1.0 / sqrt(x);
+#endif
+}
+
 // ------------------------------------------------------------------------------------
 // orbital-plane frame, find_dXYUV_kernel.cu:467-488:
 //   e_eta = g/|g|, e_zeta = (r x g)/|r x g|, e_xi = e_eta x e_zeta,
@@ -149,19 +289,21 @@ struct HypP {
 
 // residual of the dimensionless time equation and one Newton update (find_theta_end_cu :57-66)
 //   f(t) = tn0 sin t / (cos t - cs0) + ln( sin((t0+t)/2) / sin((t0-t)/2) ) - TE
+// sin/cos of the half angle (|t/2| < pi/2 inside the branch) and the logarithm of the positive
+// ratio from the range-specific routines, the quotients from reciprocals (each within an ulp)
 BIA_HDF double hyp_newton(double th, double TE, const HypP& P, double* err, double* dfd) {
   double sh, ch;
-  sincos_(0.5 * th, &sh, &ch);
+  sincos_pi(0.5 * th, &sh, &ch);
   double snp = P.s0h * ch + P.c0h * sh;
   double snm = P.s0h * ch - P.c0h * sh;
   double cs = (ch - sh) * (ch + sh);
   double sn = 2 * sh * ch;
-  double Dinv = 1 / (cs - P.cs0);
-  double d = P.tn0 * (1 - P.cs0 * cs) * Dinv * Dinv + P.sn0 / (2 * snp * snm);
-  double e = P.tn0 * sn * Dinv - TE + log(snp / snm);
+  double Dinv = fast_rcp(cs - P.cs0);
+  double d = P.tn0 * (1 - P.cs0 * cs) * Dinv * Dinv + P.sn0 * fast_rcp(2 * snp * snm);
+  double e = P.tn0 * sn * Dinv - TE + log_pos(snp * fast_rcp(snm));
   *err = e;
   *dfd = d;
-  return th - e / d;
+  return th - e * fast_rcp(d);
 }
 
 struct HypState {
@@ -190,6 +332,11 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   S.P.cs0 = bg / den;
   S.P.tn0 = S.P.sn0 / S.P.cs0;
   sincos_(0.5 * S.P.th0, &S.P.s0h, &S.P.c0h);
+  // the argument of acos with correctly rounded operations: acos near 1 amplifies its rounding
+  // (find_dXYUV_kernel.cu:119-126); the half-angle sines and cosines of theta_0 and theta_ini
+  // from the library routine too (the half-angle sums sin((t0 +- t)/2) cancel near t = -+t0);
+  // the rest uses reciprocals and the range-specific routines, whose ulp-level errors are far
+  // below the amplified one
   double q = S.P.cs0 + S.b * S.P.sn0 / r;
   double th_ini, cs;
   if (fabs(q) > 1) {  // :119-126
@@ -205,29 +352,31 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   sincos_(0.5 * th_ini, &sh, &ch);
   double sn = 2 * sh * ch;
   double x = r * cs, y = r * sn;
-  S.g0_sn0 = (L > 0) ? g0 / S.P.sn0 : -g0 / S.P.sn0;
+  const double gs = g0 * fast_rcp(S.P.sn0);
+  S.g0_sn0 = (L > 0) ? gs : -gs;
   S.u = S.g0_sn0 * (S.P.cs0 * sn);
   S.v = S.g0_sn0 * (1 - S.P.cs0 * cs);
-  double nrm = sqrt((X * X + Y * Y) * (x * x + y * y));
-  S.cs_a = (x * X + y * Y) / nrm;
-  S.sn_a = (x * Y - y * X) / nrm;
+  const double rn = fast_rsqrt((X * X + Y * Y) * (x * x + y * y));
+  S.cs_a = (x * X + y * Y) * rn;
+  S.sn_a = (x * Y - y * X) * rn;
   double snp = S.P.s0h * ch + S.P.c0h * sh;
   double snm = S.P.s0h * ch - S.P.c0h * sh;
-  double time_ini = (S.P.tn0 * sn / (cs - S.P.cs0) + log(snp / snm)) * b0 / g0;  // :153-155
+  const double rg0 = fast_rcp(g0);
+  double time_ini = (S.P.tn0 * sn * fast_rcp(cs - S.P.cs0) + log_pos(snp * fast_rcp(snm))) * b0 * rg0;  // :153-155
   double time_end = (L > 0) ? time_ini + dt : time_ini - dt;
-  S.TE = time_end * g0 / b0;
+  S.TE = time_end * g0 * fast_rcp(b0);
   S.th_ini = th_ini;
-  S.th_gss = 2 / kPi * S.P.th0 * atan(S.TE / S.P.tn0);
+  S.th_gss = 2 / kPi * S.P.th0 * atan(S.TE * fast_rcp(S.P.tn0));
 }
 
 // state after dt, rotated back to the (xi, eta) frame: :220-237
 BIA_HDF void hyp_finish(const HypState& S, double th_end, const double yz[4], double yn[4]) {
   double sn_end, cs_end;
-  sincos_(th_end, &sn_end, &cs_end);
+  sincos_pi(th_end, &sn_end, &cs_end);
   double u_end = S.g0_sn0 * S.P.cs0 * sn_end;
   double v_end = S.g0_sn0 * (1 - S.P.cs0 * cs_end);
   double du = u_end - S.u, dv = v_end - S.v;
-  double r_end = S.b * S.P.sn0 / (cs_end - S.P.cs0);
+  double r_end = S.b * S.P.sn0 * fast_rcp(cs_end - S.P.cs0);
   double x_end = r_end * cs_end, y_end = r_end * sn_end;
   yn[0] = x_end * S.cs_a - y_end * S.sn_a;
   yn[1] = x_end * S.sn_a + y_end * S.cs_a;
@@ -368,38 +517,12 @@ BIA_HDF Stumpff stumpff(double y) {
   return c;
 }
 
-// 1/x from the hardware reciprocal estimate r (relative error e = 1 - x r near 2^-24) and one
-// third-order correction r (1 + e + e^2), which leaves e^3 (within an ulp); 1/0 and non-finite x
-// give a non-finite result.  The host build divides.
-BIA_HDF double fast_rcp(double x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const double r = __builtin_amdgcn_rcp(x);
-  const double e = fma(-x, r, 1.0);
-  return fma(r, fma(e, e, e), r);
-#else
-  return 1.0 / x;
-#endif
-}
-
 // the hardware reciprocal estimate alone (relative error near 2^-24), for error estimates
 BIA_HDF double rcp_estimate(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __builtin_amdgcn_rcp(x);
 #else
   return 1.0 / x;
-#endif
-}
-
-// 1/sqrt(x) from the hardware estimate y (e = 1 - x y^2 near 2^-24) and one third-order
-// correction y (1 + e/2 + 3e^2/8), which leaves 5e^3/16 (within an ulp); the host build uses the
-// library square root
-BIA_HDF double fast_rsqrt(double x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const double y = __builtin_amdgcn_rsq(x);
-  const double e = fma(-(x * y), y, 1.0);
-  return fma(y, e * fma(e, 0.375, 0.5), y);
-#else
-  return 1.0 / sqrt(x);
 #endif
 }
 

@@ -13,7 +13,7 @@
 
 namespace bia {
 
-constexpr int kProfSlots = 64;  // pair-kernel launches whose timing events are kept
+constexpr int kProfSlots = 256;  // pair-kernel launches whose timing events are kept
 
 // one system of a (possibly batched) step: particles [off, off + n) of the SoA state, nt = ceil(n/64)
 // tiles, partial sums at part + poff laid out [nt][6][64 nt]
@@ -23,11 +23,16 @@ struct SysDesc {
   double eps;
 };
 
-// cached launch plan for a list of system sizes
+// cached launch plan for a list of system sizes.  A single system whose tile partials would exceed
+// the context's workspace limit runs in nstages stages of stage_rows row tiles each (stage g takes
+// tiles [stage_t0[g], stage_t0[g + 1]) of the list); otherwise nstages = 1.
 struct Plan {
   std::vector<int> sizes;
+  size_t limit = 0;  // the workspace limit the plan was made for
   int nsys = 0, ntiles = 0, nchunks = 0, ntotal = 0;
   size_t part_doubles = 0;
+  int nstages = 1, stage_rows = 0;
+  std::vector<int> stage_t0;
   std::vector<SysDesc> hsys;  // host copy (eps updated per call)
 };
 
@@ -141,6 +146,9 @@ struct bia_ctx {
   // caller's record buffer of pairs the solvers could not finish (bia_set_failure_log);
   // log_stats: counters for steps that log without statistics of their own
   double rkf_rel = 0.0, rkf_abs = 1e-16;
+  // largest tile-partial workspace of a step before a single system runs in stages (bytes, 0 =
+  // no limit); bia_set_workspace_limit
+  size_t workspace_limit = (size_t)1 << 30;
   void* log_rec = nullptr;
   unsigned long long* log_count = nullptr;
   int log_cap = 0;

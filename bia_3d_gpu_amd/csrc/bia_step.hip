@@ -145,8 +145,18 @@ __device__ __forceinline__ void lds_add(double* p, double x) {
 // into the column's LDS sum (lds_add; one address per lane per step, the steps in order).  A
 // diagonal tile (I = J) takes steps k = 1..32 (at k = 32 lanes 0..31 only), which covers each
 // unordered pair once.
+// The tiles one launch of the pair kernel takes, [t0, t1) of the plan's list.  Unstaged (rb NULL):
+// the whole list, partials at part[K][c][i] with K = J for the rows of tile (I, J) and K = I for
+// its columns (and the diagonal tile's both sides).  Staged: one system, the tiles of row tiles
+// [g0, g0 + G); the columns and diagonal go to part[I - g0][c][i] and the rows to
+// rb[I - g0][c][J][lane] (k_reduce_stage sums them in the order k_reduce would).
+struct Stage {
+  int t0, t1, g0;
+  double* rb;
+};
+
 template <int METHOD, int MODE, int COUP, bool STATS, int OCC>
-__global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const int4* __restrict__ tiles,
+__global__ __launch_bounds__(kWave, OCC) void k_tiles(Stage stg, int lds, const int4* __restrict__ tiles,
                                                  const SysDesc* __restrict__ sys, const double* __restrict__ st,
                                                  const double* __restrict__ mass, const double* __restrict__ charge,
                                                  double coup, double dt, int itr, double* __restrict__ part,
@@ -160,7 +170,7 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
   Hist hist{hist_lds + lane, kWave};
   PairStats ps;
   if (STATS) ps.clear();
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t = stg.t0 + blockIdx.x; t < stg.t1; t += gridDim.x) {
     const int4 td = tiles[t];
     const SysDesc S = sys[td.z];
     const int I = td.x, J = td.y;
@@ -182,6 +192,11 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
     const int jn = min(kWave, S.n - j0);
     const int ldp = S.nt * kWave;
     double* P = part + S.poff;
+    const int Ik = I - stg.g0;  // the column / diagonal slot of this tile's row tile
+    // where row component c of this tile goes: part[J][c][i], or rb[I - g0][c][J][lane] staged
+    auto row_slot = [&](int c) -> double* {
+      return stg.rb ? stg.rb + (((size_t)Ik * 6 + c) * S.nt + J) * kWave + lane : P + ((size_t)J * 6 + c) * ldp + il;
+    };
     if constexpr (MODE == kPlain) {
       // reference solvers: divergent and register-heavy, so the columns are walked in order (j
       // wave-uniform, its state through scalar loads) and each column sum is a wave reduction
@@ -224,12 +239,12 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
       }
       if (I == J) {
 #pragma unroll
-        for (int c = 0; c < 6; c++) put_partial(P + ((size_t)I * 6 + c) * ldp + il, acc[c] + col[c]);
+        for (int c = 0; c < 6; c++) put_partial(P + ((size_t)Ik * 6 + c) * ldp + il, acc[c] + col[c]);
       } else {
 #pragma unroll
         for (int c = 0; c < 6; c++) {
-          put_partial(P + ((size_t)J * 6 + c) * ldp + il, acc[c]);
-          put_partial(P + ((size_t)I * 6 + c) * ldp + j0 + lane, col[c]);
+          put_partial(row_slot(c), acc[c]);
+          put_partial(P + ((size_t)Ik * 6 + c) * ldp + j0 + lane, col[c]);
         }
       }
       continue;
@@ -299,12 +314,12 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(int ntiles, int lds, const
     __syncthreads();  // the column sums are complete
     if (diag) {
 #pragma unroll
-      for (int c = 0; c < 6; c++) put_partial(P + ((size_t)I * 6 + c) * ldp + il, acc[c] - csum[c][lane]);
+      for (int c = 0; c < 6; c++) put_partial(P + ((size_t)Ik * 6 + c) * ldp + il, acc[c] - csum[c][lane]);
     } else {
 #pragma unroll
       for (int c = 0; c < 6; c++) {
-        put_partial(P + ((size_t)J * 6 + c) * ldp + il, acc[c]);
-        put_partial(P + ((size_t)I * 6 + c) * ldp + j0 + lane, -csum[c][lane]);
+        put_partial(row_slot(c), acc[c]);
+        put_partial(P + ((size_t)Ik * 6 + c) * ldp + j0 + lane, -csum[c][lane]);
       }
     }
   }
@@ -338,6 +353,52 @@ __global__ __launch_bounds__(256) void k_reduce(const int2* __restrict__ chunks,
   const int g = S.off + il;
   if (c < 3) s += st[(size_t)(3 + c) * lds + g] * dt;
   delta[(size_t)c * ldd + g] = s;
+}
+
+// Staged reduction of one system (stage of row tiles [g0, g0 + G), G = gs): particle i of tile
+// R >= g0 continues its running sum in delta (0 at the first stage) with its slots of this stage
+// in ascending K, which is the order of k_reduce's sum: K in [g0, g0 + G) from part[K - g0] for a
+// later tile; for a tile of the stage, K in [g0, R) from part, the diagonal part[R - g0], then the
+// row slots K in (R, nt) from rb[R - g0][c][K]; that particle's sum is then complete and gets its
+// drift.  One block per 256 particles from 64 g0 on, one grid row per component.
+__global__ __launch_bounds__(256) void k_reduce_stage(int n, int nt, int g0, int gs, int first, int lds, int ldd,
+                                                      double dt, const double* __restrict__ st,
+                                                      const double* __restrict__ part, const double* __restrict__ rb,
+                                                      double* __restrict__ delta) {
+  const int i = g0 * kWave + blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int c = blockIdx.y;
+  const int R = i / kWave, l = i % kWave;
+  const int ldp = nt * kWave;
+  const bool own = R < g0 + gs;  // a tile of this stage: its sum completes here
+  const int kend = own ? R - g0 : gs;
+  double s = first ? 0.0 : delta[(size_t)c * ldd + i];
+  const double* p = part + (size_t)c * ldp + i;
+  const size_t stride = (size_t)6 * ldp;
+  int k = 0;
+  for (; k + 8 <= kend; k += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = __builtin_nontemporal_load(p + (size_t)(k + u) * stride);
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += v[u];
+  }
+  for (; k < kend; ++k) s += __builtin_nontemporal_load(p + (size_t)k * stride);
+  if (own) {
+    s += __builtin_nontemporal_load(p + (size_t)kend * stride);  // the diagonal tile
+    const double* q = rb + (((size_t)(R - g0) * 6 + c) * nt) * kWave + l;
+    int K = R + 1;
+    for (; K + 8 <= nt; K += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = __builtin_nontemporal_load(q + (size_t)(K + u) * kWave);
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; K < nt; ++K) s += __builtin_nontemporal_load(q + (size_t)K * kWave);
+    if (c < 3) s += st[(size_t)(3 + c) * lds + i] * dt;
+  }
+  delta[(size_t)c * ldd + i] = s;
 }
 
 // state += delta (the update of bia/bia.f90:203-204), SoA [6][ld], first n columns
@@ -419,9 +480,10 @@ static void drop_graph(bia_ctx* ctx) {
 static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
   Plan& P = ctx->plan;
   std::vector<int> sizes(a.sizes, a.sizes + a.nsys);
-  if (P.sizes != sizes) {
+  if (P.sizes != sizes || P.limit != ctx->workspace_limit) {
     Plan Q;
     Q.sizes = sizes;
+    Q.limit = ctx->workspace_limit;
     Q.nsys = a.nsys;
     std::vector<int4> tiles;
     std::vector<int2> chunks;
@@ -443,6 +505,24 @@ static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
     Q.ntiles = (int)tiles.size();
     Q.nchunks = (int)chunks.size();
     Q.part_doubles = (size_t)poff;
+    Q.stage_t0 = {0, Q.ntiles};
+    if (a.nsys == 1 && ctx->workspace_limit > 0 && Q.part_doubles * sizeof(double) > ctx->workspace_limit) {
+      // stages of G row tiles: partials of 12 G N_pad doubles (columns and diagonals G x 6 x N_pad,
+      // rows G x 6 x nt x 64)
+      const int nt = Q.hsys[0].nt;
+      const size_t per_row = (size_t)12 * nt * kWave * sizeof(double);
+      const int G = (int)std::max<size_t>(1, ctx->workspace_limit / per_row);
+      if (G < nt) {
+        Q.stage_rows = G;
+        Q.nstages = (nt + G - 1) / G;
+        Q.stage_t0.clear();
+        for (int g = 0; g <= Q.nstages; g++) {
+          const long long I0 = std::min(nt, g * G);
+          Q.stage_t0.push_back((int)(I0 * nt - I0 * (I0 - 1) / 2));  // tiles of rows < I0
+        }
+        Q.part_doubles = (size_t)12 * G * nt * kWave;
+      }
+    }
     P = Plan();  // invalid until every upload below has succeeded
     drop_graph(ctx);
     ctx->plan_gen++;
@@ -543,9 +623,20 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   unsigned long long* slots = ctx->stat_slots.as<unsigned long long>();
   if (st && !slots) return BIA_ERR_STATE;  // ensure_plan allocates the slots for a step with statistics
   const RkfTol tol{ctx->rkf_rel, ctx->rkf_abs};
+  const Plan& Pl = ctx->plan;
+  const bool staged = V.context_plan && Pl.nstages > 1;
+  const int nstages = staged ? Pl.nstages : 1;
+  for (int g = 0; g < nstages; g++) {
+  Stage stg{0, V.ntiles, 0, nullptr};
+  if (staged) {
+    stg = Stage{Pl.stage_t0[g], Pl.stage_t0[g + 1], g * Pl.stage_rows,
+                part + (size_t)6 * Pl.stage_rows * Pl.hsys[0].nt * kWave};
+    grid = std::min(grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : stg.t1 - stg.t0),
+                    std::min(stg.t1 - stg.t0, kMaxGrid64));
+  }
   BIA_TRY(ctx, prof_begin(ctx));
 #define L_OCC(M, MO, CP, S, O)                                                                                     \
-  hipLaunchKernelGGL((k_tiles<M, MO, CP, S, O>), dim3(grid), dim3(kWave), 0, ctx->stream, V.ntiles, a.ld, tiles, sys, \
+  hipLaunchKernelGGL((k_tiles<M, MO, CP, S, O>), dim3(grid), dim3(kWave), 0, ctx->stream, stg, a.ld, tiles, sys, \
                      a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, slots, T, tol, lg)
 #define L_(M, MO, CP, S)                                    \
   do {                                                      \
@@ -577,15 +668,28 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
 #undef L_OCC
   BIA_HIP_TRY(ctx, hipGetLastError());
   BIA_TRY(ctx, prof_end(ctx));
+  if (staged) {  // this stage's partials into the running sums (the final deltas of its own rows)
+    const SysDesc& S0 = Pl.hsys[0];
+    const int gs = std::min(Pl.stage_rows, S0.nt - stg.g0);
+    const int nrem = S0.n - stg.g0 * kWave;
+    BIA_TRY(ctx, prof_begin(ctx, 1));
+    hipLaunchKernelGGL(k_reduce_stage, dim3((nrem + 255) / 256, 6), dim3(256), 0, ctx->stream, S0.n, S0.nt, stg.g0,
+                       gs, g == 0 ? 1 : 0, a.ld, a.ld, a.dt, a.d_state, part, stg.rb, a.d_delta);
+    BIA_HIP_TRY(ctx, hipGetLastError());
+    BIA_TRY(ctx, prof_end(ctx, 1));
+  }
+  }  // stages
   if (st) {
     hipLaunchKernelGGL(k_stats_fold, dim3(BIA_STATS_LEN), dim3(256), 0, ctx->stream, slots, stats_out);
     BIA_HIP_TRY(ctx, hipGetLastError());
   }
-  BIA_TRY(ctx, prof_begin(ctx, 1));
-  hipLaunchKernelGGL(k_reduce, dim3(V.nchunks, 6), dim3(256), 0, ctx->stream, V.chunks, sys, a.ld, a.ld, a.dt,
-                     a.d_state, part, a.d_delta);
-  BIA_HIP_TRY(ctx, hipGetLastError());
-  BIA_TRY(ctx, prof_end(ctx, 1));
+  if (!staged) {
+    BIA_TRY(ctx, prof_begin(ctx, 1));
+    hipLaunchKernelGGL(k_reduce, dim3(V.nchunks, 6), dim3(256), 0, ctx->stream, V.chunks, sys, a.ld, a.ld, a.dt,
+                       a.d_state, part, a.d_delta);
+    BIA_HIP_TRY(ctx, hipGetLastError());
+    BIA_TRY(ctx, prof_end(ctx, 1));
+  }
   return BIA_OK;
 }
 
@@ -857,6 +961,20 @@ int bia_step_noalloc_dev(bia_ctx* ctx, int method, int mode, int n, double eps, 
   BIA_HIP_TRY(ctx, hipGetLastError());
   const PlanView V{tiles, sys, chunks, part, ntiles, nchunks, false};
   return launch_step(ctx, a, 0, 0, &V);
+}
+
+int bia_set_workspace_limit(bia_ctx* ctx, unsigned long long bytes) {
+  BIA_LOCKED(ctx);
+  ctx->workspace_limit = (size_t)bytes;
+  return BIA_OK;
+}
+
+int bia_plan_info(bia_ctx* ctx, int* nstages, unsigned long long* workspace_bytes) {
+  BIA_LOCKED(ctx);
+  if (ctx->plan.sizes.empty()) return BIA_ERR_STATE;
+  if (nstages) *nstages = ctx->plan.nstages;
+  if (workspace_bytes) *workspace_bytes = (unsigned long long)(ctx->plan.part_doubles * sizeof(double));
+  return BIA_OK;
 }
 
 int bia_set_rkf_tolerance(bia_ctx* ctx, double relerr, double abserr) {

@@ -172,6 +172,19 @@ int bia_run_steps_dev(bia_ctx *ctx, int method, int mode, int coupling, int nsys
                       const double *d_charge, double *d_delta, int nsteps, int use_graph,
                       unsigned long long *d_stats);
 
+/* Workspace of the step.  The pair kernel leaves per-tile partial sums that a second kernel adds
+ * in a fixed order (bitwise reproducible results); for one system of n particles they take
+ * 6 x 8 x n_pad x n_pad / 64 bytes (n_pad = 64 ceil(n / 64)), 3.2 GB at n = 65536.  When that
+ * exceeds the context's limit (default 1 GiB; 0 = no limit), a single-system step runs in stages
+ * of G row tiles, each followed by its reduction, with 12 x 8 x G x n_pad bytes of partials (G the
+ * largest that fits, at least 1): the same deltas bitwise, in O(n) workspace.  Batched steps of
+ * several systems are not staged.  No counterpart in the reference (its per-particle calls keep
+ * 96 n bytes on the device, bia/find_dXYUV_kernel.cu:532-533). */
+int bia_set_workspace_limit(bia_ctx *ctx, unsigned long long bytes);
+/* The plan of the context's last step: its number of stages and tile-partial workspace in bytes
+ * (BIA_ERR_STATE before any step). */
+int bia_plan_info(bia_ctx *ctx, int *nstages, unsigned long long *workspace_bytes);
+
 /* Tolerances of the per-pair RKF45 (BIA_METHOD_RKF) for the context's subsequent steps: the error
  * test of a trial step is |err_k| <= relerr (|y_k| + |y_k'|)/2 + abserr per component, as in
  * bia/bia-rkf.cu:72-79, where RELERR = 0 and ABSERR = 1e-16 are compile-time constants (bia-rkf.cu:5;

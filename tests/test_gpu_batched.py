@@ -251,8 +251,9 @@ def test_monitored_driver_matches_plain_steps(ctx, mode, tmp_path):
 
 def test_kernel_timing_ring(ctx):
     """With profiling on, every step's pair kernel is timed by its own pair of HIP events and
-    bia_kernel_ms returns the last 64 durations, oldest first, after steps enqueued with no
-    synchronisation in between; the newest equals bia_last_kernel_ms."""
+    bia_kernel_ms returns the last 256 durations, oldest first, after steps enqueued with no
+    synchronisation in between; the newest equals bia_last_kernel_ms; the reduction kernel of the
+    same steps has a ring of its own."""
     import torch
     from bia_3d_gpu_amd import MODE_GUESS, ic
     n = 700
@@ -266,10 +267,12 @@ def test_kernel_timing_ring(ctx):
         three = ctx.kernel_ms()
         assert len(three) == 3 and all(t > 0 for t in three)
         assert three[-1] == ctx.last_kernel_ms()
-        for _ in range(67):
+        for _ in range(267):
             ctx.step_(s, d, -2.0 / n, 1e-3, mode=MODE_GUESS)
         ring = ctx.kernel_ms()
-        assert len(ring) == 64 and all(t > 0 for t in ring) and ring[-1] == ctx.last_kernel_ms()
+        assert len(ring) == 256 and all(t > 0 for t in ring) and ring[-1] == ctx.last_kernel_ms()
         assert len(ctx.kernel_ms(5)) == 5
+        red = ctx.reduce_kernel_ms()
+        assert len(red) == 256 and all(t > 0 for t in red)
     finally:
         ctx.set_profiling(False)
