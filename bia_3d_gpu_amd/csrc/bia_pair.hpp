@@ -331,12 +331,11 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   S.P.sn0 = aL / den;
   S.P.cs0 = bg / den;
   S.P.tn0 = S.P.sn0 / S.P.cs0;
-  sincos_(0.5 * S.P.th0, &S.P.s0h, &S.P.c0h);
+  sincos_pi(0.5 * S.P.th0, &S.P.s0h, &S.P.c0h);
   // the argument of acos with correctly rounded operations: acos near 1 amplifies its rounding
-  // (find_dXYUV_kernel.cu:119-126); the half-angle sines and cosines of theta_0 and theta_ini
-  // from the library routine too (the half-angle sums sin((t0 +- t)/2) cancel near t = -+t0);
-  // the rest uses reciprocals and the range-specific routines, whose ulp-level errors are far
-  // below the amplified one
+  // (find_dXYUV_kernel.cu:119-126); the rest, the half-angle sines and cosines of theta_0 and
+  // theta_ini included (|theta_0 / 2|, |theta_ini / 2| <= pi / 2), uses reciprocals and the
+  // range-specific routines, whose ulp-level errors are far below the amplified one
   double q = S.P.cs0 + S.b * S.P.sn0 / r;
   double th_ini, cs;
   if (fabs(q) > 1) {  // :119-126
@@ -349,7 +348,7 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   }
   if (L * r_g < 0) th_ini = -th_ini;
   double sh, ch;
-  sincos_(0.5 * th_ini, &sh, &ch);
+  sincos_pi(0.5 * th_ini, &sh, &ch);
   double sn = 2 * sh * ch;
   double x = r * cs, y = r * sn;
   const double gs = g0 * fast_rcp(S.P.sn0);

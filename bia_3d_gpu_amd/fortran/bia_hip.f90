@@ -5,7 +5,7 @@ module bia_hip
   use iso_c_binding
   implicit none
   integer(c_int), parameter :: BIA_OK = 0, BIA_METHOD_RKF = 0, BIA_METHOD_ALG = 1
-  integer(c_int), parameter :: BIA_MODE_PLAIN = 0, BIA_MODE_LEARN = 1, BIA_STATS_LEN = 16
+  integer(c_int), parameter :: BIA_MODE_PLAIN = 0, BIA_MODE_LEARN = 1, BIA_MODE_GUESS = 2, BIA_STATS_LEN = 16
   integer(c_int), parameter :: BIA_COUPLING_GRAVITY = 1, BIA_COUPLING_COULOMB = 2
 
   interface
@@ -18,6 +18,28 @@ module bia_hip
     integer(c_int) function bia_destroy(ctx) bind(C, name='bia_destroy')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+
+    ! per-pair RKF45 tolerances of the following steps (the reference's 0 / 1e-16 by default)
+    integer(c_int) function bia_set_rkf_tolerance(ctx, relerr, abserr) bind(C, name='bia_set_rkf_tolerance')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: relerr, abserr
+    end function
+
+    ! largest tile-partial workspace of a step in bytes (0 = no limit; default 1 GiB)
+    integer(c_int) function bia_set_workspace_limit(ctx, bytes) bind(C, name='bia_set_workspace_limit')
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long), value :: bytes
+    end function
+
+    ! stages and tile-partial workspace bytes of the last step's plan
+    integer(c_int) function bia_plan_info(ctx, nstages, workspace_bytes) bind(C, name='bia_plan_info')
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: nstages
+      integer(c_long_long), intent(out) :: workspace_bytes
     end function
 
     integer(c_int) function bia_step_host(ctx, method, mode, n, eps, dt, itr, X, Y, Z, U, V, W, &

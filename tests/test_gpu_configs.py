@@ -476,10 +476,12 @@ def test_mass_spectrum_ensemble_against_dim(ctx):
 
 
 def test_system_262144_with_64_bit_partial_offsets():
-    """The largest single system tested: Plummer N = 262144 (4096 tiles a side; the tile partials
-    hold 6.4e9 doubles, 51.5 GB, so their offsets pass 2^32), eps = -2/N, dt = 1e-3, one
-    Kepler-drift step (3.4e10 pairs).  Checked by (1) the same system in a shuffled order: every
-    particle's delta agrees with the unshuffled one's up to summation order; (2) six particles,
+    """The largest single system tested: Plummer N = 262144 (4096 tiles a side), eps = -2/N,
+    dt = 1e-3, one Kepler-drift step (3.4e10 pairs), unstaged (workspace limit off: the tile
+    partials hold 6.4e9 doubles, 51.5 GB, so their offsets pass 2^32) and under the default 1 GiB
+    limit (98 stages of 42 row tiles, a 1.06 GB stage buffer): the two bitwise equal.  Checked
+    further by (1) the same system in a shuffled order: every particle's delta agrees with the
+    unshuffled one's up to summation order; (2) six particles,
     the first and last ones among them, against the oracle's reference-algorithm row for that
     particle (the particle moved to the front, so that its row covers all its pairs): velocities
     to 1e-9 of the largest velocity delta, positions normwise to 1e-11 — the reference
@@ -495,15 +497,20 @@ def test_system_262144_with_64_bit_partial_offsets():
     perm = np.random.default_rng(7).permutation(n)
     with Context(0) as c:
         c.set_stream(torch.cuda.current_stream().cuda_stream)
-        out = []
-        for order in (None, perm):
+        out, plans = [], []
+        for order, limit in ((None, 0), (None, 1 << 30), (perm, 1 << 30)):
+            c.set_workspace_limit(limit)
             s = torch.from_numpy(np.ascontiguousarray(st if order is None else st[:, order])).cuda()
             d = torch.empty_like(s)
             c.step_(s, d, eps, dt, mode=MODE_GUESS)
             torch.cuda.synchronize()
             out.append(d.cpu().numpy())
+            plans.append(c.plan_info())
             del s, d
-    d, dp = out
+    d_unstaged, d, dp = out
+    print(f"plans (stages, workspace bytes): unstaged {plans[0]}, staged {plans[1]}")
+    assert plans[0] == (1, 6 * n * (n // 64) * 8) and plans[1][0] == 98 and plans[1][1] <= 1 << 30
+    assert np.array_equal(d_unstaged, d)
     pair_part = d.copy()
     pair_part[:3] -= st[3:] * dt  # the position deltas without the drift v dt
     scale = np.abs(pair_part).max(axis=1, keepdims=True)

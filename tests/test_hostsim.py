@@ -387,3 +387,46 @@ def test_learned_start_cannot_derail_the_solver(hs):
             out, br, it, fl = hs(1, np.ascontiguousarray(rel[keep]), eps, dt, mode=1, table=table)
             assert (fl & 64).any()  # learned starts taken
             assert np.isfinite(out).all() and not (fl & 4).any() and it.max() <= 30, (scale, trial, it.max())
+
+
+def _ulp_error(y, ref):
+    """|y - ref| in units of the last place of ref (ref in extended precision)."""
+    ulp = np.spacing(np.abs(ref.astype(np.float64)))
+    return (np.abs(y.astype(np.longdouble) - ref) / ulp).astype(np.float64)
+
+
+def test_range_specific_sincos_and_log():
+    """The plain mode's sin/cos of |x| <= pi and log of positive normal x (bia_pair.hpp
+    sincos_pi, log_pos; host build of the same source) against 80-bit extended-precision values:
+    within 1.1 ulp for sin and cos and 1.5 ulp for log, on random arguments and on the points
+    where the range reduction switches (pi/4, 3pi/4) or cancels (near pi/2, pi, and log near 1).
+    A wrong reduction constant, a dropped series term or an uncorrected quotient would show as
+    many ulps."""
+    L = C.CDLL(LIB)
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    L.hs_sincos_pi.argtypes = [C.c_int64, dp, dp, dp]
+    L.hs_log_pos.argtypes = [C.c_int64, dp, dp]
+    if np.finfo(np.longdouble).nmant < 63:
+        pytest.skip("no 80-bit long double on this host")
+    rng = np.random.default_rng(2016)
+    pi = np.pi
+    edges = (np.array([0.25, 0.5, 0.75, 1.0]) * pi * (1 + rng.uniform(-1e-6, 1e-6, (2000, 1)))).ravel()
+    x = np.concatenate([rng.uniform(-pi, pi, 400000), edges, -edges, 10.0 ** rng.uniform(-300, 0, 20000),
+                        np.array([0.0, pi, -pi, pi / 2, pi / 4, 3 * pi / 4])])
+    x = np.ascontiguousarray(x[np.abs(x) <= pi])
+    s, c = np.zeros_like(x), np.zeros_like(x)
+    L.hs_sincos_pi(x.size, x, s, c)
+    xl = x.astype(np.longdouble)
+    es, ec = _ulp_error(s, np.sin(xl)), _ulp_error(c, np.cos(xl))
+    print(f"sincos_pi on {x.size} arguments: max {es.max():.3f} ulp (sin), {ec.max():.3f} ulp (cos)")
+    assert es.max() < 1.1 and ec.max() < 1.1
+    y = np.concatenate([10.0 ** rng.uniform(-307, 307, 200000), 1 + rng.uniform(-0.3, 0.3, 200000),
+                        1 + 10.0 ** rng.uniform(-15, -1, 20000), 1 - 10.0 ** rng.uniform(-15, -1, 20000),
+                        np.array([1.0, 2.0, 0.5, np.sqrt(2.0), np.sqrt(0.5), 1e-300, 1e300])])
+    out = np.zeros_like(y)
+    L.hs_log_pos(y.size, np.ascontiguousarray(y), out)
+    ref = np.log(y.astype(np.longdouble))
+    nz = ref != 0
+    el = _ulp_error(out[nz], ref[nz])
+    print(f"log_pos on {y.size} arguments: max {el.max():.3f} ulp; log(1) = {out[~nz]}")
+    assert el.max() < 1.5 and (out[~nz] == 0).all()

@@ -115,6 +115,34 @@ def test_rkf_matches_reference_routine():
         assert np.abs(yn - yr).max() <= 1e-15 * max(1.0, np.abs(yr).max()), (yz, yn, yr)
 
 
+
+def test_rkf_tolerance_variant():
+    """orc_rkf_pair_tol (the per-pair RKF45 at a caller's relerr / abserr) is bitwise the
+    reference-tolerance RKF45 at 0 / 1e-16; at relerr 1e-12 it takes fewer accepted steps on every
+    pair that needs more than the controller's fewest, never more, and stays within 1e-10 of the conic solution where the reference's Newton converges (an error-norm or
+    controller mistake in the variant would break the first check or the last)."""
+    rng = np.random.default_rng(12)
+    eps = -6e-7
+    fewer, compared = 0, 0
+    for yz in _random_pairs(rng, 40, eps):
+        dt = 10 ** rng.uniform(-3, -1)
+        ya, sa, fa = O.rkf2d_tol(eps, dt, yz, 0.0, 1e-16)
+        yr, sr, fr = O.solve2d("rkf", eps, dt, yz)
+        assert np.array_equal(ya, yr) and sa == sr and fa == fr
+        yt, st_, ft = O.rkf2d_tol(eps, dt, yz, 1e-12, 1e-16)
+        assert st_ <= sr
+        if sr > 5:  # more than the fewest steps the controller takes over dt
+            assert st_ < sr
+            fewer += 1
+        W = np.hypot(yz[0], yz[1]) * (yz[2] ** 2 + yz[3] ** 2) + 2 * eps
+        yc, _, fc = O.solve2d("hyperbola" if W > 0 else "ellipse", eps, dt, yz)
+        if fc or ft:
+            continue
+        compared += 1
+        scale = max(np.abs(yc).max(), 1e-300)
+        assert np.abs(yt - yc).max() <= 1e-10 * scale, (yz, dt, yt, yc)
+    assert fewer >= 5 and compared >= 20, (fewer, compared)
+
 # --------------------------------------------------------------------------------------------
 # mathematics
 # --------------------------------------------------------------------------------------------
