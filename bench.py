@@ -6,10 +6,12 @@ dt = 1e-3, eps = -2Gm, seed 65536.  One "step" = one BIA step of every unordered
 (N(N-1)/2 pair interactions) followed by the state update, state resident in HBM.
 
 Modes measured on the same input: plain (the reference's conic solvers, guesses and iteration
-control), guess (universal-variable Kepler drift, every pair from its series start, no network)
-and learned (the same drift; a pair whose series start is far off takes its second point from the
-network's table).  The network is trained on the GPU first, from samples the solver records on
-other systems (training time reported separately).
+control), plain_learned (the same solvers and iteration control, the hyperbola and ellipse Newton
+started from network 1's anomaly, the reference's guess the fallback and the judge), guess
+(universal-variable Kepler drift, every pair from its series start, no network) and learned (the
+same drift; a pair whose series start is far off takes its second point from network 0's table).
+Each network is trained on the GPU first, from samples the solvers record on other systems
+(training time reported separately).
 
 Printed: one JSON line (rank 0).  ``value`` is the pair-interactions/s of the learned mode
 (guess if training is skipped with --no-learn).  ``roofline`` prices the dominant kernel (the
@@ -25,7 +27,9 @@ samples, on one core and on one process per host core (at most 16), and its RKF 
 with an untimed per-pair RKF45 step of the same system (the reference's own integrator, an
 independent solution of the same pairs).  ``reductions`` gives the HBM rate of the step's
 tile-partial reduction kernel (its own HIP events) and of the invariants.  Side measurements:
-``learned_vs_guess`` (headline, longer steps, the reference fixture), ``reference_fixture``
+``learned_vs_guess`` (headline, longer steps, the reference fixture), ``plain_learned_vs_plain``
+(time, evaluations, caps and fallbacks of the reference solvers with and without the learned
+starts), ``reference_fixture``
 (with each mode's normwise agreement with the reference's own .BIA output), ``config1`` (1000
 steps against the DIM), ``energy_order`` (|dE/E0| over a fixed time at dt, dt/2, dt/4), and the
 two ensembles.
@@ -219,16 +223,16 @@ def run_mass_ensemble(ctx, mode, steps: int, warmup: int, seed: int = 4096):
             "dE_rel_median": float(np.median(de)), "dE_rel_max": float(de.max())}
 
 
-def run_fixture(ctx, steps: int, learned: bool):
+def run_fixture(ctx, steps: int, learned: bool, plain_learned: bool = False):
     """Side measurement on the reference's own N = 1332 input (tests/golden/0001332_10^0.bbb), the
     only case with published reference timings (one BIA step, dt = 1, eps = -6e-7, TITAN V: ALG
     0.193 s and per-pair RKF 0.293 s of GPU solver time; the DIM RKF45 run to t = 1: 2.946 s
-    total).  Times the plain ALG mode (the reference's algorithm), the guess and learned modes, the
-    per-pair RKF method and the DIM run, and gives each step's normwise agreement with the
+    total).  Times the plain ALG mode (the reference's algorithm), the reference solvers with
+    learned starts, the guess and learned modes, the per-pair RKF method and the DIM run, and gives each step's normwise agreement with the
     reference's own output on this input (OUT/0001332_10^0.BIA, its CUDA ALG run)."""
     import torch
 
-    from bia_3d_gpu_amd import MODE_GUESS, MODE_LEARN, MODE_PLAIN, io
+    from bia_3d_gpu_amd import MODE_GUESS, MODE_LEARN, MODE_PLAIN, MODE_PLAIN_LEARN, io
 
     st = io.read_bbb(os.path.join(ROOT, "tests", "golden", "0001332_10^0.bbb"))[0]
     (_, bia_ref), = io.read_timed(os.path.join(ROOT, "tests", "golden", "0001332_10^0.BIA"))
@@ -242,6 +246,8 @@ def run_fixture(ctx, steps: int, learned: bool):
     runs = [("plain", 1, MODE_PLAIN, steps), ("guess", 1, MODE_GUESS, steps), ("rkf", 0, MODE_PLAIN, 2)]
     if learned:
         runs.insert(2, ("learned", 1, MODE_LEARN, steps))
+    if plain_learned:
+        runs.insert(1, ("plain_learned", 1, MODE_PLAIN_LEARN, steps))
     stats = torch.zeros(16, dtype=torch.int64, device="cuda:0")
     for name, method, mode, k in runs:
         stats.zero_()
@@ -260,10 +266,15 @@ def run_fixture(ctx, steps: int, learned: bool):
         ctx.set_profiling(False)
         out[name] = {"s_per_step": sec, "pairs_per_s": pairs / sec, "kernel_ms": kms,
                      "evaluations_per_pair": float(c[4:8].sum() / max(1, c[0:4].sum())), "learned_starts": int(c[14]),
+                     "fallbacks": int(c[15]),
                      "vs_reference_BIA": {"positions": agree["positions"], "velocities": agree["velocities"]}}
     if learned:
         out["learned_vs_guess"] = {"kernel_ms_ratio": out["learned"]["kernel_ms"] / out["guess"]["kernel_ms"],
                                    "s_per_step_ratio": out["learned"]["s_per_step"] / out["guess"]["s_per_step"]}
+    if plain_learned:
+        out["plain_learned_vs_plain"] = {
+            "kernel_ms_ratio": out["plain_learned"]["kernel_ms"] / out["plain"]["kernel_ms"],
+            "s_per_step_ratio": out["plain_learned"]["s_per_step"] / out["plain"]["s_per_step"]}
     y = np.ascontiguousarray(st.T).reshape(-1)
     t0 = time.perf_counter()
     r = ctx.dim_rkf45(y, -3e-7, 0.0, 1.0)
@@ -271,16 +282,17 @@ def run_fixture(ctx, steps: int, learned: bool):
     return out
 
 
-def run_config1(ctx, steps: int = 1000):
+def run_config1(ctx, steps: int = 1000, plain_learned: bool = False):
     """Side measurement, config 1: N = 1332 uniform in the unit ball, virial ratio 0.5, G = 1,
     m = 1/N, seed 1332; `steps` graph-replayed BIA steps of dt = 1e-3 in the plain (reference
     solvers) and guess modes, against the DIM RKF45 (abserr 1e-16, the reference's controller) to
     t = steps dt on the GPU: ms per step, the median and worst particle's position error, |dE/E0|
     from the invariants.  The reference's ellipse Newton fails on near-radial bound pairs of this
-    system (from step 20), so the plain trajectory ends far from the DIM."""
+    system (from step 20), so the plain trajectory ends far from the DIM; with the learned starts
+    (plain_learned, when network 1 is trained) the same solvers are run again."""
     import torch
 
-    from bia_3d_gpu_amd import MODE_GUESS, MODE_PLAIN, ic
+    from bia_3d_gpu_amd import MODE_GUESS, MODE_PLAIN, MODE_PLAIN_LEARN, ic
 
     n, dt = 1332, 1e-3
     st = ic.uniform_ball_virial(n, seed=1332)
@@ -298,7 +310,8 @@ def run_config1(ctx, steps: int = 1000):
     dim = np.ascontiguousarray(y.T)
     out = {"n": n, "dt": dt, "steps": steps, "dim": {"s": dim_s, "accepted_steps": acc,
                                                       "dE_rel": abs(E(ctx.invariants(dim, eps)) - e0) / abs(e0)}}
-    for name, mode in (("plain", MODE_PLAIN), ("guess", MODE_GUESS)):
+    runs = [("plain", MODE_PLAIN), ("guess", MODE_GUESS)] + ([("plain_learned", MODE_PLAIN_LEARN)] if plain_learned else [])
+    for name, mode in runs:
         s = torch.from_numpy(st).to("cuda:0")
         d = torch.empty_like(s)
         ctx.run_steps_(s, d, [n], 1, eps=eps, dt=dt, mode=mode, use_graph=True)  # capture
@@ -487,8 +500,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble side measurements")
     ap.add_argument("--no-fixture", action="store_true", help="skip the N=1332 reference-fixture side measurement")
-    ap.add_argument("--modes", default="plain,guess,learned",
-                    help="comma-separated subset of plain,guess,learned (profiling runs)")
+    ap.add_argument("--modes", default="plain,guess,learned,plain_learned",
+                    help="comma-separated subset of plain,guess,learned,plain_learned (profiling runs)")
     ap.add_argument("--copy-mb", type=float, default=0.0,
                     help="also run one device copy of this many MB (HBM counter calibration)")
     ap.add_argument("--workspace-mb", type=float, default=None,
@@ -503,7 +516,7 @@ def main():
 
     import torch
 
-    from bia_3d_gpu_amd import METHOD_RKF, Context, MODE_GUESS, MODE_LEARN, MODE_PLAIN, learn
+    from bia_3d_gpu_amd import METHOD_RKF, Context, MODE_GUESS, MODE_LEARN, MODE_PLAIN, MODE_PLAIN_LEARN, learn
 
     n = args.n
     eps = -2.0 / n
@@ -551,6 +564,22 @@ def main():
         res["learned"] = r
         log(f"learned: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms ({r['tflops']:.1f} TFLOP/s), "
             f"|dE/E0| {r['dE_rel']:.2e}, tuned {r['tuned']}")
+    train_ref = None
+    if not args.no_learn and "plain_learned" in modes:
+        # network 1: the starting anomaly of the reference's own Newton iterations
+        t0 = time.perf_counter()
+        w, curve, rows = learn.train_reference(ctx, nsamples=args.train_samples, steps=args.train_steps, log=log)
+        torch.cuda.synchronize()
+        train_ref = {"seconds": time.perf_counter() - t0, "samples": int(rows.shape[0]), "steps": args.train_steps,
+                     "loss_first": float(curve[0]), "loss_last": float(np.mean(curve[-20:]))}
+        del rows
+        r = run_mode(ctx, st, eps, args.dt, args.itr, MODE_PLAIN_LEARN, args.steps, args.warmup, not args.no_autotune)
+        r["pairs_per_s"] = pairs / (r["ms_per_step"] / 1e3)
+        res["plain_learned"] = r
+        sr = np.asarray(r["stats"], dtype=np.float64)
+        log(f"plain_learned: {r['ms_per_step']:.2f} ms/step, kernel {r['kernel_ms']:.2f} ms "
+            f"({r['tflops']:.1f} TFLOP/s), evaluations/pair {sr[4:8].sum() / sr[0:4].sum():.3f}, "
+            f"fallbacks {int(sr[15])}, tuned {r['tuned']}")
     ensemble = None
     if not args.no_ensemble:
         emode = MODE_LEARN if "learned" in res else MODE_GUESS
@@ -567,18 +596,22 @@ def main():
         mass_ensemble["mode"] = "learned" if emode == MODE_LEARN else "guess"
         log(f"mass-spectrum ensemble (64 systems, {mass_ensemble['particles']} particles, gravity coupling): "
             f"{mass_ensemble['ms_per_step']:.2f} ms/step, {mass_ensemble['pairs_per_s']:.3e} pairs/s")
-    fixture = None if args.no_fixture else run_fixture(ctx, 20, "learned" in res)
+    fixture = None if args.no_fixture else run_fixture(ctx, 20, "learned" in res, "plain_learned" in res)
     if fixture:
         log(f"reference fixture N=1332: plain {fixture['plain']['s_per_step'] * 1e3:.3f} ms/step, guess "
             f"{fixture['guess']['s_per_step'] * 1e3:.3f} ms/step, RKF {fixture['rkf']['s_per_step'] * 1e3:.1f} "
             f"ms/step, DIM {fixture['dim']['s']:.3f} s ({fixture['dim']['accepted_steps']} steps)")
-    config1 = None if args.no_fixture else run_config1(ctx)
+    config1 = None if args.no_fixture else run_config1(ctx, plain_learned="plain_learned" in res)
     if config1:
         log(f"config 1 (N=1332 ball, 1000 steps of 1e-3 vs DIM, {config1['dim']['accepted_steps']} DIM steps in "
             f"{config1['dim']['s']:.1f} s): plain {config1['plain']['ms_per_step']:.3f} ms/step, median error "
             f"{config1['plain']['position_error_median']:.2e}, |dE/E0| {config1['plain']['dE_rel']:.2e}; guess "
             f"{config1['guess']['ms_per_step']:.3f} ms/step, median error {config1['guess']['position_error_median']:.2e}, "
             f"|dE/E0| {config1['guess']['dE_rel']:.2e}")
+        if "plain_learned" in config1:
+            c1 = config1["plain_learned"]
+            log(f"config 1 with learned starts: {c1['ms_per_step']:.3f} ms/step, median error "
+                f"{c1['position_error_median']:.2e}, max {c1['position_error_max']:.2e}, |dE/E0| {c1['dE_rel']:.2e}")
     energy_order = None if args.no_ensemble else run_energy_order(ctx, st, eps, args.dt, "learned" in res)
     if energy_order:
         log(f"energy over T={energy_order['T']}: " + "; ".join(
@@ -638,12 +671,22 @@ def main():
             "longer_steps": learned_gain,
             "reference_fixture": None if not fixture else fixture.get("learned_vs_guess")},
         "reductions": {"tile_partials": head["reduction"], "invariants": head["invariants"]},
+        "plain_learned_vs_plain": None if not {"plain_learned", "plain"} <= set(res) else {
+            "ms_per_step_ratio": res["plain_learned"]["ms_per_step"] / res["plain"]["ms_per_step"],
+            "kernel_ms_ratio": res["plain_learned"]["kernel_ms"] / res["plain"]["kernel_ms"],
+            "evaluations_per_pair": {k: float(np.sum(res[k]["stats"][4:8]) / np.sum(res[k]["stats"][0:4]))
+                                     for k in ("plain", "plain_learned")},
+            "iteration_caps": {k: int(res[k]["stats"][10]) for k in ("plain", "plain_learned")},
+            "learned_starts": int(res["plain_learned"]["stats"][14]),
+            "fallbacks": int(res["plain_learned"]["stats"][15]),
+            "reference_fixture": None if not fixture else fixture.get("plain_learned_vs_plain")},
         "config1": config1,
         "energy_order": energy_order,
         "speedup_vs_plain": None if plain is None else {k: v["pairs_per_s"] / plain["pairs_per_s"]
                                                         for k, v in res.items()},
         "agreement_vs_rkf45": parity,
         "training": train,
+        "training_reference_starts": train_ref,
         "ensemble": ensemble,
         "mass_spectrum_ensemble": mass_ensemble,
         "reference_fixture": fixture,

@@ -9,7 +9,7 @@ driver (:mod:`.driver`) and the learned-initialiser training (:mod:`.learn`).
 """
 from . import io, ic  # noqa: F401
 from .api import (  # noqa: F401
-    METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, PAIR_RECORD_DTYPE, STATS_LEN, Context, FailureLog,
+    METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, MODE_PLAIN_LEARN, PAIR_RECORD_DTYPE, STATS_LEN, Context, FailureLog,
     find_dxyuv_kernel, flops_per_step,
     invariance_gpu, reference_step, rkf_c,
 )

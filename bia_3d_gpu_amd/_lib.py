@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libbia_hip.so")
 
 BIA_OK, BIA_ERR_ARG, BIA_ERR_HIP, BIA_ERR_OOM, BIA_ERR_STATE, BIA_ERR_RKF = 0, 1, 2, 3, 4, 5
 METHOD_RKF, METHOD_ALG = 0, 1
-MODE_PLAIN, MODE_LEARN, MODE_GUESS = 0, 1, 2
+MODE_PLAIN, MODE_LEARN, MODE_GUESS, MODE_PLAIN_LEARN = 0, 1, 2, 3
 STATS_LEN = 16
 COUPLING_SCALAR, COUPLING_GRAVITY, COUPLING_COULOMB = 0, 1, 2
 
@@ -68,6 +68,7 @@ SIGNATURES = {
     "bia_invariants_coupled_batched_dev": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, _ip32, _vp, C.c_int, _vp,
                                                      _vp, _vp]),
     "bia_mlp_nparam": (C.c_int, []),
+    "bia_mlp_select": (C.c_int, [_vp, C.c_int]),
     "bia_mlp_init": (C.c_int, [_vp, C.c_ulonglong]),
     "bia_mlp_set_weights": (C.c_int, [_vp, _fp, C.c_int]),
     "bia_mlp_get_weights": (C.c_int, [_vp, _fp, C.c_int]),

@@ -16,11 +16,12 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, STATS_LEN, check, load
+from ._lib import METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, MODE_PLAIN_LEARN, STATS_LEN, check, load
 
 __all__ = ["find_dxyuv_kernel", "invariance_gpu", "rkf_c", "reference_step", "Context", "FailureLog",
            "PAIR_RECORD_DTYPE",
-           "METHOD_ALG", "METHOD_RKF", "MODE_PLAIN", "MODE_LEARN", "MODE_GUESS", "STATS_LEN", "flops_per_step"]
+           "METHOD_ALG", "METHOD_RKF", "MODE_PLAIN", "MODE_LEARN", "MODE_GUESS", "MODE_PLAIN_LEARN", "STATS_LEN",
+           "flops_per_step"]
 
 INVARIANT_NAMES = ("KE", "PE", "Lx", "Ly", "Lz", "Ua", "Va", "Wa", "Xa", "Ya", "Za")
 
@@ -192,14 +193,20 @@ def flops_per_step(stats, mode: int = MODE_PLAIN) -> float:
     functions, accumulation 12) + 58 per evaluation of the time equation (kep_g 22, kep_halley
     32, step and convergence test 4), + 120 per learned start (three log2 features, trilinear lookup,
     exp, scale).  The bound-orbit period reduction, taken by few pairs, is not counted.
+
+    Reference solvers with learned starts: the plain-mode counts with the evaluations the pairs
+    made (the learned start's one evaluation takes the place of the reference's two, which the
+    hyperbola's 635 include), + 80 per learned start (two log2 features, a reciprocal, the
+    bilinear lookup, expm1 or the ellipse's reduction).
     """
     s = np.asarray(stats, dtype=np.float64)
-    if mode != MODE_PLAIN:
+    if mode not in (MODE_PLAIN, MODE_PLAIN_LEARN):
         return 109 * s[0:4].sum() + 58 * s[4:8].sum() + 120 * s[14]
     n_h, n_p, n_e, n_r = s[0:4]
     it_h, it_p, it_e, it_r = s[4:8]
     k_h = it_h - 2 * n_h
-    return (635 * n_h + 122 * k_h) + (472 * n_p + 76 * it_p) + (524 * n_e + 47 * it_e) + (80 * n_r + 790 * it_r)
+    extra = 80 * s[14] if mode == MODE_PLAIN_LEARN else 0.0
+    return (635 * n_h + 122 * k_h) + (472 * n_p + 76 * it_p) + (524 * n_e + 47 * it_e) + (80 * n_r + 790 * it_r) + extra
 
 
 # ------------------------------------------------------------------------------------------------
@@ -467,6 +474,11 @@ class Context:
         return out
 
     # -- learned initialiser ---------------------------------------------------------------------
+    def mlp_select(self, net: int):
+        """Make network ``net`` (0: the Kepler drift's start, 1: the reference solvers' starts) the
+        one the following mlp_* and record_samples_ calls act on."""
+        check(load().bia_mlp_select(self._h, int(net)), "bia_mlp_select", self._h)
+
     def mlp_init(self, seed: int = 1):
         check(load().bia_mlp_init(self._h, int(seed)), "bia_mlp_init", self._h)
 

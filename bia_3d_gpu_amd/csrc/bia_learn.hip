@@ -20,6 +20,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <utility>
 #include <vector>
 
 #include "bia_pair.hpp"
@@ -323,6 +324,75 @@ __global__ __launch_bounds__(64) void k_record(int n, int ld, double eps, double
   }
 }
 
+// Recorder of the reference solvers' starts (network 1): sample s draws random pairs until it
+// finds one of its branch (hyperbola for even s, ellipse for odd s; at most kDraws draws) with a
+// clean solution, and writes the row [u1, u2, u3, target] (features and targets as
+// hyp_learned_start and ell_learned_start read them):
+//   hyperbola: the reference's guess selection and loop (hyp_solve_plain, ITR 18) ended by an
+//     exact repeat or its tolerance, not at the cap or with its default iterate, with a best
+//     residual within kFallbackTolH max(1, |TE|) (an exact repeat away from the root also ends
+//     the loop); target
+//     e log(theta* / theta_gss), kept when |target| <= 8;
+//   ellipse: the root E* of E - e sin E = M_r by bisection of [M_r - e, M_r + e] (60 halvings;
+//     the function increases, so the bracket holds it); target sin E*.
+// u1 = NaN when no such pair was found.
+__global__ __launch_bounds__(64) void k_record_ref(int n, int ld, double eps, double dt, const double* __restrict__ st,
+                                                   int nsamples, unsigned long long seed, float* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsamples) return;
+  float* o = out + (size_t)s * 4;
+  o[0] = __builtin_nanf("");
+  o[1] = o[2] = o[3] = 0.f;
+  const bool want_ellipse = (s & 1) != 0;
+  double hbuf[kMaxItr];
+  const Hist hist{hbuf, 1};
+  for (int draw = 0; draw < kDraws; draw++) {
+    const unsigned long long h =
+        mix64(seed ^ (0xd1b54a32d192ed03ULL * (unsigned long long)(s + 1)) ^ (0x9e3779b97f4a7c15ULL * (draw + 1)));
+    const int i = (int)(h % (unsigned long long)n);
+    int j = (int)((h >> 32) % (unsigned long long)(n - 1));
+    if (j >= i) j++;
+    double rel[6], exi[3], eeta[3], yz[4];
+    for (int c = 0; c < 6; c++) rel[c] = st[(size_t)c * ld + i] - st[(size_t)c * ld + j];
+    frame(rel, rel + 3, exi, eeta, yz);
+    const double W = sqrt(yz[0] * yz[0] + yz[1] * yz[1]) * (yz[2] * yz[2] + yz[3] * yz[3]) + 2 * eps;
+    if (want_ellipse) {
+      if (!(W < 0)) continue;
+      const EllSetup E = ell_setup(eps, dt, yz);
+      if (!(E.ecc >= 0 && E.ecc < 1) || !isfinite(E.tau_end)) continue;
+      const double k = rint(E.tau_end * (0.5 / kPi));
+      const double Mr = E.tau_end - (2 * kPi) * k;
+      double lo = Mr - E.ecc, hi = Mr + E.ecc;
+      for (int q = 0; q < 60; q++) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid - E.ecc * sin(mid) - Mr > 0) hi = mid;
+        else lo = mid;
+      }
+      o[0] = pl_feat_e_ell(E.ecc);
+      o[1] = (float)(Mr * (1 / kPi));
+      o[2] = -1.f;
+      o[3] = (float)sin(0.5 * (lo + hi));
+      return;
+    }
+    if (!(W > 0)) continue;
+    HypState S;
+    unsigned fl = 0;
+    hyp_setup(eps, dt, yz, S, &fl);
+    int it = 0;
+    double res = 0;
+    const double th = hyp_solve_plain(S, 18, hist, &it, &fl, &res);
+    if ((fl & (kItrCap | kStockDefault | kAcosClamp)) || !(fabs(res) <= kFallbackTolH * fmax(1.0, fabs(S.TE)))) continue;
+    const double e = 1 / fabs(S.P.cs0);
+    const double t = e * log(th / S.th_gss);
+    if (!(fabs(t) <= 8)) continue;
+    o[0] = pl_feat_e_hyp(e);
+    o[1] = pl_feat_t(S.TE, S.P.cs0);
+    o[2] = 1.f;
+    o[3] = (float)t;
+    return;
+  }
+}
+
 static int mlp_bufs(bia_ctx* ctx) {
   // [w | m1 | m2 | grad(+loss)] floats
   return ctx->mlp_w.ensure((size_t)4 * (kNParam + 4) * sizeof(float));
@@ -371,6 +441,21 @@ using namespace bia;
 extern "C" {
 
 int bia_mlp_nparam(void) { return kNParam; }
+
+int bia_mlp_select(bia_ctx* ctx, int net) {
+  BIA_LOCK(ctx);
+  if (net != 0 && net != 1) return BIA_ERR_ARG;
+  if (net != ctx->mlp_net) {
+    std::swap(ctx->mlp_w, ctx->alt_w);
+    std::swap(ctx->mlp_table, ctx->alt_table);
+    std::swap(ctx->mlp_init, ctx->alt_init);
+    std::swap(ctx->mlp_ready, ctx->alt_ready);
+    std::swap(ctx->table_n, ctx->alt_table_n);
+    std::swap(ctx->mlp_step, ctx->alt_step);
+    ctx->mlp_net = net;
+  }
+  return BIA_OK;
+}
 
 int bia_mlp_init(bia_ctx* ctx, unsigned long long seed) {
   BIA_LOCK(ctx);
@@ -472,8 +557,12 @@ int bia_record_samples_dev(bia_ctx* ctx, int n, double eps, double dt, const dou
                            unsigned long long seed, float* d_out) {
   BIA_LOCK(ctx);
   if (n < 2 || ld < n || !d_state || !d_out || nsamples < 1) return BIA_ERR_ARG;
-  hipLaunchKernelGGL(k_record, dim3((nsamples + 63) / 64), dim3(64), 0, ctx->stream, n, ld, eps, dt, d_state, nsamples,
-                     seed, d_out);
+  if (ctx->mlp_net == 1)
+    hipLaunchKernelGGL(k_record_ref, dim3((nsamples + 63) / 64), dim3(64), 0, ctx->stream, n, ld, eps, dt, d_state,
+                       nsamples, seed, d_out);
+  else
+    hipLaunchKernelGGL(k_record, dim3((nsamples + 63) / 64), dim3(64), 0, ctx->stream, n, ld, eps, dt, d_state,
+                       nsamples, seed, d_out);
   BIA_HIP_TRY(ctx, hipGetLastError());
   return BIA_OK;
 }

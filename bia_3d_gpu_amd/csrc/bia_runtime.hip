@@ -102,7 +102,7 @@ int bia_destroy(bia_ctx* c) {
   bia::DevBuf* bufs[] = {&c->part, &c->state, &c->delta, &c->stat_slots, &c->log_stats, &c->inv_state, &c->inv_part,
                          &c->inv_out,
                          &c->inv_tiles, &c->inv_sys, &c->dim_work, &c->dim_ctl,
-                         &c->mlp_w, &c->mlp_ws, &c->mlp_table};
+                         &c->mlp_w, &c->mlp_ws, &c->mlp_table, &c->alt_w, &c->alt_table};
   for (auto* b : bufs) b->release();
   c->plan_tiles.release();
   c->plan_sys.release();

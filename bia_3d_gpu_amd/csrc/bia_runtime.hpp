@@ -136,12 +136,19 @@ struct bia_ctx {
   // DIM workspace; dim_h: pinned staging of the step sizes of the captured steps
   bia::DevBuf dim_work, dim_ctl;
   bia::HostBuf dim_h;
-  // learned initialiser: fp32 master weights + Adam moments, training workspace, tabulated output
+  // learned initialisers: fp32 master weights + Adam moments, training workspace, tabulated
+  // output.  Two networks: 0 starts the Kepler drift (BIA_MODE_LEARN), 1 the reference solvers
+  // (BIA_MODE_PLAIN_LEARN).  The mlp_* fields hold the network bia_mlp_select chose (mlp_net),
+  // the alt_* fields the other one.
   bia::DevBuf mlp_w, mlp_ws, mlp_table;
   int mlp_init = 0;   // weights present
   int mlp_ready = 0;  // table built from the current weights
   int table_n = 0;
   long long mlp_step = 0;
+  int mlp_net = 0;
+  bia::DevBuf alt_w, alt_table;
+  int alt_init = 0, alt_ready = 0, alt_table_n = 0;
+  long long alt_step = 0;
   // per-pair RKF45 tolerances (the reference's RELERR 0, ABSERR 1e-16 by default) and the
   // caller's record buffer of pairs the solvers could not finish (bia_set_failure_log);
   // log_stats: counters for steps that log without statistics of their own
@@ -185,6 +192,16 @@ struct bia_ctx {
   } while (0)
 
 namespace bia {
+// the table of network k (0: the Kepler drift's start, 1: the reference solvers' starts) and
+// whether it is built
+struct NetTable {
+  const float* t;
+  int n, ready;
+};
+inline NetTable net_table(const bia_ctx* c, int k) {
+  return (k == c->mlp_net) ? NetTable{c->mlp_table.as<float>(), c->table_n, c->mlp_ready}
+                           : NetTable{c->alt_table.as<float>(), c->alt_table_n, c->alt_ready};
+}
 // default context used by the legacy Fortran-callable symbols (the reference ABI has no handle)
 bia_ctx* default_ctx();
 // abort with a message: the legacy symbols have no error return, so failures are loud

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(Stage stg, int lds, const 
     auto row_slot = [&](int c) -> double* {
       return stg.rb ? stg.rb + (((size_t)Ik * 6 + c) * S.nt + J) * kWave + lane : P + ((size_t)J * 6 + c) * ldp + il;
     };
-    if constexpr (MODE == kPlain) {
+    if constexpr (reference_solvers(MODE)) {
       // reference solvers: divergent and register-heavy, so the columns are walked in order (j
       // wave-uniform, its state through scalar loads) and each column sum is a wave reduction
       double acc[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
@@ -565,10 +565,19 @@ static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
   return BIA_OK;
 }
 
+// the learned-start table a mode reads (none for the plain and guess modes)
+static LearnTable mode_table(const bia_ctx* ctx, int mode) {
+  if (mode != BIA_MODE_LEARN && mode != BIA_MODE_PLAIN_LEARN) return LearnTable{nullptr, 0};
+  const NetTable t = net_table(ctx, mode == BIA_MODE_LEARN ? 0 : 1);
+  return LearnTable{t.t, t.n};
+}
+
 static int validate(bia_ctx* ctx, const StepArgs& a) {
   if (a.nsys < 1 || !a.sizes || !a.d_state || !a.d_delta) return BIA_ERR_ARG;
   if (a.method != BIA_METHOD_ALG && a.method != BIA_METHOD_RKF) return BIA_ERR_ARG;
-  if (a.mode != BIA_MODE_PLAIN && a.mode != BIA_MODE_LEARN && a.mode != BIA_MODE_GUESS) return BIA_ERR_ARG;
+  if (a.mode != BIA_MODE_PLAIN && a.mode != BIA_MODE_LEARN && a.mode != BIA_MODE_GUESS &&
+      a.mode != BIA_MODE_PLAIN_LEARN)
+    return BIA_ERR_ARG;
   if (a.mode != BIA_MODE_PLAIN && a.method != BIA_METHOD_ALG) return BIA_ERR_ARG;
   if (a.coupling < BIA_COUPLING_SCALAR || a.coupling > BIA_COUPLING_COULOMB) return BIA_ERR_ARG;
   if (a.coupling == BIA_COUPLING_SCALAR && !a.eps) return BIA_ERR_ARG;
@@ -578,8 +587,14 @@ static int validate(bia_ctx* ctx, const StepArgs& a) {
   long long tot = 0;
   for (int s = 0; s < a.nsys; s++) tot += a.sizes[s];
   if (a.ld < tot) return BIA_ERR_ARG;
-  if (a.mode == BIA_MODE_LEARN && !ctx->mlp_ready) {
+  if (a.mode == BIA_MODE_LEARN && !net_table(ctx, 0).ready) {
     snprintf(ctx->last_error, sizeof(ctx->last_error), "learned mode needs a trained network (bia_mlp_build_table)");
+    return BIA_ERR_STATE;
+  }
+  if (a.mode == BIA_MODE_PLAIN_LEARN && !net_table(ctx, 1).ready) {
+    snprintf(ctx->last_error, sizeof(ctx->last_error),
+             "reference solvers with learned starts need network 1 trained and tabulated (bia_mlp_select, "
+             "bia_mlp_build_table)");
     return BIA_ERR_STATE;
   }
   return BIA_OK;
@@ -605,7 +620,7 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   const auto tuned = V.context_plan ? tuned_for(ctx, a) : std::pair<int, int>(0, 1);
   const int tuned_grid = tuned.first, tuned_occ = tuned.second;
   const int occ = occ_override > 0 ? occ_override : tuned_occ;
-  const LearnTable T{ctx->mlp_table.as<float>(), ctx->table_n};
+  const LearnTable T = mode_table(ctx, a.mode);
   int grid = grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : V.ntiles);
   // a dispatch holds fewer than 2^32 work-items; the pair kernel is grid-stride, so a grid capped
   // there still covers every tile (one system of more than about 741k particles)
@@ -641,7 +656,7 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
 #define L_(M, MO, CP, S)                                    \
   do {                                                      \
     if (occ == 4) L_OCC(M, MO, CP, S, 4);                   \
-    else if (M == 1 && MO != 0 && occ == 5) L_OCC(M, MO, CP, S, 5); \
+    else if (M == 1 && !reference_solvers(MO) && occ == 5) L_OCC(M, MO, CP, S, 5); \
     else L_OCC(M, MO, CP, S, 1);                            \
   } while (0)
 #define L_COUP(M, MO, S)                                                  \
@@ -659,6 +674,9 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   } else if (a.mode == BIA_MODE_LEARN) {
     if (st) L_COUP(1, kLearn, true);
     else L_COUP(1, kLearn, false);
+  } else if (a.mode == BIA_MODE_PLAIN_LEARN) {
+    if (st) L_COUP(1, kPlainLearn, true);
+    else L_COUP(1, kPlainLearn, false);
   } else {
     if (st) L_COUP(1, kGuess, true);
     else L_COUP(1, kGuess, false);
@@ -721,7 +739,7 @@ static int autotune_locked(bia_ctx* ctx, const StepArgs& a, float* best_ms, int*
   int bestg = ntiles, besto = 1;
   int status = BIA_OK;
   std::vector<int> occs = {1, 4};
-  if (a.method == BIA_METHOD_ALG && a.mode != BIA_MODE_PLAIN) occs.push_back(5);
+  if (a.method == BIA_METHOD_ALG && (a.mode == BIA_MODE_LEARN || a.mode == BIA_MODE_GUESS)) occs.push_back(5);
   for (int o : occs)
     for (int g : cands) {
       if ((status = launch_step(ctx, b, g, o)) != BIA_OK) break;  // warm-up
@@ -784,8 +802,9 @@ static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, 
   std::vector<long long> key = {a.method, a.mode, a.coupling, a.itr, (long long)(size_t)d_state_rw,
                                 (long long)(size_t)a.d_delta, (long long)(size_t)a.d_mass,
                                 (long long)(size_t)a.d_charge, a.ld, (long long)(size_t)ctx->stream, tuned.first, tuned.second,
-                                ctx->mlp_ready, (long long)(size_t)ctx->part.p,
-                                (long long)(size_t)ctx->mlp_table.p, ctx->table_n, (long long)(size_t)a.d_stats,
+                                (long long)(size_t)ctx->part.p,
+                                (long long)(size_t)mode_table(ctx, a.mode).t, mode_table(ctx, a.mode).n,
+                                (long long)(size_t)a.d_stats,
                                 (long long)(size_t)ctx->stat_slots.p, ctx->plan_gen,
                                 (long long)(size_t)ctx->plan_tiles.p, (long long)(size_t)ctx->plan_sys.p,
                                 (long long)(size_t)ctx->plan_chunks.p, (long long)(size_t)ctx->log_rec,

@@ -46,6 +46,9 @@ extern "C" {
 #define BIA_MODE_PLAIN 0  /* the reference's conic solvers, starting guesses and iteration control */
 #define BIA_MODE_LEARN 1  /* the Kepler drift; a pair whose series start is far off takes its second point from the learned table */
 #define BIA_MODE_GUESS 2  /* the same Kepler drift, every pair from its series start (no network) */
+#define BIA_MODE_PLAIN_LEARN 3  /* the reference's conic solvers and iteration control, the hyperbola and
+                                   ellipse Newton started from network 1's anomaly; the reference's
+                                   guess is the fallback and the judge (see bia_mlp_select) */
 
 /* pair couplings of the batched entry points */
 #define BIA_COUPLING_SCALAR 0  /* the reference's r'' = eps r/|r|^3, eps per system, equal-mass half split */
@@ -60,7 +63,7 @@ extern "C" {
  *         evaluations of the universal time equation)
  * [8..15] pairs with status bit b set: 1 best-iterate never updated, 2 exact-repeat exit,
  *         4 iteration cap, 8 acos clamp, 16 RKF failure, 32 non-finite, 64 learned start used,
- *         128 unused */
+ *         128 learned start's run rejected for the reference guess's (BIA_MODE_PLAIN_LEARN) */
 
 typedef struct bia_ctx bia_ctx;
 
@@ -282,6 +285,23 @@ int bia_dim_rkf45_coupled_host(bia_ctx *ctx, int coupling, double coup, int np, 
  * floats: W1[64][3] b1[64] W2[64][64] b2[64] w3[64] b3.  Products run on BF16 MFMA with FP32
  * accumulation; the master weights are FP32. */
 int bia_mlp_nparam(void);
+/* A context holds two such networks, with their own weights, optimiser state and table:
+ *   net 0 (default): the Kepler drift's start above (BIA_MODE_LEARN);
+ *   net 1: the starting anomaly of the reference's own Newton iterations (BIA_MODE_PLAIN_LEARN),
+ *          replacing the hyperbola's theta_ini / theta_gss pair (bia/find_dXYUV_kernel.cu:169-185)
+ *          and the ellipse's xi_2 guess (:253-265).  Inputs, u3 = +1 hyperbola / -1 ellipse:
+ *          hyperbola u1 = log10(e - 1) / 7 (clamped to [-1, 1]), u2 = s (log2|T| + 10) / 37
+ *          (magnitude clamped to [0, 1]) of the dimensionless target time T, s = -1 attractive / +1 repulsive
+ *          (the sign of cos theta_0), target e log(theta* / theta_gss); ellipse
+ *          u1 = 1 - 2 sqrt(1 - e), u2 = M_r / pi of the end mean anomaly reduced to [-pi, pi],
+ *          target sin E* at the root of Kepler's equation.  The pair kernel reads the table's end slices u3 = -1,
+ *          +1.  The learned start's iteration keeps the reference's control (best iterate, exact
+ *          repeat, cap); where it stops at the cap unconverged, the reference's own guess runs as
+ *          well and the smaller best residual is kept.
+ * bia_mlp_select(ctx, net) makes `net` the one the bia_mlp_* and bia_record_samples_dev calls
+ * below act on (0 or 1; BIA_ERR_ARG otherwise); the step entry points read each mode's own
+ * network whichever is selected. */
+int bia_mlp_select(bia_ctx *ctx, int net);
 int bia_mlp_init(bia_ctx *ctx, unsigned long long seed);
 int bia_mlp_set_weights(bia_ctx *ctx, const float *w, int n);
 int bia_mlp_get_weights(bia_ctx *ctx, float *w, int n);
@@ -297,7 +317,10 @@ int bia_mlp_train_dev(bia_ctx *ctx, const float *d_rows, int nrows, int steps, i
  * pair kernel's trilinear lookup; enables BIA_MODE_LEARN */
 int bia_mlp_build_table(bia_ctx *ctx, int n);
 /* record nsamples pairs of a system that are not weak over dt (each of up to 4096 random draws):
- * rows [u1, u2, u3, target] solved to convergence, u1 = NaN when none was found (device) */
+ * rows [u1, u2, u3, target] solved to convergence, u1 = NaN when none was found (device).
+ * With net 1 selected: rows of the reference solvers' starts (above) from hyperbolic pairs
+ * (even samples) and elliptic pairs (odd samples) that the reference's iteration solves to its
+ * exit without the cap (hyperbola) or whose root of Kepler's equation bisection finds (ellipse). */
 int bia_record_samples_dev(bia_ctx *ctx, int n, double eps, double dt, const double *d_state, int ld, int nsamples,
                            unsigned long long seed, float *d_out);
 

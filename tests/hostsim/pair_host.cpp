@@ -17,6 +17,8 @@ extern "C" void hs_pairs(int method, int64_t n, double eps, double dt, int itr, 
       bia::pair_eval<1, bia::kLearn>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h, T);
     else if (mode == 2)
       bia::pair_eval<1, bia::kGuess>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h, T);
+    else if (mode == 3)
+      bia::pair_eval<1, bia::kPlainLearn>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h, T);
     else
       bia::pair_plain<1>(eps, dt, itr, rel + 6 * k, out + 6 * k, &br, &it, &fl, h);
     branch[k] = br;

@@ -430,3 +430,109 @@ def test_range_specific_sincos_and_log():
     el = _ulp_error(out[nz], ref[nz])
     print(f"log_pos on {y.size} arguments: max {el.max():.3f} ulp; log(1) = {out[~nz]}")
     assert el.max() < 1.5 and (out[~nz] == 0).all()
+
+
+def _binary_pairs(n=65536, seed=65536):
+    """Each particle of the benchmark system with its nearest neighbour (its binary companion when
+    it has one): 6526 elliptic pairs with eccentricities up to 0.9993, the rest hyperbolic."""
+    from scipy.spatial import cKDTree
+    from bia_3d_gpu_amd import ic
+    st = ic.plummer_with_binaries(n, seed)
+    _, idx = cKDTree(st[:3].T).query(st[:3].T, k=2)
+    i = np.arange(n)
+    keep = i < idx[:, 1]
+    return np.ascontiguousarray(st[:, i[keep]].T - st[:, idx[keep, 1]].T), -2.0 / n, 1e-3
+
+
+def _quad_error(rel, eps, dt, d):
+    err = np.zeros(len(rel))
+    for k in range(len(rel)):
+        hi, _ = O.pair_quad(eps, dt, rel[k])
+        err[k] = np.abs(d[k] - hi).max() / np.abs(hi).max()
+    return err
+
+
+def test_reference_solvers_with_learned_starts_mechanics(hs, fixture1332):
+    """BIA_MODE_PLAIN_LEARN with an all-zero table, where the starts are the reference's own
+    theta_gss (hyperbola) and E = M (ellipse): on the fixture's pairs the results agree with the
+    reference solvers' to 1e-9 of each pair's delta and 1e-12 in position deltas, and the pairs
+    take one evaluation fewer (the learned start's single set-up evaluation replaces the
+    reference's two).  On the benchmark system's binaries, the reference ellipse Newton stops at
+    its cap unconverged on 5 high-eccentricity pairs (e > 0.96) and returns wrong deltas; there
+    the reference guess's run is kept only where its residual is smaller (the judge), and all 5
+    come out within 1e-10 of the quad-precision two-body solution."""
+    table = np.zeros((64, 64, 64), np.float32)
+    rel = _pairs(fixture1332["state"])[::53]
+    a, bra, ita, fla = hs(1, rel, -6e-7, 1.0)
+    b, brb, itb, flb = hs(1, rel, -6e-7, 1.0, mode=3, table=table)
+    assert np.all(bra == brb) and ((flb & 64) != 0).all()
+    scale = np.abs(a).max(axis=1)
+    assert (np.abs(a - b).max(axis=1) <= 1e-9 * scale).all()
+    assert np.abs(a[:, :3] - b[:, :3]).max() < 1e-12
+    print(f"fixture pairs: evaluations per pair {ita.mean():.3f} (reference) -> {itb.mean():.3f}")
+    assert itb.mean() < ita.mean() - 0.9
+    rel, eps, dt = _binary_pairs()
+    a, bra, ita, fla = hs(1, rel, eps, dt)
+    b, brb, itb, flb = hs(1, rel, eps, dt, mode=3, table=table)
+    ell = bra == 2
+    wrong = ell & (_quad_error(rel, eps, dt, a) > 1e-10) & ((fla & 4) != 0)
+    print(f"binaries: {ell.sum()} ellipses, the reference wrong on {wrong.sum()} (capped unconverged); "
+          f"{((flb & 128) != 0).sum()} fallbacks to the reference guess")
+    assert wrong.sum() == 5
+    assert (_quad_error(rel[wrong], eps, dt, b[wrong]) < 1e-10).all()
+    # elsewhere the two agree to the reference formulas' own rounding: weak hyperbolic pairs carry
+    # ~1e-8 of it in their position departure (both modes alike against the quad solution)
+    ok = ~wrong
+    d = np.abs(a[ok] - b[ok]).max(axis=1) / np.abs(a[ok]).max(axis=1)
+    assert (d <= 1e-7).all()
+    far = d > 1e-9
+    if far.any():
+        qa, qb = _quad_error(rel[ok][far], eps, dt, a[ok][far]), _quad_error(rel[ok][far], eps, dt, b[ok][far])
+        assert (qb <= 10 * qa + 1e-8).all()
+
+
+def test_reference_solvers_with_learned_starts_survive_any_table(hs):
+    """Tables of random values (none trained): the hyperbola's proposal is kept inside its branch
+    and its factor exp(T / e) within e^4, the ellipse's within e of M, and a run that has not
+    converged (cap, residual above 1e-12 of the time to reach, or — the reference loop's quirk —
+    no iterate better than the start, which returns theta = 0) falls back to the reference's
+    guess; so on strong pairs of every branch (attractive and repulsive, dt up to 3) every output
+    is finite and agrees with the reference solvers' wherever neither stops at its cap.  Without
+    the default-iterate rule, three such pairs came out 0.8 to 4474 times their delta off."""
+    rng = np.random.default_rng(79)
+    for scale in (5.0, 50.0):
+        table = np.ascontiguousarray((rng.standard_normal((64, 64, 64)) * scale).astype(np.float32))
+        for trial in range(6):
+            m = 4000
+            r = rng.standard_normal((m, 3)) * 10.0 ** rng.uniform(-2, 0, (m, 1))
+            v = rng.standard_normal((m, 3)) * 10.0 ** rng.uniform(-2, 0, (m, 1))
+            rel = np.ascontiguousarray(np.hstack([r, v]))
+            eps = -10.0 ** rng.uniform(-3, -0.5) * (1 if trial % 3 else -1)
+            dt = 10.0 ** rng.uniform(-2, 0.5)
+            a, bra, ita, fla = hs(1, rel, eps, dt)
+            b, brb, itb, flb = hs(1, rel, eps, dt, mode=3, table=table)
+            assert np.isfinite(b).all() and np.all(bra == brb)
+            both = ((fla | flb) & 4) == 0  # neither run capped
+            d = np.abs(a[both] - b[both]).max(axis=1) / np.abs(a[both]).max(axis=1)
+            assert (d < 1e-6).all(), (scale, trial, d.max())
+
+
+def test_reference_solvers_with_learned_starts_last_resort(hs):
+    """A hyperbolic pair of the config-1 ball (N = 1332, eps = -2/N, dt = 1e-3) at step 457 of its
+    trajectory, near-radial and close (theta_ini 0.14 inside -theta_0): the reference's Newton
+    overshoots out of the branch, stops at its cap with no improved iterate and returns theta = 0,
+    a velocity change of 0.81 where the two-body solution gives 0.0074.  With learned starts (an
+    all-zero table, so the learned start is the reference's theta_gss) that run fails as well, and
+    the bisected start that comes last converges to within 1e-10 of the quad-precision solution."""
+    rel = np.array([float.fromhex(h) for h in ('-0x1.ef10c9e790b60p-8', '0x1.af59a8aa314e0p-8',
+                                              '0x1.9392f96f9dd00p-9', '0x1.d2198a3cc9fc8p-1',
+                                              '-0x1.3b5e176a59078p-1', '-0x1.a2f6248c2c580p-2')])[None, :]
+    eps, dt = -2.0 / 1332, 1e-3
+    table = np.zeros((64, 64, 64), np.float32)
+    a, bra, ita, fla = hs(1, rel, eps, dt)
+    b, brb, itb, flb = hs(1, rel, eps, dt, mode=3, table=table)
+    ea, eb = _quad_error(rel, eps, dt, a)[0], _quad_error(rel, eps, dt, b)[0]
+    print(f"reference: flags {fla[0]}, error {ea:.1e} of the pair's delta; learned starts: flags {flb[0]}, "
+          f"{itb[0]} evaluations, error {eb:.1e}")
+    assert bra[0] == 0 and (fla[0] & 4) and (fla[0] & 1) and ea > 1
+    assert (flb[0] & 128) and eb < 1e-10
