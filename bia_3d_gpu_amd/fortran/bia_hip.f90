@@ -5,7 +5,8 @@ module bia_hip
   use iso_c_binding
   implicit none
   integer(c_int), parameter :: BIA_OK = 0, BIA_METHOD_RKF = 0, BIA_METHOD_ALG = 1
-  integer(c_int), parameter :: BIA_MODE_PLAIN = 0, BIA_MODE_LEARN = 1, BIA_MODE_GUESS = 2, BIA_STATS_LEN = 16
+  integer(c_int), parameter :: BIA_MODE_PLAIN = 0, BIA_MODE_LEARN = 1, BIA_MODE_GUESS = 2, BIA_MODE_PLAIN_LEARN = 3
+  integer(c_int), parameter :: BIA_STATS_LEN = 16
   integer(c_int), parameter :: BIA_COUPLING_GRAVITY = 1, BIA_COUPLING_COULOMB = 2
 
   interface
@@ -18,6 +19,14 @@ module bia_hip
     integer(c_int) function bia_destroy(ctx) bind(C, name='bia_destroy')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+
+    ! which of the context's two learned initialisers the bia_mlp_* calls act on (0: the Kepler
+    ! drift's start, 1: the reference solvers' starts of BIA_MODE_PLAIN_LEARN)
+    integer(c_int) function bia_mlp_select(ctx, net) bind(C, name='bia_mlp_select')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: net
     end function
 
     ! per-pair RKF45 tolerances of the following steps (the reference's 0 / 1e-16 by default)
