@@ -389,7 +389,14 @@ BIA_HDF void hyp_finish(const HypState& S, double th_end, const double yz[4], do
 }
 
 // the reference's best-iterate loop from its first Newton point th_end (residual err of the
-// start): :186-217.  Returns the best iterate; *ir the loop's evaluations.
+// start): :186-217.  Returns the best iterate; *ir the loop's evaluations (the reference's count).
+//
+// The residual is a function of the anomaly alone, and every residual the loop evaluated before
+// is in the history.  So when the anomaly to evaluate equals one of the last two the loop
+// evaluated (a Newton step that rounds to zero, or a two-cycle; 95 % of the exact-repeat exits
+// on the benchmark system), its residual would repeat a history entry and the reference loop
+// would exit on it: the exit is taken without the evaluation, counted as one, with the same
+// iterate, residual and flags.
 BIA_HDF double hyp_loop(const HypState& S, double th_end, double err, int itr, Hist hist, int* ir_out,
                         unsigned* fl, double* resid) {
   double d;
@@ -397,9 +404,16 @@ BIA_HDF double hyp_loop(const HypState& S, double th_end, double err, int itr, H
   int ir = 0;
   double err_stock = err, th_stock = 0;
   bool updated = false, cycled = false;
+  double th_p1 = __builtin_nan(""), th_p2 = __builtin_nan("");  // the last two evaluated anomalies
   while (ir < itr) {
     double th_now = th_end;
     ir++;
+    if (th_now == th_p1 || th_now == th_p2) {
+      cycled = true;
+      break;
+    }
+    th_p2 = th_p1;
+    th_p1 = th_now;
     th_end = hyp_newton(th_now, S.TE, S.P, &err, &d);
     for (int num = ir - 1; num >= 1; num--) {
       if (hist.get(num - 1) == err) {

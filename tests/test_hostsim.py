@@ -536,3 +536,41 @@ def test_reference_solvers_with_learned_starts_last_resort(hs):
           f"{itb[0]} evaluations, error {eb:.1e}")
     assert bra[0] == 0 and (fla[0] & 4) and (fla[0] & 1) and ea > 1
     assert (flb[0] & 128) and eb < 1e-10
+
+
+@pytest.mark.parametrize("case", ["fixture", "benchmark", "strong", "coulomb"])
+def test_hyperbola_loop_repeat_exit_without_the_evaluation(case, fixture1332):
+    """The library's hyperbola loop takes the reference's exact-repeat exit without evaluating an
+    anomaly that repeats one of the last two it evaluated (the residual is a function of the
+    anomaly, so it would repeat a history entry).  Against the reference loop with every
+    iteration evaluated (bia/find_dXYUV_kernel.cu:186-217), from the same set-up and guess
+    selection: best iterate, best residual, iteration count and flags bitwise equal on every
+    hyperbolic pair, and the shortcut taken on most exits."""
+    from bia_3d_gpu_amd import ic
+    if case == "fixture":
+        st, eps, dt, step = fixture1332["state"], fixture1332["eps"], fixture1332["dt"], 7
+    elif case == "benchmark":
+        st, eps, dt, step = ic.plummer_with_binaries(2048, 65536), -2 / 2048, 1e-3, 3
+    elif case == "strong":
+        st = ic.uniform_ball_virial(200, seed=5)
+        st[3:] *= 0.3
+        eps, dt, step = -2 / 200, 5e-3, 1
+    else:
+        st = ic.uniform_ball_virial(200, seed=6)
+        eps, dt, step = 0.05, 1e-2, 1  # repulsive: every pair hyperbolic, theta_0 < pi/2
+    rel = _pairs(st)[::step]
+    L = C.CDLL(LIB)
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C")
+    L.hs_hyp_loops.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_int, dp, dp, dp, ip, ip]
+    m = len(rel)
+    lib, ref = np.zeros((m, 4)), np.zeros((m, 4))
+    hyp, rep = np.zeros(m, np.int32), np.zeros(m, np.int32)
+    L.hs_hyp_loops(m, eps, dt, 18, np.ascontiguousarray(rel), lib, ref, hyp, rep)
+    h = hyp.astype(bool)
+    cyc = (ref[h, 3].astype(int) & 2) != 0
+    print(f"{case}: {h.sum()} hyperbolic pairs, {cyc.sum()} exact-repeat exits, {rep[h].sum()} on a repeated "
+          f"anomaly; evaluations {ref[h, 2].sum():.0f}")
+    assert h.sum() > 1000
+    assert np.array_equal(lib[h].view(np.int64), ref[h].view(np.int64))
+    assert rep[h].sum() > 0.5 * cyc.sum()
