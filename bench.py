@@ -116,10 +116,11 @@ def run_mode(ctx, st_host, eps, dt, itr, mode, steps, warmup, autotune=True):
     a, b = inv0.cpu().numpy(), inv1.cpu().numpy()
     e_0, e_1 = a[0] + a[1], b[0] + b[1]  # KE + PE = 2E/m
     n = s.shape[1]
-    nt = (n + 63) // 64
-    # the reduction reads every tile partial once (6 components x nt tiles x 64 nt slots) and the
-    # velocities, and writes the deltas (once per stage, but the last, for the running sums)
-    red_bytes = 8 * 6 * nt * nt * 64 + 24 * n + 48 * n + 2 * 48 * n * max(0, nstages - 1)
+    from bia_3d_gpu_amd import partial_doubles
+    # the reduction reads every partial once (bia_3d_gpu_amd.partial_doubles: the column partials
+    # of each block of four row tiles and the row partials of its items) and the velocities, and
+    # writes the deltas (once per stage, but the last, for the running sums)
+    red_bytes = 8 * partial_doubles(n) + 24 * n + 48 * n + 2 * 48 * n * max(0, nstages - 1)
     return dict(ms_per_step=ms, wall_s=wall, kernel_ms=kms, stats=st.tolist(), flops=flops, tuned=tuned,
                 tflops=flops / (kms / 1e3) / 1e12, dE_rel=abs(e_1 - e_0) / abs(e_0), dE_steps=steps,
                 first_delta=first_delta,

@@ -177,6 +177,33 @@ def reference_step(state_soa: np.ndarray, eps: float, dt: float, method: int = M
 # ------------------------------------------------------------------------------------------------
 # algorithmic work per step (flop convention: + - x / sqrt = 1, transcendental = 20)
 # ------------------------------------------------------------------------------------------------
+def partial_doubles(n: int, blocks: int | None = None) -> int:
+    """Doubles of the pair kernel's partial sums for one n-particle system (the first ``blocks``
+    blocks of four row tiles; all of them by default), as the library lays them out
+    (csrc/bia_step.hip, blk_off): block R holds its column partials for the columns from 256 R to
+    the padded end, 6 (npad - 256 R), then one row partial set per work item, 6 x 256 x (nb - R)."""
+    nt = (n + 63) // 64
+    npad, nb = 64 * nt, (nt + 3) // 4
+    r = nb if blocks is None else blocks
+    return 6 * (r * npad - 128 * r * (r - 1)) + 6 * 256 * (r * nb - r * (r - 1) // 2)
+
+
+def staging(n: int, limit_bytes: int) -> tuple[int, int]:
+    """(stages, workspace bytes) of one n-particle system under a workspace limit (0 = none): the
+    largest G blocks per stage whose first stage fits the limit (at least one block)."""
+    nt = (n + 63) // 64
+    nb = (nt + 3) // 4
+    full = 8 * partial_doubles(n)
+    if limit_bytes <= 0 or full <= limit_bytes:
+        return 1, full
+    g = 1
+    while g < nb and 8 * partial_doubles(n, g + 1) <= limit_bytes:
+        g += 1
+    if g >= nb:
+        return 1, full
+    return -(-nb // g), 8 * partial_doubles(n, g)
+
+
 def flops_per_step(stats, mode: int = MODE_PLAIN) -> float:
     """Algorithmic FP64 flops of one step from its BIA_STATS counters (+ - x / sqrt = 1 flop,
     each libm transcendental = 20).

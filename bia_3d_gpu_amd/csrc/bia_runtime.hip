@@ -104,7 +104,7 @@ int bia_destroy(bia_ctx* c) {
                          &c->inv_tiles, &c->inv_sys, &c->dim_work, &c->dim_ctl,
                          &c->mlp_w, &c->mlp_ws, &c->mlp_table, &c->alt_w, &c->alt_table};
   for (auto* b : bufs) b->release();
-  c->plan_tiles.release();
+  c->plan_items.release();
   c->plan_sys.release();
   c->plan_chunks.release();
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);

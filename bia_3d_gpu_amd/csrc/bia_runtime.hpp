@@ -16,22 +16,24 @@ namespace bia {
 constexpr int kProfSlots = 256;  // pair-kernel launches whose timing events are kept
 
 // one system of a (possibly batched) step: particles [off, off + n) of the SoA state, nt = ceil(n/64)
-// tiles, partial sums at part + poff laid out [nt][6][64 nt]
+// tiles, partial sums at part + poff (blocks of four row tiles, each its column partials and then
+// its row partials; bia_step.hip, blk_off)
 struct SysDesc {
   int off, n, nt, pad;
   long long poff;
   double eps;
 };
 
-// cached launch plan for a list of system sizes.  A single system whose tile partials would exceed
-// the context's workspace limit runs in nstages stages of stage_rows row tiles each (stage g takes
-// tiles [stage_t0[g], stage_t0[g + 1]) of the list); otherwise nstages = 1.
+// cached launch plan for a list of system sizes: the pair kernel's work items (block of four row
+// tiles, segment of four column tiles, system).  A single system whose partials would exceed the
+// context's workspace limit runs in nstages stages of stage_blocks blocks each (stage g takes items
+// [stage_t0[g], stage_t0[g + 1]) of the list); otherwise nstages = 1.
 struct Plan {
   std::vector<int> sizes;
   size_t limit = 0;  // the workspace limit the plan was made for
-  int nsys = 0, ntiles = 0, nchunks = 0, ntotal = 0;
+  int nsys = 0, nitems = 0, nchunks = 0, ntotal = 0;
   size_t part_doubles = 0;
-  int nstages = 1, stage_rows = 0;
+  int nstages = 1, stage_blocks = 0;
   std::vector<int> stage_t0;
   std::vector<SysDesc> hsys;  // host copy (eps updated per call)
 };
@@ -117,7 +119,7 @@ struct bia_ctx {
   bia::DevBuf part, state, delta, stat_slots;
   // batched plans: device tile list (int4 I, J, sys, 0), system descriptors, reduce chunks
   bia::Plan plan;
-  bia::DevBuf plan_tiles, plan_sys, plan_chunks;
+  bia::DevBuf plan_items, plan_sys, plan_chunks;
   std::map<std::vector<int>, std::pair<int, int>> tuned_grid;  // autotuner cache: (method, mode, coupling, sizes) -> (grid, occ)
   // multi-step driver graph; plan_gen counts the rebuilds of the plan's device buffers (the
   // graph's kernel arguments point into them, so a graph is only replayed on the generation it

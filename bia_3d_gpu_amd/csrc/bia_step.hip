@@ -2,18 +2,23 @@
 // the multi-step driver (hipGraph), the launch autotuner, and the reference-compatible
 // per-particle entry point find_dxyuv_kernel_.
 //
-// Tiling.  The i<j triangle of each system is cut into 64x64 tiles (I <= J).  One wave owns one
-// tile: lane l holds row particle i = 64 I + l in registers, the column tile sits in LDS, and
-// the lanes walk the columns in a skewed order so that every lane works on a different column at
-// every step and the column sums rotate with their columns (k_tiles).  Row sums stay in registers.
-// Results go to a per-system partial buffer part[K][c][i] in which every (tile K, particle i)
-// slot has exactly one writer:
-//   tile (I,J), I<J : rows of I  -> part[J][c][i],  columns of J -> part[I][c][j]
-//   tile (I,I)      : both sides -> part[I][c][i]
-// so there are no atomics and nothing to clear, and a second kernel sums part over K in a
-// fixed order (deterministic).  The drift v dt is added there.
+// Tiling.  The i<j triangle of each system is cut into 64x64 tiles (I <= J), and the row tiles
+// into blocks of kBlk = 4 (block R: row tiles 4R .. 4R+3, particles 256R .. 256R+255).  A work item
+// is (block R, segment k): the column tiles J in [4R + 4k, 4R + 4k + 4).  One workgroup of four
+// waves takes an item; wave w holds row tile I = 4R + w, lane l row particle i = 64 I + l in
+// registers, and the four waves walk the item's column tiles together: for each J, every wave
+// with a tile (I <= J) computes it, its column sums go to its own LDS row, and after a barrier
+// the four rows are added in wave order into one column partial per (block, column).  The row
+// sums stay in registers across the item's four column tiles.  So every slot of the per-system
+// partial buffer has exactly one writer:
+//   block R, column j >= 256 R : colpart(R)[c][j]   (the block's four row tiles, added in order)
+//   block R, item k, row i     : rowpart(R)[k][c][i]
+// with no atomics and nothing to clear, and a second kernel sums, for particle i of block R, the
+// column partials of blocks 0..R and then its own block's row partials, in that fixed order
+// (deterministic).  The drift v dt is added there.  Against one partial per tile and side, a
+// block of four row tiles and a segment of four column tiles write a quarter of the partials.
 //
-// Batching.  The tiles of all systems form one list (I, J, system); the pair kernel is a
+// Batching.  The items of all systems form one list (R, k, system); the pair kernel is a
 // persistent grid-stride launch over it, so a batch of systems of uneven N is one launch.
 //
 // Couplings.  SCALAR: the reference's r'' = eps r/|r|^3 with eps per system and the equal-mass
@@ -32,8 +37,28 @@
 namespace bia {
 
 constexpr int kWave = 64;
+constexpr int kBlk = 4;                 // row tiles of a block = waves of a pair-kernel workgroup
+constexpr int kSeg = 4;                 // column tiles of a work item
+constexpr int kGroup = kWave * kBlk;    // threads of a pair-kernel workgroup = particles of a block
+static_assert(kSeg == kBlk, "a block's items are nb - R: one per kSeg column tiles from its diagonal");
 constexpr int kStatSlots = 4096;  // statistics slots of the pair kernel (a power of two)
-constexpr int kMaxGrid64 = (int)(0xffffffffu / kWave);  // largest grid of 64-lane workgroups
+constexpr int kMaxGrid = (int)(0xffffffffu / kGroup);  // largest grid of pair-kernel workgroups
+
+// Partial buffer of one system of nt tiles (npad = 64 nt, nb = ceil(nt / 4) blocks): block R
+// holds its column partials colpart[6][W_R] for the columns j in [256 R, npad), W_R = npad - 256 R,
+// then its row partials rowpart[nb - R][6][256] (one per item of the block).  blk_off(R) is where
+// block R starts, blk_off(nb) the system's total (about 12 N nt / 8 doubles: a quarter of one
+// partial per tile and side).
+__host__ __device__ constexpr long long blk_w(int npad, int R) { return (long long)npad - (long long)kGroup * R; }
+__host__ __device__ constexpr long long blk_off(int npad, int nb, int R) {
+  const long long r = R;
+  return 6 * (r * npad - (long long)(kGroup / 2) * r * (r - 1)) + 6LL * kGroup * (r * nb - r * (r - 1) / 2);
+}
+__host__ __device__ constexpr int n_blocks(int nt) { return (nt + kBlk - 1) / kBlk; }
+// items of the blocks before block R (a block has nb - R items)
+__host__ __device__ constexpr long long items_before(int nb, int R) {
+  return (long long)R * nb - (long long)R * (R - 1) / 2;
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -81,8 +106,9 @@ struct PairStats {
     }
     tc = ti = 0;
   }
-  // the workgroup's totals into its slot: workgroup b uses slot b mod kStatSlots, which spreads
-  // a launch's atomics over kStatSlots x 16 addresses instead of 16; k_stats_fold adds the slots up
+  // each wave's totals into its workgroup's slot: workgroup b uses slot b mod kStatSlots, which
+  // spreads a launch's atomics over kStatSlots x 16 addresses instead of 16; k_stats_fold adds the
+  // slots up
   __device__ void flush(unsigned long long* slots) {
     unsigned v[16];
     for (int k = 0; k < 4; k++) {
@@ -91,7 +117,7 @@ struct PairStats {
     }
     for (int k = 0; k < 8; k++) v[8 + k] = wave_sum_u(fl[k]);
     unsigned long long* out = slots + (size_t)(blockIdx.x & (kStatSlots - 1)) * BIA_STATS_LEN;
-    if (threadIdx.x == 0)
+    if (threadIdx.x % kWave == 0)
       for (int k = 0; k < 16; k++)
         if (v[k]) atomicAdd(out + k, (unsigned long long)v[k]);
   }
@@ -138,44 +164,52 @@ __device__ __forceinline__ void lds_add(double* p, double x) {
 // OCC: minimum waves per SIMD asked of the register allocator (1 = its own choice; 4 = at most 128
 // VGPRs; 5 = at most 96, for the Kepler-drift modes); picked per shape by the autotuner.
 //
-// One wave per tile.  Lane l holds row i = 64 I + l.  Plain mode walks the columns in order
-// (below).  The Kepler modes stage the column tile's states in LDS and at step k lane l takes
-// column j = (l + k) mod 64, so the 64 lanes work on 64 different
-// columns and no column needs a cross-lane reduction: each lane adds its pair's column share
-// into the column's LDS sum (lds_add; one address per lane per step, the steps in order).  A
-// diagonal tile (I = J) takes steps k = 1..32 (at k = 32 lanes 0..31 only), which covers each
-// unordered pair once.
-// The tiles one launch of the pair kernel takes, [t0, t1) of the plan's list.  Unstaged (rb NULL):
-// the whole list, partials at part[K][c][i] with K = J for the rows of tile (I, J) and K = I for
-// its columns (and the diagonal tile's both sides).  Staged: one system, the tiles of row tiles
-// [g0, g0 + G); the columns and diagonal go to part[I - g0][c][i] and the rows to
-// rb[I - g0][c][J][lane] (k_reduce_stage sums them in the order k_reduce would).
+// One workgroup of four waves per work item (block R, segment k); wave w holds row tile
+// I = 4R + w, lane l row i = 64 I + l.  For each column tile J of the segment, every wave with a
+// tile (I <= J) computes it:
+//  * reference solvers (plain modes, RKF45): divergent and register-heavy, so the columns are
+//    walked in order (j wave-uniform, its state through scalar loads) and each column sum is a
+//    wave reduction, kept by lane j; the wave's LDS residual history is free between tiles, and
+//    its first six rows take the tile's column sums;
+//  * Kepler modes: the column tile's states are staged in LDS by the whole workgroup and at step
+//    k lane l takes column j = (l + k) mod 64, so the 64 lanes work on 64 different columns and no
+//    column needs a cross-lane reduction: each lane adds its pair's column share into the
+//    column's sum in its wave's LDS row (lds_add; one address per lane per step, the steps in
+//    order).  A diagonal tile (I = J) takes steps k = 1..32 (at k = 32 lanes 0..31 only), which
+//    covers each unordered pair once.
+// Then the four waves' column sums are added in wave order into the block's column partial of J.
+// Stage: the items [t0, t1) of the list this launch takes; staged (one system), the partials of
+// blocks g0 and up, relative to block g0 (k_reduce_stage sums them in the order k_reduce would).
 struct Stage {
-  int t0, t1, g0;
-  double* rb;
+  int t0, t1, g0, staged;
 };
 
 template <int METHOD, int MODE, int COUP, bool STATS, int OCC>
-__global__ __launch_bounds__(kWave, OCC) void k_tiles(Stage stg, int lds, const int4* __restrict__ tiles,
-                                                 const SysDesc* __restrict__ sys, const double* __restrict__ st,
-                                                 const double* __restrict__ mass, const double* __restrict__ charge,
-                                                 double coup, double dt, int itr, double* __restrict__ part,
-                                                 unsigned long long* __restrict__ stats, LearnTable T, RkfTol tol,
-                                                 PairLog lg) {
-  __shared__ double hist_lds[kMaxItr * kWave];
-  __shared__ double cst[6][kWave];  // column tile state
+__global__ __launch_bounds__(kGroup, OCC) void k_tiles(Stage stg, int lds, const int4* __restrict__ items,
+                                                  const SysDesc* __restrict__ sys, const double* __restrict__ st,
+                                                  const double* __restrict__ mass, const double* __restrict__ charge,
+                                                  double coup, double dt, int itr, double* __restrict__ part,
+                                                  unsigned long long* __restrict__ stats, LearnTable T, RkfTol tol,
+                                                  PairLog lg) {
+  constexpr bool REF = reference_solvers(MODE);
+  // per wave: the reference iteration's residual history (reference solvers; its first six rows
+  // take the wave's column sums between tiles), or the column sums (Kepler modes)
+  __shared__ double wrow[kBlk][REF ? kMaxItr : 6][kWave];
+  __shared__ double cst[6][kWave];  // column tile state (Kepler modes)
   __shared__ double cmq[3][kWave];  // column masses, charges and reciprocal masses (couplings only)
-  __shared__ double csum[6][kWave];  // column sums of the tile's debits (Kepler modes)
-  const int lane = threadIdx.x;
-  Hist hist{hist_lds + lane, kWave};
+  // the wave's index, in a scalar register (it is the same for the wave's 64 lanes)
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
+  Hist hist{&wrow[w][0][lane], kWave};
   PairStats ps;
   if (STATS) ps.clear();
   for (int t = stg.t0 + blockIdx.x; t < stg.t1; t += gridDim.x) {
-    const int4 td = tiles[t];
-    const SysDesc S = sys[td.z];
-    const int I = td.x, J = td.y;
+    const int4 item = items[t];
+    const SysDesc S = sys[item.z];
+    const int R = item.x, seg = item.y;
+    const int I = kBlk * R + w;
+    const bool wave_ok = I < S.nt;  // wave-uniform: the last block may hold fewer than four row tiles
     const int il = I * kWave + lane;  // local row index
-    const bool row_ok = il < S.n;
+    const bool row_ok = wave_ok && il < S.n;
     const int gi = S.off + il;
     double ri[6];
 #pragma unroll
@@ -188,146 +222,155 @@ __global__ __launch_bounds__(kWave, OCC) void k_tiles(Stage stg, int lds, const 
         rmi = 1 / mi;
       }
     }
-    const int j0 = J * kWave;
-    const int jn = min(kWave, S.n - j0);
-    const int ldp = S.nt * kWave;
-    double* P = part + S.poff;
-    const int Ik = I - stg.g0;  // the column / diagonal slot of this tile's row tile
-    // where row component c of this tile goes: part[J][c][i], or rb[I - g0][c][J][lane] staged
-    auto row_slot = [&](int c) -> double* {
-      return stg.rb ? stg.rb + (((size_t)Ik * 6 + c) * S.nt + J) * kWave + lane : P + ((size_t)J * 6 + c) * ldp + il;
-    };
-    if constexpr (reference_solvers(MODE)) {
-      // reference solvers: divergent and register-heavy, so the columns are walked in order (j
-      // wave-uniform, its state through scalar loads) and each column sum is a wave reduction
-      double acc[6] = {0, 0, 0, 0, 0, 0}, col[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll 1
-      for (int jj = 0; jj < jn; ++jj) {
-        const int gj = S.off + j0 + jj;
-        double rj[6];
-#pragma unroll
-        for (int c = 0; c < 6; c++) rj[c] = st[(size_t)c * lds + gj];
-        double d[6] = {0, 0, 0, 0, 0, 0};
-        double wi = 1, wj = 1;
-        if (row_ok && (I != J || lane < jj)) {
-          double eps = S.eps;
-          if (COUP != kScalar) {
-            const double mj = mass ? mass[gj] : 1.0;
-            if (COUP == kGravity) eps = -coup * (mi + mj);
-            else eps = coup * qi * charge[gj] * (1 / mi + 1 / mj);
-            wi = 2 * mj / (mi + mj);
-            wj = 2 * mi / (mi + mj);
-          }
-          double rel[6];
-#pragma unroll
-          for (int c = 0; c < 6; c++) rel[c] = ri[c] - rj[c];
-          int br = 0, it = 0;
-          unsigned fl = 0;
-          double res = 0;
-          pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T, tol, &res);
-          if (STATS) {
-            ps.add(br, it, fl);
-            if (lg.count && (fl & kLogMask)) log_pair(lg, td.z, il, j0 + jj, br, it, fl, res);
-          }
-        }
-#pragma unroll
-        for (int c = 0; c < 6; c++) {
-          acc[c] += wi * d[c];
-          const double sum = wave_sum(wj * d[c]);
-          if (lane == jj) col[c] = -sum;
-        }
-      }
-      if (I == J) {
-#pragma unroll
-        for (int c = 0; c < 6; c++) put_partial(P + ((size_t)Ik * 6 + c) * ldp + il, acc[c] + col[c]);
-      } else {
-#pragma unroll
-        for (int c = 0; c < 6; c++) {
-          put_partial(row_slot(c), acc[c]);
-          put_partial(P + ((size_t)Ik * 6 + c) * ldp + j0 + lane, col[c]);
-        }
-      }
-      continue;
-    }
-    __syncthreads();  // the previous tile's readers are done with the LDS column tile
-    {
-      const bool ok = lane < jn;
-      const int gj = S.off + j0 + lane;
-#pragma unroll
-      for (int c = 0; c < 6; c++) {
-        cst[c][lane] = ok ? st[(size_t)c * lds + gj] : 0.0;
-        csum[c][lane] = 0.0;
-      }
-      if (COUP != kScalar) {
-        const double mj = (ok && mass) ? mass[gj] : 1.0;
-        cmq[0][lane] = mj;
-        cmq[1][lane] = (ok && COUP == kCoulomb) ? charge[gj] : 0.0;
-        if (COUP == kCoulomb) cmq[2][lane] = 1 / mj;
-      }
-    }
-    __syncthreads();
+    const int npad = S.nt * kWave, nb = n_blocks(S.nt);
+    double* B = part + (stg.staged ? blk_off(npad, nb, R) - blk_off(npad, nb, stg.g0) : S.poff + blk_off(npad, nb, R));
+    const long long WR = blk_w(npad, R);
+    const int Ja = kBlk * R + kSeg * seg, Jb = min(Ja + kSeg, S.nt);
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    const bool diag = (I == J);
-    const int k0 = diag ? 1 : 0, k1 = diag ? 33 : kWave;
 #pragma unroll 1
-    for (int k = k0; k < k1; ++k) {
-      const int jl = (lane + k) & (kWave - 1);
-      const bool active = row_ok & (jl < jn) & (!diag | (k < 32) | (lane < 32));
-      if (active) {
-        double d[6];
-        double wi = 1, wj = 1;  // multiples of the half deltas credited to i and debited from j
-        double eps = S.eps;
-        if (COUP != kScalar) {
-          const double mj = cmq[0][jl];
-          if (COUP == kGravity) eps = -coup * (mi + mj);
-          else eps = coup * qi * cmq[1][jl] * (rmi + cmq[2][jl]);
-          // the centre-of-mass split 2 m_j/(m_i+m_j), 2 m_i/(m_i+m_j) from one reciprocal (within
-          // an ulp); equal masses split exactly in half, as the scalar coupling does
-          const double rs = fast_rcp(mi + mj);
-          wi = (mi == mj) ? 1.0 : 2 * mj * rs;
-          wj = (mi == mj) ? 1.0 : 2 * mi * rs;
-        }
-        double rel[6];
+    for (int J = Ja; J < Jb; ++J) {
+      const int j0 = J * kWave;
+      const int jn = min(kWave, S.n - j0);
+      const bool tile = wave_ok && I <= J;  // wave-uniform
+      // the last wave with a tile in column tile J
+      const int wlast = min(min(kBlk - 1, J - kBlk * R), S.nt - 1 - kBlk * R);
+      if constexpr (REF) {
+        if (tile) {
+          double col[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+          for (int jj = 0; jj < jn; ++jj) {
+            const int gj = S.off + j0 + jj;
+            double rj[6];
 #pragma unroll
-        for (int c = 0; c < 6; c++) rel[c] = ri[c] - cst[c][jl];
-        int br = 0, it = 0;
-        unsigned fl = 0;
-        double res = 0;
-        pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T, tol, &res);
-        if (STATS) {
-          ps.add_tile(br, it, fl);
-          if (lg.count && (fl & kLogMask)) log_pair(lg, td.z, il, j0 + jl, br, it, fl, res);
-        }
+            for (int c = 0; c < 6; c++) rj[c] = st[(size_t)c * lds + gj];
+            double d[6] = {0, 0, 0, 0, 0, 0};
+            double wi = 1, wj = 1;
+            if (row_ok && (I != J || lane < jj)) {
+              double eps = S.eps;
+              if (COUP != kScalar) {
+                const double mj = mass ? mass[gj] : 1.0;
+                if (COUP == kGravity) eps = -coup * (mi + mj);
+                else eps = coup * qi * charge[gj] * (1 / mi + 1 / mj);
+                wi = 2 * mj / (mi + mj);
+                wj = 2 * mi / (mi + mj);
+              }
+              double rel[6];
 #pragma unroll
-        for (int c = 0; c < 6; c++) {
-          if (COUP == kScalar) {
-            acc[c] += d[c];
-            lds_add(&csum[c][jl], d[c]);
-          } else {
-            acc[c] += wi * d[c];
-            lds_add(&csum[c][jl], wj * d[c]);
+              for (int c = 0; c < 6; c++) rel[c] = ri[c] - rj[c];
+              int br = 0, it = 0;
+              unsigned fl = 0;
+              double res = 0;
+              pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T, tol, &res);
+              if (STATS) {
+                ps.add(br, it, fl);
+                if (lg.count && (fl & kLogMask)) log_pair(lg, item.z, il, j0 + jj, br, it, fl, res);
+              }
+            }
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+              acc[c] += wi * d[c];
+              const double sum = wave_sum(wj * d[c]);
+              if (lane == jj) col[c] = sum;
+            }
           }
+          // the history is free between tiles: its first rows take the column sums
+#pragma unroll
+          for (int c = 0; c < 6; c++) wrow[w][c][lane] = col[c];
         }
+        __syncthreads();
+      } else {
+        __syncthreads();  // the previous column tile's readers are done with cst, cmq and wrow
+        for (int q = threadIdx.x; q < 6 * kWave; q += kGroup) {
+          const int c = q / kWave, l = q % kWave;
+          cst[c][l] = l < jn ? st[(size_t)c * lds + S.off + j0 + l] : 0.0;
+        }
+        if (COUP != kScalar && threadIdx.x < kWave) {
+          const bool ok = lane < jn;
+          const double mj = (ok && mass) ? mass[S.off + j0 + lane] : 1.0;
+          cmq[0][lane] = mj;
+          cmq[1][lane] = (ok && COUP == kCoulomb) ? charge[S.off + j0 + lane] : 0.0;
+          if (COUP == kCoulomb) cmq[2][lane] = 1 / mj;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) wrow[w][c][lane] = 0.0;
+        __syncthreads();
+        if (tile) {
+          const bool diag = (I == J);
+          const int k0 = diag ? 1 : 0, k1 = diag ? 33 : kWave;
+#pragma unroll 1
+          for (int k = k0; k < k1; ++k) {
+            const int jl = (lane + k) & (kWave - 1);
+            const bool active = row_ok & (jl < jn) & (!diag | (k < 32) | (lane < 32));
+            if (active) {
+              double d[6];
+              double wi = 1, wj = 1;  // multiples of the half deltas credited to i and debited from j
+              double eps = S.eps;
+              if (COUP != kScalar) {
+                const double mj = cmq[0][jl];
+                if (COUP == kGravity) eps = -coup * (mi + mj);
+                else eps = coup * qi * cmq[1][jl] * (rmi + cmq[2][jl]);
+                // the centre-of-mass split 2 m_j/(m_i+m_j), 2 m_i/(m_i+m_j) from one reciprocal
+                // (within an ulp); equal masses split exactly in half, as the scalar coupling does
+                const double rs = fast_rcp(mi + mj);
+                wi = (mi == mj) ? 1.0 : 2 * mj * rs;
+                wj = (mi == mj) ? 1.0 : 2 * mi * rs;
+              }
+              double rel[6];
+#pragma unroll
+              for (int c = 0; c < 6; c++) rel[c] = ri[c] - cst[c][jl];
+              int br = 0, it = 0;
+              unsigned fl = 0;
+              double res = 0;
+              pair_eval<METHOD, MODE>(eps, dt, itr, rel, d, &br, &it, &fl, hist, T, tol, &res);
+              if (STATS) {
+                ps.add_tile(br, it, fl);
+                if (lg.count && (fl & kLogMask)) log_pair(lg, item.z, il, j0 + jl, br, it, fl, res);
+              }
+#pragma unroll
+              for (int c = 0; c < 6; c++) {
+                if (COUP == kScalar) {
+                  acc[c] += d[c];
+                  lds_add(&wrow[w][c][jl], d[c]);
+                } else {
+                  acc[c] += wi * d[c];
+                  lds_add(&wrow[w][c][jl], wj * d[c]);
+                }
+              }
+            }
+          }
+          if (STATS) ps.fold_tile();
+        }
+        __syncthreads();  // the column sums are complete
       }
+      // the waves' column sums of tile column J, added in wave order, debited from the columns.
+      // The thread index goes through an empty asm statement so that the compiler computes these
+      // addresses here, not once before the loops (where they would hold registers across the
+      // pair loop and cost the Kepler modes a wave per SIMD)
+      int q0 = threadIdx.x;
+      asm volatile("" : "+v"(q0));
+      for (int q = q0; q < 6 * kWave; q += kGroup) {
+        const int c = q / kWave, j = q % kWave;
+        double v = wrow[0][c][j];
+        for (int ww = 1; ww <= wlast; ++ww) v += wrow[ww][c][j];
+        put_partial(B + c * WR + (j0 - kGroup * R) + j, -v);
+      }
+      if (REF) __syncthreads();  // the history rows are the residual history again
     }
-    if (STATS) ps.fold_tile();
-    __syncthreads();  // the column sums are complete
-    if (diag) {
+    if (wave_ok) {
+      int l = lane;  // as above: the row partials' addresses are computed here
+      asm volatile("" : "+v"(l));
+      double* rp = B + 6 * WR + (size_t)seg * 6 * kGroup + w * kWave + l;
 #pragma unroll
-      for (int c = 0; c < 6; c++) put_partial(P + ((size_t)Ik * 6 + c) * ldp + il, acc[c] - csum[c][lane]);
-    } else {
-#pragma unroll
-      for (int c = 0; c < 6; c++) {
-        put_partial(row_slot(c), acc[c]);
-        put_partial(P + ((size_t)Ik * 6 + c) * ldp + j0 + lane, -csum[c][lane]);
-      }
+      for (int c = 0; c < 6; c++) put_partial(rp + c * kGroup, acc[c]);
     }
   }
   if (STATS) ps.flush(stats);
 }
 
-// delta[c][g] = sum_K part[K][c][local] (+ v dt for positions), K ascending; one block per
-// 256-particle chunk {system, first local index}
+// Particle i (local index il of block R = il / 256) of a system: the column partials of blocks
+// 0..R at i in block order, then the row partials of block R's items in item order (+ v dt for
+// positions).  One workgroup per 256-particle chunk {system, first local index}, which is one
+// block; one grid row per component.
 __global__ __launch_bounds__(256) void k_reduce(const int2* __restrict__ chunks, const SysDesc* __restrict__ sys,
                                                 int lds, int ldd, double dt, const double* __restrict__ st,
                                                 const double* __restrict__ part, double* __restrict__ delta) {
@@ -336,69 +379,79 @@ __global__ __launch_bounds__(256) void k_reduce(const int2* __restrict__ chunks,
   const int il = ch.y + threadIdx.x;
   if (il >= S.n) return;
   const int c = blockIdx.y;
-  const int ldp = S.nt * kWave;
-  const double* p = part + S.poff + (size_t)c * ldp + il;
-  const size_t stride = (size_t)6 * ldp;
+  const int R = ch.y / kGroup;
+  const int npad = S.nt * kWave, nb = n_blocks(S.nt);
+  const double* P = part + S.poff;
   double s = 0;
-  int K = 0;
-  // 16 loads in flight per thread, then the adds in K order (the sum is the sequential one)
-  for (; K + 16 <= S.nt; K += 16) {
+  // 16 loads in flight per thread, then the adds in order (the sum is the sequential one)
+  int r = 0;
+  for (; r + 16 <= R + 1; r += 16) {
     double v[16];
 #pragma unroll
-    for (int u = 0; u < 16; u++) v[u] = __builtin_nontemporal_load(p + (size_t)(K + u) * stride);
+    for (int u = 0; u < 16; u++)
+      v[u] = __builtin_nontemporal_load(P + blk_off(npad, nb, r + u) + c * blk_w(npad, r + u) + (il - kGroup * (r + u)));
 #pragma unroll
     for (int u = 0; u < 16; u++) s += v[u];
   }
-  for (; K < S.nt; ++K) s += __builtin_nontemporal_load(p + (size_t)K * stride);
+  for (; r <= R; ++r) s += __builtin_nontemporal_load(P + blk_off(npad, nb, r) + c * blk_w(npad, r) + (il - kGroup * r));
+  const double* q = P + blk_off(npad, nb, R) + 6 * blk_w(npad, R) + (size_t)c * kGroup + threadIdx.x;
+  const int nk = nb - R;
+  int k = 0;
+  for (; k + 16 <= nk; k += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = __builtin_nontemporal_load(q + (size_t)(k + u) * 6 * kGroup);
+#pragma unroll
+    for (int u = 0; u < 16; u++) s += v[u];
+  }
+  for (; k < nk; ++k) s += __builtin_nontemporal_load(q + (size_t)k * 6 * kGroup);
   const int g = S.off + il;
   if (c < 3) s += st[(size_t)(3 + c) * lds + g] * dt;
   delta[(size_t)c * ldd + g] = s;
 }
 
-// Staged reduction of one system (stage of row tiles [g0, g0 + G), G = gs): particle i of tile
-// R >= g0 continues its running sum in delta (0 at the first stage) with its slots of this stage
-// in ascending K, which is the order of k_reduce's sum: K in [g0, g0 + G) from part[K - g0] for a
-// later tile; for a tile of the stage, K in [g0, R) from part, the diagonal part[R - g0], then the
-// row slots K in (R, nt) from rb[R - g0][c][K]; that particle's sum is then complete and gets its
-// drift.  One block per 256 particles from 64 g0 on, one grid row per component.
+// Staged reduction of one system (stage of blocks [g0, g0 + gs), partials relative to block g0):
+// particle i of block R >= g0 continues its running sum in delta (0 at the first stage) with the
+// column partials of the stage's blocks g0 .. min(R, g0 + gs - 1), which continues k_reduce's
+// block order; for a block of the stage, then its row partials in item order, after which that
+// particle's sum is complete and gets its drift.  One workgroup per block from g0 on, one grid
+// row per component.
 __global__ __launch_bounds__(256) void k_reduce_stage(int n, int nt, int g0, int gs, int first, int lds, int ldd,
                                                       double dt, const double* __restrict__ st,
-                                                      const double* __restrict__ part, const double* __restrict__ rb,
-                                                      double* __restrict__ delta) {
-  const int i = g0 * kWave + blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+                                                      const double* __restrict__ part, double* __restrict__ delta) {
+  const int R = g0 + blockIdx.x;
+  const int il = R * kGroup + threadIdx.x;
+  if (il >= n) return;
   const int c = blockIdx.y;
-  const int R = i / kWave, l = i % kWave;
-  const int ldp = nt * kWave;
-  const bool own = R < g0 + gs;  // a tile of this stage: its sum completes here
-  const int kend = own ? R - g0 : gs;
-  double s = first ? 0.0 : delta[(size_t)c * ldd + i];
-  const double* p = part + (size_t)c * ldp + i;
-  const size_t stride = (size_t)6 * ldp;
-  int k = 0;
-  for (; k + 8 <= kend; k += 8) {
+  const int npad = nt * kWave, nb = n_blocks(nt);
+  const double* P = part - blk_off(npad, nb, g0);  // block offsets relative to the stage's first
+  double s = first ? 0.0 : delta[(size_t)c * ldd + il];
+  const int rend = min(R, g0 + gs - 1);
+  int r = g0;
+  for (; r + 8 <= rend + 1; r += 8) {
     double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) v[u] = __builtin_nontemporal_load(p + (size_t)(k + u) * stride);
+    for (int u = 0; u < 8; u++)
+      v[u] = __builtin_nontemporal_load(P + blk_off(npad, nb, r + u) + c * blk_w(npad, r + u) + (il - kGroup * (r + u)));
 #pragma unroll
     for (int u = 0; u < 8; u++) s += v[u];
   }
-  for (; k < kend; ++k) s += __builtin_nontemporal_load(p + (size_t)k * stride);
-  if (own) {
-    s += __builtin_nontemporal_load(p + (size_t)kend * stride);  // the diagonal tile
-    const double* q = rb + (((size_t)(R - g0) * 6 + c) * nt) * kWave + l;
-    int K = R + 1;
-    for (; K + 8 <= nt; K += 8) {
+  for (; r <= rend; ++r) s += __builtin_nontemporal_load(P + blk_off(npad, nb, r) + c * blk_w(npad, r) + (il - kGroup * r));
+  if (R < g0 + gs) {
+    const double* q = P + blk_off(npad, nb, R) + 6 * blk_w(npad, R) + (size_t)c * kGroup + threadIdx.x;
+    const int nk = nb - R;
+    int k = 0;
+    for (; k + 8 <= nk; k += 8) {
       double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = __builtin_nontemporal_load(q + (size_t)(K + u) * kWave);
+      for (int u = 0; u < 8; u++) v[u] = __builtin_nontemporal_load(q + (size_t)(k + u) * 6 * kGroup);
 #pragma unroll
       for (int u = 0; u < 8; u++) s += v[u];
     }
-    for (; K < nt; ++K) s += __builtin_nontemporal_load(q + (size_t)K * kWave);
-    if (c < 3) s += st[(size_t)(3 + c) * lds + i] * dt;
+    for (; k < nk; ++k) s += __builtin_nontemporal_load(q + (size_t)k * 6 * kGroup);
+    if (c < 3) s += st[(size_t)(3 + c) * lds + il] * dt;
   }
-  delta[(size_t)c * ldd + i] = s;
+  delta[(size_t)c * ldd + il] = s;
 }
 
 // state += delta (the update of bia/bia.f90:203-204), SoA [6][ld], first n columns
@@ -485,52 +538,49 @@ static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
     Q.sizes = sizes;
     Q.limit = ctx->workspace_limit;
     Q.nsys = a.nsys;
-    std::vector<int4> tiles;
+    std::vector<int4> items;  // the work items (R, k, system)
     std::vector<int2> chunks;
     long long off = 0, poff = 0;
     for (int s = 0; s < a.nsys; s++) {
       const int n = sizes[s];
       if (n < 1) return BIA_ERR_ARG;
-      const int nt = (n + kWave - 1) / kWave;
+      const int nt = (n + kWave - 1) / kWave, nb = n_blocks(nt);
       SysDesc d{(int)off, n, nt, 0, poff, 0.0};
       Q.hsys.push_back(d);
-      for (int I = 0; I < nt; I++)
-        for (int J = I; J < nt; J++) tiles.push_back(make_int4(I, J, s, 0));
-      for (int c0 = 0; c0 < n; c0 += 256) chunks.push_back(make_int2(s, c0));
+      for (int R = 0; R < nb; R++)
+        for (int k = 0; k < nb - R; k++) items.push_back(make_int4(R, k, s, 0));
+      for (int c0 = 0; c0 < n; c0 += kGroup) chunks.push_back(make_int2(s, c0));  // one chunk per block
       off += n;
-      poff += (long long)nt * 6 * nt * kWave;
+      poff += blk_off(nt * kWave, nb, nb);
     }
-    if (off > 0x7fffffffLL || tiles.size() > 0x7fffffffULL) return BIA_ERR_ARG;
+    if (off > 0x7fffffffLL || items.size() > 0x7fffffffULL) return BIA_ERR_ARG;
     Q.ntotal = (int)off;
-    Q.ntiles = (int)tiles.size();
+    Q.nitems = (int)items.size();
     Q.nchunks = (int)chunks.size();
     Q.part_doubles = (size_t)poff;
-    Q.stage_t0 = {0, Q.ntiles};
+    Q.stage_t0 = {0, Q.nitems};
     if (a.nsys == 1 && ctx->workspace_limit > 0 && Q.part_doubles * sizeof(double) > ctx->workspace_limit) {
-      // stages of G row tiles: partials of 12 G N_pad doubles (columns and diagonals G x 6 x N_pad,
-      // rows G x 6 x nt x 64)
-      const int nt = Q.hsys[0].nt;
-      const size_t per_row = (size_t)12 * nt * kWave * sizeof(double);
-      const int G = (int)std::max<size_t>(1, ctx->workspace_limit / per_row);
-      if (G < nt) {
-        Q.stage_rows = G;
-        Q.nstages = (nt + G - 1) / G;
+      // stages of G blocks: the first stage's blocks hold the most partials (a block's column
+      // partials shrink with R), blk_off(G) doubles, which the limit bounds (at least one block)
+      const int nt = Q.hsys[0].nt, nb = n_blocks(nt), npad = nt * kWave;
+      int G = 1;
+      while (G < nb && (size_t)blk_off(npad, nb, G + 1) * sizeof(double) <= ctx->workspace_limit) ++G;
+      if (G < nb) {
+        Q.stage_blocks = G;
+        Q.nstages = (nb + G - 1) / G;
         Q.stage_t0.clear();
-        for (int g = 0; g <= Q.nstages; g++) {
-          const long long I0 = std::min(nt, g * G);
-          Q.stage_t0.push_back((int)(I0 * nt - I0 * (I0 - 1) / 2));  // tiles of rows < I0
-        }
-        Q.part_doubles = (size_t)12 * G * nt * kWave;
+        for (int g = 0; g <= Q.nstages; g++) Q.stage_t0.push_back((int)items_before(nb, std::min(nb, g * G)));
+        Q.part_doubles = (size_t)blk_off(npad, nb, G);
       }
     }
     P = Plan();  // invalid until every upload below has succeeded
     drop_graph(ctx);
     ctx->plan_gen++;
     BIA_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    BIA_TRY(ctx, ctx->plan_tiles.ensure(tiles.size() * sizeof(int4)));
+    BIA_TRY(ctx, ctx->plan_items.ensure(items.size() * sizeof(int4)));
     BIA_TRY(ctx, ctx->plan_chunks.ensure(chunks.size() * sizeof(int2)));
     BIA_TRY(ctx, ctx->plan_sys.ensure(Q.hsys.size() * sizeof(SysDesc)));
-    BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_tiles.p, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+    BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
     BIA_HIP_TRY(ctx, hipMemcpy(ctx->plan_chunks.p, chunks.data(), chunks.size() * sizeof(int2), hipMemcpyHostToDevice));
     for (auto& d : Q.hsys) d.eps = __builtin_nan("");  // force the descriptor upload below
     P = Q;
@@ -602,11 +652,11 @@ static int validate(bia_ctx* ctx, const StepArgs& a) {
 
 // the device plan a launch reads: the context's buffers, or the no-allocation scratch array
 struct PlanView {
-  const int4* tiles;
+  const int4* items;
   const SysDesc* sys;
   const int2* chunks;
   double* part;
-  int ntiles, nchunks;
+  int nitems, nchunks;
   bool context_plan;  // the context's own plan: statistics, failure records and autotuned grids apply
 };
 
@@ -615,17 +665,17 @@ struct PlanView {
 static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int occ_override = 0,
                        const PlanView* view = nullptr) {
   const PlanView V = view ? *view
-                          : PlanView{ctx->plan_tiles.as<int4>(), ctx->plan_sys.as<SysDesc>(), ctx->plan_chunks.as<int2>(),
-                                     ctx->part.as<double>(), ctx->plan.ntiles, ctx->plan.nchunks, true};
+                          : PlanView{ctx->plan_items.as<int4>(), ctx->plan_sys.as<SysDesc>(), ctx->plan_chunks.as<int2>(),
+                                     ctx->part.as<double>(), ctx->plan.nitems, ctx->plan.nchunks, true};
   const auto tuned = V.context_plan ? tuned_for(ctx, a) : std::pair<int, int>(0, 1);
   const int tuned_grid = tuned.first, tuned_occ = tuned.second;
   const int occ = occ_override > 0 ? occ_override : tuned_occ;
   const LearnTable T = mode_table(ctx, a.mode);
-  int grid = grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : V.ntiles);
+  int grid = grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : V.nitems);
   // a dispatch holds fewer than 2^32 work-items; the pair kernel is grid-stride, so a grid capped
-  // there still covers every tile (one system of more than about 741k particles)
-  grid = std::min(grid, std::min(V.ntiles, kMaxGrid64));
-  const int4* tiles = V.tiles;
+  // there still covers every item
+  grid = std::min(grid, std::min(V.nitems, kMaxGrid));
+  const int4* items = V.items;
   const SysDesc* sys = V.sys;
   double* part = V.part;
   // a failure log needs the statistics variant of the pair kernel, which writes the records; its
@@ -642,16 +692,15 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   const bool staged = V.context_plan && Pl.nstages > 1;
   const int nstages = staged ? Pl.nstages : 1;
   for (int g = 0; g < nstages; g++) {
-  Stage stg{0, V.ntiles, 0, nullptr};
+  Stage stg{0, V.nitems, 0, 0};
   if (staged) {
-    stg = Stage{Pl.stage_t0[g], Pl.stage_t0[g + 1], g * Pl.stage_rows,
-                part + (size_t)6 * Pl.stage_rows * Pl.hsys[0].nt * kWave};
+    stg = Stage{Pl.stage_t0[g], Pl.stage_t0[g + 1], g * Pl.stage_blocks, 1};
     grid = std::min(grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : stg.t1 - stg.t0),
-                    std::min(stg.t1 - stg.t0, kMaxGrid64));
+                    std::min(stg.t1 - stg.t0, kMaxGrid));
   }
   BIA_TRY(ctx, prof_begin(ctx));
 #define L_OCC(M, MO, CP, S, O)                                                                                     \
-  hipLaunchKernelGGL((k_tiles<M, MO, CP, S, O>), dim3(grid), dim3(kWave), 0, ctx->stream, stg, a.ld, tiles, sys, \
+  hipLaunchKernelGGL((k_tiles<M, MO, CP, S, O>), dim3(grid), dim3(kGroup), 0, ctx->stream, stg, a.ld, items, sys, \
                      a.d_state, a.d_mass, a.d_charge, a.coup, a.dt, a.itr, part, slots, T, tol, lg)
 #define L_(M, MO, CP, S)                                    \
   do {                                                      \
@@ -688,11 +737,11 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   BIA_TRY(ctx, prof_end(ctx));
   if (staged) {  // this stage's partials into the running sums (the final deltas of its own rows)
     const SysDesc& S0 = Pl.hsys[0];
-    const int gs = std::min(Pl.stage_rows, S0.nt - stg.g0);
-    const int nrem = S0.n - stg.g0 * kWave;
+    const int nb = n_blocks(S0.nt);
+    const int gs = std::min(Pl.stage_blocks, nb - stg.g0);
     BIA_TRY(ctx, prof_begin(ctx, 1));
-    hipLaunchKernelGGL(k_reduce_stage, dim3((nrem + 255) / 256, 6), dim3(256), 0, ctx->stream, S0.n, S0.nt, stg.g0,
-                       gs, g == 0 ? 1 : 0, a.ld, a.ld, a.dt, a.d_state, part, stg.rb, a.d_delta);
+    hipLaunchKernelGGL(k_reduce_stage, dim3(nb - stg.g0, 6), dim3(kGroup), 0, ctx->stream, S0.n, S0.nt, stg.g0, gs,
+                       g == 0 ? 1 : 0, a.ld, a.ld, a.dt, a.d_state, part, a.d_delta);
     BIA_HIP_TRY(ctx, hipGetLastError());
     BIA_TRY(ctx, prof_end(ctx, 1));
   }
@@ -703,7 +752,7 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   }
   if (!staged) {
     BIA_TRY(ctx, prof_begin(ctx, 1));
-    hipLaunchKernelGGL(k_reduce, dim3(V.nchunks, 6), dim3(256), 0, ctx->stream, V.chunks, sys, a.ld, a.ld, a.dt,
+    hipLaunchKernelGGL(k_reduce, dim3(V.nchunks, 6), dim3(kGroup), 0, ctx->stream, V.chunks, sys, a.ld, a.ld, a.dt,
                        a.d_state, part, a.d_delta);
     BIA_HIP_TRY(ctx, hipGetLastError());
     BIA_TRY(ctx, prof_end(ctx, 1));
@@ -724,10 +773,10 @@ static int autotune_locked(bia_ctx* ctx, const StepArgs& a, float* best_ms, int*
   BIA_TRY(ctx, ensure_plan(ctx, a));
   int ncu = 0;
   BIA_HIP_TRY(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  const int ntiles = ctx->plan.ntiles;
-  std::vector<int> cands = {ntiles};
-  for (int m : {4, 8, 16, 32, 64})
-    if (m * ncu < ntiles) cands.push_back(m * ncu);
+  const int nitems = ctx->plan.nitems;
+  std::vector<int> cands = {nitems};
+  for (int m : {2, 4, 8, 16})
+    if (m * ncu < nitems) cands.push_back(m * ncu);
   hipEvent_t e0, e1;
   BIA_HIP_TRY(ctx, hipEventCreate(&e0));
   BIA_HIP_TRY(ctx, hipEventCreate(&e1));
@@ -736,7 +785,7 @@ static int autotune_locked(bia_ctx* ctx, const StepArgs& a, float* best_ms, int*
   const int saved_prof = ctx->profiling;
   ctx->profiling = 0;
   float best = 1e30f;
-  int bestg = ntiles, besto = 1;
+  int bestg = nitems, besto = 1;
   int status = BIA_OK;
   std::vector<int> occs = {1, 4};
   if (a.method == BIA_METHOD_ALG && (a.mode == BIA_MODE_LEARN || a.mode == BIA_MODE_GUESS)) occs.push_back(5);
@@ -806,7 +855,7 @@ static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, 
                                 (long long)(size_t)mode_table(ctx, a.mode).t, mode_table(ctx, a.mode).n,
                                 (long long)(size_t)a.d_stats,
                                 (long long)(size_t)ctx->stat_slots.p, ctx->plan_gen,
-                                (long long)(size_t)ctx->plan_tiles.p, (long long)(size_t)ctx->plan_sys.p,
+                                (long long)(size_t)ctx->plan_items.p, (long long)(size_t)ctx->plan_sys.p,
                                 (long long)(size_t)ctx->plan_chunks.p, (long long)(size_t)ctx->log_rec,
                                 (long long)(size_t)ctx->log_count, ctx->log_cap};
   for (const double v : {a.dt, a.coup, ctx->rkf_rel, ctx->rkf_abs}) {
@@ -844,20 +893,22 @@ static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, 
 }
 
 // ------------------------------------------------------------------------------------------------
-// no-allocation step (bia_step_noalloc_dev).  The plan and the tile partials of one system of at
-// most kNoAllocMaxN particles live in the file-scope device array `scratch` (48.2 MiB, held by
-// every GPU the library runs on until the process ends), so the call allocates nothing.  In host
-// code `scratch` names a host-side stand-in, not device memory: its device address on each GPU
-// comes from hipGetSymbolAddress, once per device at bia_create, and only that address is used.
+// no-allocation step (bia_step_noalloc_dev).  The plan and the partials of one system of at most
+// kNoAllocMaxN particles live in the file-scope device array `scratch` (13 MB, held by every GPU
+// the library runs on until the process ends), so the call allocates nothing.  In host code
+// `scratch` names a host-side stand-in, not device memory: its device address on each GPU comes
+// from hipGetSymbolAddress, once per device at bia_create, and only that address is used.
 // ------------------------------------------------------------------------------------------------
 constexpr int kNoAllocMaxN = 8192;
 constexpr int kNoAllocMaxNt = kNoAllocMaxN / kWave;
+constexpr int kNoAllocMaxNb = (kNoAllocMaxNt + kBlk - 1) / kBlk;
 constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
-constexpr size_t kOffTiles = 0;
-constexpr size_t kOffChunks = align256(kOffTiles + (size_t)kNoAllocMaxNt * (kNoAllocMaxNt + 1) / 2 * sizeof(int4));
-constexpr size_t kOffSys = align256(kOffChunks + (size_t)(kNoAllocMaxN + 255) / 256 * sizeof(int2));
+constexpr size_t kOffItems = 0;
+constexpr size_t kOffChunks = align256(kOffItems + (size_t)kNoAllocMaxNb * (kNoAllocMaxNb + 1) / 2 * sizeof(int4));
+constexpr size_t kOffSys = align256(kOffChunks + (size_t)(kNoAllocMaxN + kGroup - 1) / kGroup * sizeof(int2));
 constexpr size_t kOffPart = align256(kOffSys + sizeof(SysDesc));
-constexpr size_t kScratchBytes = kOffPart + (size_t)6 * kNoAllocMaxNt * kNoAllocMaxNt * kWave * sizeof(double);
+constexpr size_t kScratchBytes =
+    kOffPart + (size_t)blk_off(kNoAllocMaxNt * kWave, kNoAllocMaxNb, kNoAllocMaxNb) * sizeof(double);
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -886,22 +937,23 @@ static void* scratch_on(int device) {
   return g_scratch[device];
 }
 
-// the plan of one n-particle system, as ensure_plan lists it: tiles (I, J), I <= J, row by row;
-// reduction chunks of 256 particles; the system descriptor.  Rewritten on every call (a fill
-// kernel: a copy or memset node would not be safe to capture in a graph).
-__global__ __launch_bounds__(256) void k_noalloc_plan(int n, int nt, double eps, int4* __restrict__ tiles,
+// the plan of one n-particle system, as ensure_plan lists it: items (R, k), k < nb - R, block by
+// block; reduction chunks of one block (256 particles); the system descriptor.  Rewritten on every
+// call (a fill kernel: a copy or memset node would not be safe to capture in a graph).
+__global__ __launch_bounds__(256) void k_noalloc_plan(int n, int nt, double eps, int4* __restrict__ items,
                                                       int2* __restrict__ chunks, SysDesc* __restrict__ sys) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ntiles = nt * (nt + 1) / 2;
-  if (t < ntiles) {
-    int I = 0, first = 0;  // row I holds tiles [first, first + nt - I)
-    while (first + (nt - I) <= t) {
-      first += nt - I;
-      ++I;
+  const int nb = n_blocks(nt);
+  const int nitems = nb * (nb + 1) / 2;
+  if (t < nitems) {
+    int R = 0, first = 0;  // block R holds items [first, first + nb - R)
+    while (first + (nb - R) <= t) {
+      first += nb - R;
+      ++R;
     }
-    tiles[t] = make_int4(I, I + (t - first), 0, 0);
+    items[t] = make_int4(R, t - first, 0, 0);
   }
-  if (t < (n + 255) / 256) chunks[t] = make_int2(0, 256 * t);
+  if (t < (n + kGroup - 1) / kGroup) chunks[t] = make_int2(0, kGroup * t);
   if (t == 0) *sys = SysDesc{0, n, nt, 0, 0LL, eps};
 }
 
@@ -969,16 +1021,16 @@ int bia_step_noalloc_dev(bia_ctx* ctx, int method, int mode, int n, double eps, 
     snprintf(ctx->last_error, sizeof(ctx->last_error), "no scratch address for device %d", ctx->device);
     return BIA_ERR_STATE;
   }
-  const int nt = (n + kWave - 1) / kWave;
-  const int ntiles = nt * (nt + 1) / 2, nchunks = (n + 255) / 256;
-  int4* tiles = reinterpret_cast<int4*>(base + kOffTiles);
+  const int nt = (n + kWave - 1) / kWave, nb = n_blocks(nt);
+  const int nitems = nb * (nb + 1) / 2, nchunks = (n + kGroup - 1) / kGroup;
+  int4* items = reinterpret_cast<int4*>(base + kOffItems);
   int2* chunks = reinterpret_cast<int2*>(base + kOffChunks);
   SysDesc* sys = reinterpret_cast<SysDesc*>(base + kOffSys);
   double* part = reinterpret_cast<double*>(base + kOffPart);
-  hipLaunchKernelGGL(k_noalloc_plan, dim3((ntiles + 255) / 256), dim3(256), 0, ctx->stream, n, nt, eps, tiles, chunks,
+  hipLaunchKernelGGL(k_noalloc_plan, dim3((nitems + 255) / 256), dim3(256), 0, ctx->stream, n, nt, eps, items, chunks,
                      sys);
   BIA_HIP_TRY(ctx, hipGetLastError());
-  const PlanView V{tiles, sys, chunks, part, ntiles, nchunks, false};
+  const PlanView V{items, sys, chunks, part, nitems, nchunks, false};
   return launch_step(ctx, a, 0, 0, &V);
 }
 

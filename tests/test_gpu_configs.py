@@ -477,9 +477,10 @@ def test_mass_spectrum_ensemble_against_dim(ctx):
 
 def test_system_262144_with_64_bit_partial_offsets():
     """The largest single system tested: Plummer N = 262144 (4096 tiles a side), eps = -2/N,
-    dt = 1e-3, one Kepler-drift step (3.4e10 pairs), unstaged (workspace limit off: the tile
-    partials hold 6.4e9 doubles, 51.5 GB, so their offsets pass 2^32) and under the default 1 GiB
-    limit (98 stages of 42 row tiles, a 1.06 GB stage buffer): the two bitwise equal.  Checked
+    dt = 1e-3, one Kepler-drift step (3.4e10 pairs), unstaged (workspace limit off: the partials
+    hold 1.6e9 doubles, 12.9 GB, so their byte offsets pass 2^32) and under the default 1 GiB
+    limit (24 stages of 43 blocks of four row tiles, a 1.06 GB stage buffer): the two bitwise
+    equal.  Checked
     further by (1) the same system in a shuffled order: every particle's delta agrees with the
     unshuffled one's up to summation order; (2) six particles,
     the first and last ones among them, against the oracle's reference-algorithm row for that
@@ -490,7 +491,7 @@ def test_system_262144_with_64_bit_partial_offsets():
     over a particle's 262143 pairs; (3) the total velocity change is rounding."""
     import torch
     import oracle as O
-    from bia_3d_gpu_amd import MODE_GUESS, Context, ic
+    from bia_3d_gpu_amd import MODE_GUESS, Context, ic, partial_doubles, staging
     n, dt = 262144, 1e-3
     eps = -2.0 / n
     st = ic.plummer_system(n, seed=n)
@@ -509,7 +510,8 @@ def test_system_262144_with_64_bit_partial_offsets():
             del s, d
     d_unstaged, d, dp = out
     print(f"plans (stages, workspace bytes): unstaged {plans[0]}, staged {plans[1]}")
-    assert plans[0] == (1, 6 * n * (n // 64) * 8) and plans[1][0] == 98 and plans[1][1] <= 1 << 30
+    assert plans[0] == (1, 8 * partial_doubles(n)) and plans[1] == staging(n, 1 << 30) and plans[1][0] == 24
+    assert plans[1][1] <= 1 << 30
     assert np.array_equal(d_unstaged, d)
     pair_part = d.copy()
     pair_part[:3] -= st[3:] * dt  # the position deltas without the drift v dt
