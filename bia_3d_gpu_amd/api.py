@@ -177,15 +177,32 @@ def reference_step(state_soa: np.ndarray, eps: float, dt: float, method: int = M
 # ------------------------------------------------------------------------------------------------
 # algorithmic work per step (flop convention: + - x / sqrt = 1, transcendental = 20)
 # ------------------------------------------------------------------------------------------------
+def _items_before(nt: int, seg: int, r: int) -> int:
+    """Work items of the first r blocks of four row tiles (csrc/bia_step.hip, Layout): block R has
+    ceil(nt / seg) - (4 / seg) R segments of seg column tiles."""
+    return r * (-(-nt // seg)) - (4 // seg) * r * (r - 1) // 2
+
+
+def segment_length(n: int) -> int:
+    """Column tiles per work item of an n-particle system (csrc/bia_step.hip, choose_seg): the
+    longest of 4, 2, 1 that still gives the system 8192 items or more, else 1."""
+    nt = (n + 63) // 64
+    nb = (nt + 3) // 4
+    for seg in (4, 2):
+        if _items_before(nt, seg, nb) >= 8192:
+            return seg
+    return 1
+
+
 def partial_doubles(n: int, blocks: int | None = None) -> int:
     """Doubles of the pair kernel's partial sums for one n-particle system (the first ``blocks``
     blocks of four row tiles; all of them by default), as the library lays them out
-    (csrc/bia_step.hip, blk_off): block R holds its column partials for the columns from 256 R to
-    the padded end, 6 (npad - 256 R), then one row partial set per work item, 6 x 256 x (nb - R)."""
+    (csrc/bia_step.hip, Layout): block R holds its column partials for the columns from 256 R to
+    the padded end, 6 (npad - 256 R), then one row partial set of 6 x 256 per work item."""
     nt = (n + 63) // 64
     npad, nb = 64 * nt, (nt + 3) // 4
     r = nb if blocks is None else blocks
-    return 6 * (r * npad - 128 * r * (r - 1)) + 6 * 256 * (r * nb - r * (r - 1) // 2)
+    return 6 * (r * npad - 128 * r * (r - 1)) + 6 * 256 * _items_before(nt, segment_length(n), r)
 
 
 def staging(n: int, limit_bytes: int) -> tuple[int, int]:

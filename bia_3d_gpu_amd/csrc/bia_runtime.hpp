@@ -19,7 +19,7 @@ constexpr int kProfSlots = 256;  // pair-kernel launches whose timing events are
 // tiles, partial sums at part + poff (blocks of four row tiles, each its column partials and then
 // its row partials; bia_step.hip, blk_off)
 struct SysDesc {
-  int off, n, nt, pad;
+  int off, n, nt, seg;  // seg: column tiles per pair-kernel work item (bia_step.hip, choose_seg)
   long long poff;
   double eps;
 };

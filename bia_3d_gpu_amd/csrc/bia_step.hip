@@ -38,26 +38,40 @@ namespace bia {
 
 constexpr int kWave = 64;
 constexpr int kBlk = 4;                 // row tiles of a block = waves of a pair-kernel workgroup
-constexpr int kSeg = 4;                 // column tiles of a work item
 constexpr int kGroup = kWave * kBlk;    // threads of a pair-kernel workgroup = particles of a block
-static_assert(kSeg == kBlk, "a block's items are nb - R: one per kSeg column tiles from its diagonal");
 constexpr int kStatSlots = 4096;  // statistics slots of the pair kernel (a power of two)
 constexpr int kMaxGrid = (int)(0xffffffffu / kGroup);  // largest grid of pair-kernel workgroups
 
-// Partial buffer of one system of nt tiles (npad = 64 nt, nb = ceil(nt / 4) blocks): block R
-// holds its column partials colpart[6][W_R] for the columns j in [256 R, npad), W_R = npad - 256 R,
-// then its row partials rowpart[nb - R][6][256] (one per item of the block).  blk_off(R) is where
-// block R starts, blk_off(nb) the system's total (about 12 N nt / 8 doubles: a quarter of one
-// partial per tile and side).
-__host__ __device__ constexpr long long blk_w(int npad, int R) { return (long long)npad - (long long)kGroup * R; }
-__host__ __device__ constexpr long long blk_off(int npad, int nb, int R) {
-  const long long r = R;
-  return 6 * (r * npad - (long long)(kGroup / 2) * r * (r - 1)) + 6LL * kGroup * (r * nb - r * (r - 1) / 2);
-}
-__host__ __device__ constexpr int n_blocks(int nt) { return (nt + kBlk - 1) / kBlk; }
-// items of the blocks before block R (a block has nb - R items)
-__host__ __device__ constexpr long long items_before(int nb, int R) {
-  return (long long)R * nb - (long long)R * (R - 1) / 2;
+// Work items and partial buffer of one system of nt tiles (npad = 64 nt, nb = ceil(nt / 4) blocks),
+// with segments of seg column tiles (seg divides kBlk): block R has nseg(R) = ceil(nt / seg) -
+// (4 / seg) R items, the segments of its column tiles J in [4R, nt).  Block R holds its column
+// partials colpart[6][W_R] for the columns j in [256 R, npad), W_R = npad - 256 R, then its row
+// partials rowpart[nseg(R)][6][256] (one per item).  off(R) is where block R starts, off(nb) the
+// system's total (with seg = 4 about 12 N nt / 8 doubles: a quarter of one partial per tile and
+// side).
+struct Layout {
+  int nt, npad, nb, seg, ns0, step;  // ns0 = ceil(nt / seg) items in block 0, step = 4 / seg fewer per block
+  __host__ __device__ constexpr Layout(int nt_, int seg_)
+      : nt(nt_), npad(nt_ * kWave), nb((nt_ + kBlk - 1) / kBlk), seg(seg_), ns0((nt_ + seg_ - 1) / seg_),
+        step(kBlk / seg_) {}
+  __host__ __device__ constexpr long long w(int R) const { return (long long)npad - (long long)kGroup * R; }
+  __host__ __device__ constexpr int nseg(int R) const { return ns0 - step * R; }
+  __host__ __device__ constexpr long long items_before(int R) const {
+    return (long long)R * ns0 - (long long)step * R * (R - 1) / 2;
+  }
+  __host__ __device__ constexpr long long off(int R) const {
+    const long long r = R;
+    return 6 * (r * npad - (long long)(kGroup / 2) * r * (r - 1)) + 6LL * kGroup * items_before(R);
+  }
+};
+
+// segment length of a system: the longest (fewest partials) that still gives it 8192 items or more
+// (about eight four-wave workgroups per CU), else single column tiles (small systems: a segment
+// of four tiles would leave most CUs idle and serialise four tiles per wave)
+constexpr int kMinItems = 8192;
+__host__ __device__ constexpr int choose_seg(int nt) {
+  return Layout(nt, 4).items_before(Layout(nt, 4).nb) >= kMinItems ? 4
+         : Layout(nt, 2).items_before(Layout(nt, 2).nb) >= kMinItems ? 2 : 1;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -222,10 +236,10 @@ __global__ __launch_bounds__(kGroup, OCC) void k_tiles(Stage stg, int lds, const
         rmi = 1 / mi;
       }
     }
-    const int npad = S.nt * kWave, nb = n_blocks(S.nt);
-    double* B = part + (stg.staged ? blk_off(npad, nb, R) - blk_off(npad, nb, stg.g0) : S.poff + blk_off(npad, nb, R));
-    const long long WR = blk_w(npad, R);
-    const int Ja = kBlk * R + kSeg * seg, Jb = min(Ja + kSeg, S.nt);
+    const Layout L(S.nt, S.seg);
+    double* B = part + (stg.staged ? L.off(R) - L.off(stg.g0) : S.poff + L.off(R));
+    const long long WR = L.w(R);
+    const int Ja = kBlk * R + L.seg * seg, Jb = min(Ja + L.seg, S.nt);
     double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll 1
     for (int J = Ja; J < Jb; ++J) {
@@ -280,11 +294,15 @@ __global__ __launch_bounds__(kGroup, OCC) void k_tiles(Stage stg, int lds, const
         __syncthreads();
       } else {
         __syncthreads();  // the previous column tile's readers are done with cst, cmq and wrow
-        for (int q = threadIdx.x; q < 6 * kWave; q += kGroup) {
+        // the thread index from the wave index and the lane, through an empty asm statement: the
+        // compiler then forms it here instead of keeping it in a register across the pair loop
+        int t0 = w * kWave + lane;
+        asm volatile("" : "+v"(t0));
+        for (int q = t0; q < 6 * kWave; q += kGroup) {
           const int c = q / kWave, l = q % kWave;
           cst[c][l] = l < jn ? st[(size_t)c * lds + S.off + j0 + l] : 0.0;
         }
-        if (COUP != kScalar && threadIdx.x < kWave) {
+        if (COUP != kScalar && w == 0) {
           const bool ok = lane < jn;
           const double mj = (ok && mass) ? mass[S.off + j0 + lane] : 1.0;
           cmq[0][lane] = mj;
@@ -346,7 +364,7 @@ __global__ __launch_bounds__(kGroup, OCC) void k_tiles(Stage stg, int lds, const
       // The thread index goes through an empty asm statement so that the compiler computes these
       // addresses here, not once before the loops (where they would hold registers across the
       // pair loop and cost the Kepler modes a wave per SIMD)
-      int q0 = threadIdx.x;
+      int q0 = w * kWave + lane;
       asm volatile("" : "+v"(q0));
       for (int q = q0; q < 6 * kWave; q += kGroup) {
         const int c = q / kWave, j = q % kWave;
@@ -380,7 +398,7 @@ __global__ __launch_bounds__(256) void k_reduce(const int2* __restrict__ chunks,
   if (il >= S.n) return;
   const int c = blockIdx.y;
   const int R = ch.y / kGroup;
-  const int npad = S.nt * kWave, nb = n_blocks(S.nt);
+  const Layout L(S.nt, S.seg);
   const double* P = part + S.poff;
   double s = 0;
   // 16 loads in flight per thread, then the adds in order (the sum is the sequential one)
@@ -389,13 +407,13 @@ __global__ __launch_bounds__(256) void k_reduce(const int2* __restrict__ chunks,
     double v[16];
 #pragma unroll
     for (int u = 0; u < 16; u++)
-      v[u] = __builtin_nontemporal_load(P + blk_off(npad, nb, r + u) + c * blk_w(npad, r + u) + (il - kGroup * (r + u)));
+      v[u] = __builtin_nontemporal_load(P + L.off(r + u) + c * L.w(r + u) + (il - kGroup * (r + u)));
 #pragma unroll
     for (int u = 0; u < 16; u++) s += v[u];
   }
-  for (; r <= R; ++r) s += __builtin_nontemporal_load(P + blk_off(npad, nb, r) + c * blk_w(npad, r) + (il - kGroup * r));
-  const double* q = P + blk_off(npad, nb, R) + 6 * blk_w(npad, R) + (size_t)c * kGroup + threadIdx.x;
-  const int nk = nb - R;
+  for (; r <= R; ++r) s += __builtin_nontemporal_load(P + L.off(r) + c * L.w(r) + (il - kGroup * r));
+  const double* q = P + L.off(R) + 6 * L.w(R) + (size_t)c * kGroup + threadIdx.x;
+  const int nk = L.nseg(R);
   int k = 0;
   for (; k + 16 <= nk; k += 16) {
     double v[16];
@@ -416,15 +434,15 @@ __global__ __launch_bounds__(256) void k_reduce(const int2* __restrict__ chunks,
 // block order; for a block of the stage, then its row partials in item order, after which that
 // particle's sum is complete and gets its drift.  One workgroup per block from g0 on, one grid
 // row per component.
-__global__ __launch_bounds__(256) void k_reduce_stage(int n, int nt, int g0, int gs, int first, int lds, int ldd,
+__global__ __launch_bounds__(256) void k_reduce_stage(int n, int nt, int seg, int g0, int gs, int first, int lds, int ldd,
                                                       double dt, const double* __restrict__ st,
                                                       const double* __restrict__ part, double* __restrict__ delta) {
   const int R = g0 + blockIdx.x;
   const int il = R * kGroup + threadIdx.x;
   if (il >= n) return;
   const int c = blockIdx.y;
-  const int npad = nt * kWave, nb = n_blocks(nt);
-  const double* P = part - blk_off(npad, nb, g0);  // block offsets relative to the stage's first
+  const Layout L(nt, seg);
+  const double* P = part - L.off(g0);  // block offsets relative to the stage's first
   double s = first ? 0.0 : delta[(size_t)c * ldd + il];
   const int rend = min(R, g0 + gs - 1);
   int r = g0;
@@ -432,14 +450,14 @@ __global__ __launch_bounds__(256) void k_reduce_stage(int n, int nt, int g0, int
     double v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++)
-      v[u] = __builtin_nontemporal_load(P + blk_off(npad, nb, r + u) + c * blk_w(npad, r + u) + (il - kGroup * (r + u)));
+      v[u] = __builtin_nontemporal_load(P + L.off(r + u) + c * L.w(r + u) + (il - kGroup * (r + u)));
 #pragma unroll
     for (int u = 0; u < 8; u++) s += v[u];
   }
-  for (; r <= rend; ++r) s += __builtin_nontemporal_load(P + blk_off(npad, nb, r) + c * blk_w(npad, r) + (il - kGroup * r));
+  for (; r <= rend; ++r) s += __builtin_nontemporal_load(P + L.off(r) + c * L.w(r) + (il - kGroup * r));
   if (R < g0 + gs) {
-    const double* q = P + blk_off(npad, nb, R) + 6 * blk_w(npad, R) + (size_t)c * kGroup + threadIdx.x;
-    const int nk = nb - R;
+    const double* q = P + L.off(R) + 6 * L.w(R) + (size_t)c * kGroup + threadIdx.x;
+    const int nk = L.nseg(R);
     int k = 0;
     for (; k + 8 <= nk; k += 8) {
       double v[8];
@@ -544,14 +562,15 @@ static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
     for (int s = 0; s < a.nsys; s++) {
       const int n = sizes[s];
       if (n < 1) return BIA_ERR_ARG;
-      const int nt = (n + kWave - 1) / kWave, nb = n_blocks(nt);
-      SysDesc d{(int)off, n, nt, 0, poff, 0.0};
+      const int nt = (n + kWave - 1) / kWave;
+      const Layout L(nt, choose_seg(nt));
+      SysDesc d{(int)off, n, nt, L.seg, poff, 0.0};
       Q.hsys.push_back(d);
-      for (int R = 0; R < nb; R++)
-        for (int k = 0; k < nb - R; k++) items.push_back(make_int4(R, k, s, 0));
+      for (int R = 0; R < L.nb; R++)
+        for (int k = 0; k < L.nseg(R); k++) items.push_back(make_int4(R, k, s, 0));
       for (int c0 = 0; c0 < n; c0 += kGroup) chunks.push_back(make_int2(s, c0));  // one chunk per block
       off += n;
-      poff += blk_off(nt * kWave, nb, nb);
+      poff += L.off(L.nb);
     }
     if (off > 0x7fffffffLL || items.size() > 0x7fffffffULL) return BIA_ERR_ARG;
     Q.ntotal = (int)off;
@@ -560,17 +579,17 @@ static int ensure_plan(bia_ctx* ctx, const StepArgs& a) {
     Q.part_doubles = (size_t)poff;
     Q.stage_t0 = {0, Q.nitems};
     if (a.nsys == 1 && ctx->workspace_limit > 0 && Q.part_doubles * sizeof(double) > ctx->workspace_limit) {
-      // stages of G blocks: the first stage's blocks hold the most partials (a block's column
-      // partials shrink with R), blk_off(G) doubles, which the limit bounds (at least one block)
-      const int nt = Q.hsys[0].nt, nb = n_blocks(nt), npad = nt * kWave;
+      // stages of G blocks: the first stage's blocks hold the most partials (a block's partials
+      // shrink with R), L.off(G) doubles, which the limit bounds (at least one block)
+      const Layout L(Q.hsys[0].nt, Q.hsys[0].seg);
       int G = 1;
-      while (G < nb && (size_t)blk_off(npad, nb, G + 1) * sizeof(double) <= ctx->workspace_limit) ++G;
-      if (G < nb) {
+      while (G < L.nb && (size_t)L.off(G + 1) * sizeof(double) <= ctx->workspace_limit) ++G;
+      if (G < L.nb) {
         Q.stage_blocks = G;
-        Q.nstages = (nb + G - 1) / G;
+        Q.nstages = (L.nb + G - 1) / G;
         Q.stage_t0.clear();
-        for (int g = 0; g <= Q.nstages; g++) Q.stage_t0.push_back((int)items_before(nb, std::min(nb, g * G)));
-        Q.part_doubles = (size_t)blk_off(npad, nb, G);
+        for (int g = 0; g <= Q.nstages; g++) Q.stage_t0.push_back((int)L.items_before(std::min(L.nb, g * G)));
+        Q.part_doubles = (size_t)L.off(G);
       }
     }
     P = Plan();  // invalid until every upload below has succeeded
@@ -737,10 +756,10 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   BIA_TRY(ctx, prof_end(ctx));
   if (staged) {  // this stage's partials into the running sums (the final deltas of its own rows)
     const SysDesc& S0 = Pl.hsys[0];
-    const int nb = n_blocks(S0.nt);
+    const int nb = Layout(S0.nt, S0.seg).nb;
     const int gs = std::min(Pl.stage_blocks, nb - stg.g0);
     BIA_TRY(ctx, prof_begin(ctx, 1));
-    hipLaunchKernelGGL(k_reduce_stage, dim3(nb - stg.g0, 6), dim3(kGroup), 0, ctx->stream, S0.n, S0.nt, stg.g0, gs,
+    hipLaunchKernelGGL(k_reduce_stage, dim3(nb - stg.g0, 6), dim3(kGroup), 0, ctx->stream, S0.n, S0.nt, S0.seg, stg.g0, gs,
                        g == 0 ? 1 : 0, a.ld, a.ld, a.dt, a.d_state, part, a.d_delta);
     BIA_HIP_TRY(ctx, hipGetLastError());
     BIA_TRY(ctx, prof_end(ctx, 1));
@@ -894,21 +913,22 @@ static int run_steps_graph(bia_ctx* ctx, const StepArgs& a, double* d_state_rw, 
 
 // ------------------------------------------------------------------------------------------------
 // no-allocation step (bia_step_noalloc_dev).  The plan and the partials of one system of at most
-// kNoAllocMaxN particles live in the file-scope device array `scratch` (13 MB, held by every GPU
+// kNoAllocMaxN particles live in the file-scope device array `scratch` (32 MB, held by every GPU
 // the library runs on until the process ends), so the call allocates nothing.  In host code
 // `scratch` names a host-side stand-in, not device memory: its device address on each GPU comes
 // from hipGetSymbolAddress, once per device at bia_create, and only that address is used.
 // ------------------------------------------------------------------------------------------------
 constexpr int kNoAllocMaxN = 8192;
 constexpr int kNoAllocMaxNt = kNoAllocMaxN / kWave;
-constexpr int kNoAllocMaxNb = (kNoAllocMaxNt + kBlk - 1) / kBlk;
+// sized for single-tile segments, which hold the most items and partials of any segment length
+constexpr Layout kNoAllocLayout(kNoAllocMaxNt, 1);
 constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 constexpr size_t kOffItems = 0;
-constexpr size_t kOffChunks = align256(kOffItems + (size_t)kNoAllocMaxNb * (kNoAllocMaxNb + 1) / 2 * sizeof(int4));
+constexpr size_t kOffChunks =
+    align256(kOffItems + (size_t)kNoAllocLayout.items_before(kNoAllocLayout.nb) * sizeof(int4));
 constexpr size_t kOffSys = align256(kOffChunks + (size_t)(kNoAllocMaxN + kGroup - 1) / kGroup * sizeof(int2));
 constexpr size_t kOffPart = align256(kOffSys + sizeof(SysDesc));
-constexpr size_t kScratchBytes =
-    kOffPart + (size_t)blk_off(kNoAllocMaxNt * kWave, kNoAllocMaxNb, kNoAllocMaxNb) * sizeof(double);
+constexpr size_t kScratchBytes = kOffPart + (size_t)kNoAllocLayout.off(kNoAllocLayout.nb) * sizeof(double);
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -937,24 +957,23 @@ static void* scratch_on(int device) {
   return g_scratch[device];
 }
 
-// the plan of one n-particle system, as ensure_plan lists it: items (R, k), k < nb - R, block by
+// the plan of one n-particle system, as ensure_plan lists it: items (R, k), k < nseg(R), block by
 // block; reduction chunks of one block (256 particles); the system descriptor.  Rewritten on every
 // call (a fill kernel: a copy or memset node would not be safe to capture in a graph).
-__global__ __launch_bounds__(256) void k_noalloc_plan(int n, int nt, double eps, int4* __restrict__ items,
+__global__ __launch_bounds__(256) void k_noalloc_plan(int n, int nt, int seg, double eps, int4* __restrict__ items,
                                                       int2* __restrict__ chunks, SysDesc* __restrict__ sys) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nb = n_blocks(nt);
-  const int nitems = nb * (nb + 1) / 2;
-  if (t < nitems) {
-    int R = 0, first = 0;  // block R holds items [first, first + nb - R)
-    while (first + (nb - R) <= t) {
-      first += nb - R;
+  const Layout L(nt, seg);
+  if (t < L.items_before(L.nb)) {
+    int R = 0, first = 0;  // block R holds items [first, first + nseg(R))
+    while (first + L.nseg(R) <= t) {
+      first += L.nseg(R);
       ++R;
     }
     items[t] = make_int4(R, t - first, 0, 0);
   }
   if (t < (n + kGroup - 1) / kGroup) chunks[t] = make_int2(0, kGroup * t);
-  if (t == 0) *sys = SysDesc{0, n, nt, 0, 0LL, eps};
+  if (t == 0) *sys = SysDesc{0, n, nt, seg, 0LL, eps};
 }
 
 static_assert(sizeof(PairRecord) == sizeof(bia_pair_record) && offsetof(PairRecord, error) ==
@@ -1021,14 +1040,15 @@ int bia_step_noalloc_dev(bia_ctx* ctx, int method, int mode, int n, double eps, 
     snprintf(ctx->last_error, sizeof(ctx->last_error), "no scratch address for device %d", ctx->device);
     return BIA_ERR_STATE;
   }
-  const int nt = (n + kWave - 1) / kWave, nb = n_blocks(nt);
-  const int nitems = nb * (nb + 1) / 2, nchunks = (n + kGroup - 1) / kGroup;
+  const int nt = (n + kWave - 1) / kWave;
+  const Layout L(nt, choose_seg(nt));
+  const int nitems = (int)L.items_before(L.nb), nchunks = (n + kGroup - 1) / kGroup;
   int4* items = reinterpret_cast<int4*>(base + kOffItems);
   int2* chunks = reinterpret_cast<int2*>(base + kOffChunks);
   SysDesc* sys = reinterpret_cast<SysDesc*>(base + kOffSys);
   double* part = reinterpret_cast<double*>(base + kOffPart);
-  hipLaunchKernelGGL(k_noalloc_plan, dim3((nitems + 255) / 256), dim3(256), 0, ctx->stream, n, nt, eps, items, chunks,
-                     sys);
+  hipLaunchKernelGGL(k_noalloc_plan, dim3((nitems + 255) / 256), dim3(256), 0, ctx->stream, n, nt, L.seg, eps, items,
+                     chunks, sys);
   BIA_HIP_TRY(ctx, hipGetLastError());
   const PlanView V{items, sys, chunks, part, nitems, nchunks, false};
   return launch_step(ctx, a, 0, 0, &V);
