@@ -405,6 +405,9 @@ BIA_HDF double hyp_loop(const HypState& S, double th_end, double err, int itr, H
   double err_stock = err, th_stock = 0;
   bool updated = false, cycled = false;
   double th_p1 = __builtin_nan(""), th_p2 = __builtin_nan("");  // the last two evaluated anomalies
+  // the residual history: its first four entries in registers (NaN until set: equal to nothing),
+  // the rest in the history array (rare: most loops end by their fifth evaluation)
+  double h0 = __builtin_nan(""), h1 = h0, h2 = h0, h3 = h0;
   while (ir < itr) {
     double th_now = th_end;
     ir++;
@@ -415,14 +418,25 @@ BIA_HDF double hyp_loop(const HypState& S, double th_end, double err, int itr, H
     th_p2 = th_p1;
     th_p1 = th_now;
     th_end = hyp_newton(th_now, S.TE, S.P, &err, &d);
-    for (int num = ir - 1; num >= 1; num--) {
-      if (hist.get(num - 1) == err) {
-        cycled = true;
-        break;
-      }
+    // an exact repeat of any earlier residual of the loop (entries 0 .. ir - 2); entries from 4 on
+    // are read four at a time (an index past the last reads the last again)
+    bool rep = (h0 == err) | (h1 == err) | (h2 == err) | (h3 == err);
+    for (int b = 4; b < ir - 1; b += 4) {
+      double h[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) h[u] = hist.get(b + u < ir - 1 ? b + u : ir - 2);
+#pragma unroll
+      for (int u = 0; u < 4; u++) rep |= (h[u] == err);
     }
-    if (cycled) break;
-    hist.set(ir - 1, err);
+    if (rep) {
+      cycled = true;
+      break;
+    }
+    if (ir == 1) h0 = err;
+    else if (ir == 2) h1 = err;
+    else if (ir == 3) h2 = err;
+    else if (ir == 4) h3 = err;
+    else hist.set(ir - 1, err);
     if (fabs(err) < fabs(err_stock)) {
       err_stock = err;
       th_stock = th_end;
