@@ -140,9 +140,9 @@ int bia_step_dev(bia_ctx *ctx, int method, int mode, int n, double eps, double d
 
 /* The step of bia_step_dev (same kernels, bitwise the same deltas) that allocates nothing.  No
  * counterpart in the reference: for callers that cannot allocate device memory (code under HIP
- * graph capture, tight serving loops, harnesses that forbid allocation).  Its plan and tile
- * partials live in a library-owned static device array of 48.1 MiB that each GPU the library
- * runs on holds until the process ends, which limits n to 1 <= n <= 8192 (BIA_ERR_ARG beyond,
+ * graph capture, tight serving loops, harnesses that forbid allocation).  Its plan and partial
+ * sums live in a library-owned static device array of 31.0 MiB that each GPU the library runs
+ * on holds until the process ends, which limits n to 1 <= n <= 8192 (BIA_ERR_ARG beyond,
  * d_delta untouched).  No statistics and no failure records; the context's autotuned grids do
  * not apply.  Every call on one GPU shares that GPU's array (from any context): calls that
  * overlap on one GPU race, so order them on one stream. */
@@ -175,14 +175,16 @@ int bia_run_steps_dev(bia_ctx *ctx, int method, int mode, int coupling, int nsys
                       const double *d_charge, double *d_delta, int nsteps, int use_graph,
                       unsigned long long *d_stats);
 
-/* Workspace of the step.  The pair kernel leaves per-tile partial sums that a second kernel adds
- * in a fixed order (bitwise reproducible results); for one system of n particles they take
- * 6 x 8 x n_pad x n_pad / 64 bytes (n_pad = 64 ceil(n / 64)), 3.2 GB at n = 65536.  When that
- * exceeds the context's limit (default 1 GiB; 0 = no limit), a single-system step runs in stages
- * of G row tiles, each followed by its reduction, with 12 x 8 x G x n_pad bytes of partials (G the
- * largest that fits, at least 1): the same deltas bitwise, in O(n) workspace.  Batched steps of
- * several systems are not staged.  No counterpart in the reference (its per-particle calls keep
- * 96 n bytes on the device, bia/find_dXYUV_kernel.cu:532-533). */
+/* Workspace of the step.  The pair kernel leaves partial sums that a second kernel adds in a
+ * fixed order (bitwise reproducible results): per block of four 64-particle row tiles, one column
+ * partial per later particle and one row partial set per work item of up to four column tiles;
+ * for one system of n particles about 12 x 8 x n x n_pad / 256 bytes with four-tile items (n_pad =
+ * 64 ceil(n / 64); 0.81 GB at n = 65536), up to four times that for the single-tile items of
+ * small systems (1 MB at n = 1332).  When that exceeds the context's limit (default 1 GiB; 0 = no
+ * limit), a single-system step runs in stages of G blocks, each followed by its reduction (G the
+ * largest whose partials fit, at least 1): the same deltas bitwise, in O(n) workspace.  Batched
+ * steps of several systems are not staged.  No counterpart in the reference (its per-particle
+ * calls keep 96 n bytes on the device, bia/find_dXYUV_kernel.cu:532-533). */
 int bia_set_workspace_limit(bia_ctx *ctx, unsigned long long bytes);
 /* The plan of the context's last step: its number of stages and tile-partial workspace in bytes
  * (BIA_ERR_STATE before any step). */
