@@ -10,7 +10,7 @@ driver (:mod:`.driver`) and the learned-initialiser training (:mod:`.learn`).
 from . import io, ic  # noqa: F401
 from .api import (  # noqa: F401
     METHOD_ALG, METHOD_RKF, MODE_GUESS, MODE_LEARN, MODE_PLAIN, MODE_PLAIN_LEARN, PAIR_RECORD_DTYPE, STATS_LEN, Context, FailureLog,
-    find_dxyuv_kernel, flops_per_step, partial_doubles, staging,
+    find_dxyuv_kernel, flops_per_step, partial_doubles, segment_length, staging,
     invariance_gpu, reference_step, rkf_c,
 )
 from ._lib import COUPLING_COULOMB, COUPLING_GRAVITY, COUPLING_SCALAR, LIB_PATH, load  # noqa: F401

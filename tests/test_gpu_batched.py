@@ -31,6 +31,35 @@ def test_batched_equals_single_system_calls(ctx, mode):
         off += n
 
 
+def test_batched_segment_lengths_equal_single_system_calls(ctx):
+    """One list mixing the three segment lengths of the pair kernel's work items (four column tiles
+    for N = 32768, two for N = 24576, one for the small systems): each system's deltas bitwise
+    those of its own single-system step, and the small systems within the condition bound of the
+    oracle's reference-algorithm step."""
+    import torch
+    from bia_3d_gpu_amd import MODE_GUESS, ic, segment_length
+    sizes = [130, 32768, 777, 24576, 64]
+    assert [segment_length(n) for n in sizes] == [1, 4, 1, 2, 1]
+    states = [ic.plummer_system(n, seed=90 + k) for k, n in enumerate(sizes)]
+    eps = [-2.0 / n for n in sizes]
+    s = torch.from_numpy(_cat(states)).cuda()
+    d = torch.empty_like(s)
+    ctx.step_batched_(s, d, sizes, eps=eps, dt=1e-3, mode=MODE_GUESS)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    off = 0
+    for st, n, e in zip(states, sizes, eps):
+        si = torch.from_numpy(st).cuda()
+        di = torch.empty_like(si)
+        ctx.step_(si, di, e, 1e-3, mode=MODE_GUESS)
+        torch.cuda.synchronize()
+        assert np.array_equal(got[:, off:off + n], di.cpu().numpy()), n
+        if n <= 777:  # against the oracle's reference-algorithm step, within its condition bound
+            ok, rp, rv = O.within_condition_bound(st, got[:, off:off + n], O.bia_step(st, e, 1e-3), e, dt=1e-3)
+            assert ok, (n, rp, rv)
+        off += n
+
+
 def test_gravity_coupling_equal_masses_is_the_reference_step(ctx):
     import torch
     from bia_3d_gpu_amd import ic, COUPLING_GRAVITY

@@ -73,6 +73,26 @@ def test_staged_graph_replay_and_autotuner():
         assert np.array_equal(d.cpu().numpy(), one)
 
 
+@pytest.mark.parametrize("n", [24576, 32768])
+def test_staged_segments_of_two_and_four_tiles(n):
+    """Staging with the longer work-item segments (two column tiles at N = 24576, four at
+    N = 32768): three blocks per stage, the same deltas bitwise as the unstaged step."""
+    from bia_3d_gpu_amd import Context, MODE_GUESS, ic, partial_doubles, segment_length, staging
+    st = ic.plummer_with_binaries(n, 79)
+    eps, dt = -2.0 / n, 1e-3
+    nb = ((n + 63) // 64 + 3) // 4
+    with Context(0) as c:
+        c.set_workspace_limit(0)
+        want, _, _ = _steps(c, st, eps, dt, 1, MODE_GUESS)
+        limit = 8 * partial_doubles(n, 3)
+        c.set_workspace_limit(limit)
+        got, _, _ = _steps(c, st, eps, dt, 1, MODE_GUESS)
+        ns, wb = c.plan_info()
+    print(f"N = {n} (segments of {segment_length(n)}): {ns} stages of 3 blocks, {wb / 1e6:.1f} MB")
+    assert (ns, wb) == staging(n, limit) == (-(-nb // 3), limit)
+    assert np.array_equal(got, want)
+
+
 def test_benchmark_system_staged(ctx):
     """N = 65536 (0.81 GB of partials unstaged, within the default 1 GiB limit) under a 256 MB
     limit: staged, with the unstaged step's deltas bitwise; the staged workspace is reported."""
