@@ -120,20 +120,26 @@ struct PairStats {
     }
     tc = ti = 0;
   }
-  // each wave's totals into its workgroup's slot: workgroup b uses slot b mod kStatSlots, which
-  // spreads a launch's atomics over kStatSlots x 16 addresses instead of 16; k_stats_fold adds the
-  // slots up
-  __device__ void flush(unsigned long long* slots) {
+  // the workgroup's totals into its slot: workgroup b uses slot b mod kStatSlots, which spreads a
+  // launch's atomics over kStatSlots x 16 addresses instead of 16, and the four waves' totals are
+  // added in LDS first (red: kBlk x 16), so a workgroup makes 16 atomics, not 64, all on one
+  // cache line; k_stats_fold adds the slots up.  Every thread of the workgroup calls this.
+  __device__ void flush(unsigned long long* slots, unsigned (*red)[16], int w) {
     unsigned v[16];
     for (int k = 0; k < 4; k++) {
       v[k] = wave_sum_u(cnt[k]);
       v[4 + k] = wave_sum_u(its[k]);
     }
     for (int k = 0; k < 8; k++) v[8 + k] = wave_sum_u(fl[k]);
-    unsigned long long* out = slots + (size_t)(blockIdx.x & (kStatSlots - 1)) * BIA_STATS_LEN;
     if (threadIdx.x % kWave == 0)
-      for (int k = 0; k < 16; k++)
-        if (v[k]) atomicAdd(out + k, (unsigned long long)v[k]);
+      for (int k = 0; k < 16; k++) red[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      unsigned long long t = 0;
+      for (int q = 0; q < kBlk; q++) t += red[q][threadIdx.x];
+      unsigned long long* out = slots + (size_t)(blockIdx.x & (kStatSlots - 1)) * BIA_STATS_LEN;
+      if (t) atomicAdd(out + threadIdx.x, t);
+    }
   }
 };
 
@@ -382,7 +388,10 @@ __global__ __launch_bounds__(kGroup, OCC) void k_tiles(Stage stg, int lds, const
       for (int c = 0; c < 6; c++) put_partial(rp + c * kGroup, acc[c]);
     }
   }
-  if (STATS) ps.flush(stats);
+  if (STATS) {
+    __shared__ unsigned red[kBlk][16];
+    ps.flush(stats, red, w);
+  }
 }
 
 // Particle i (local index il of block R = il / 256) of a system: the column partials of blocks
@@ -688,7 +697,7 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
                                      ctx->part.as<double>(), ctx->plan.nitems, ctx->plan.nchunks, true};
   const auto tuned = V.context_plan ? tuned_for(ctx, a) : std::pair<int, int>(0, 1);
   const int tuned_grid = tuned.first, tuned_occ = tuned.second;
-  const int occ = occ_override > 0 ? occ_override : tuned_occ;
+  int occ = occ_override > 0 ? occ_override : tuned_occ;
   const LearnTable T = mode_table(ctx, a.mode);
   int grid = grid_override > 0 ? grid_override : (tuned_grid > 0 ? tuned_grid : V.nitems);
   // a dispatch holds fewer than 2^32 work-items; the pair kernel is grid-stride, so a grid capped
@@ -704,6 +713,9 @@ static int launch_step(bia_ctx* ctx, const StepArgs& a, int grid_override, int o
   if (!V.context_plan && a.d_stats) return BIA_ERR_ARG;
   unsigned long long* stats_out = a.d_stats ? a.d_stats : lg.count ? ctx->log_stats.as<unsigned long long>() : nullptr;
   const bool st = stats_out != nullptr;
+  // the statistics variant holds 20 more registers per lane: at the autotuner's 5 waves per SIMD
+  // it spills (52-108 bytes per lane) and runs a third slower, at 4 it does not
+  if (st && occ == 5 && occ_override <= 0) occ = 4;
   unsigned long long* slots = ctx->stat_slots.as<unsigned long long>();
   if (st && !slots) return BIA_ERR_STATE;  // ensure_plan allocates the slots for a step with statistics
   const RkfTol tol{ctx->rkf_rel, ctx->rkf_abs};
