@@ -137,38 +137,46 @@ BIA_HDF double fast_rcp(double x) {
 #endif
 }
 
-// sin and cos of |x| <= pi in FP64 (J.-M. Muller, "Elementary Functions: Algorithms and
-// Implementation", 3rd ed., 2016: Cody-Waite reduction with a two-part constant, then the Taylor
-// series in Horner form): |x| - pi/2 and pi - |x| are exact where they are taken (Sterbenz) and
-// the constants' low parts restore pi/2 and pi to 2^-106; the reduced argument b lies in
-// [-pi/4, pi/4], where the series through b^19 / 19! and b^20 / 20! leave less than 1e-22.  Within
-// about an ulp of the correctly rounded values (tests/test_hostsim.py).  Other arguments go to
-// the library sincos.
+// sin and cos of |x| <= pi in FP64: a Cody-Waite reduction with a two-part constant (J.-M. Muller,
+// "Elementary Functions: Algorithms and Implementation", 3rd ed., 2016) to b in [-pi/4, pi/4],
+// then minimax polynomials: |x| - pi/2 and pi - |x| are exact where they are taken (Sterbenz) and
+// the constants' low parts restore pi/2 and pi to 2^-106.  The polynomials (degree 5 in z = b^2
+// for sin b = b + b^3 p_s(z) and for cos b = 1 - z/2 + z^2 p_c(z)) minimise the error relative to
+// the result over [0, (pi/4)^2]; their coefficients were computed for this library by the Remez
+// exchange algorithm in 60-digit arithmetic (csrc/minimax_coefficients.py), and with the
+// coefficients rounded to double they leave below 5.4e-18 relative (sin) and 9.1e-19 (cos).
+// Within about an ulp of the correctly rounded values (tests/test_hostsim.py).  Other arguments
+// go to the library sincos.
+constexpr double kSin0 = -0.16666666666666632;
+constexpr double kSin1 = 0.008333333333322425;
+constexpr double kSin2 = -0.00019841269829816953;
+constexpr double kSin3 = 2.7557313695226773e-06;
+constexpr double kSin4 = -2.5050758653292795e-08;
+constexpr double kSin5 = 1.5896827929447055e-10;
+constexpr double kCos0 = 0.0416666666666666;
+constexpr double kCos1 = -0.0013888888888874138;
+constexpr double kCos2 = 2.4801587289491837e-05;
+constexpr double kCos3 = -2.7557314355244335e-07;
+constexpr double kCos4 = 2.087572368477449e-09;
+constexpr double kCos5 = -1.1359669885573145e-11;
+
 BIA_HDF void sincos_small(double b, double* s, double* c) {
   const double z = b * b;
-  // sin b = b (1 - z/3! + z^2/5! - ... - z^9/19!)
-  double p = -1.0 / 121645100408832000.0;  // -1/19!
-  p = p * z + 1.0 / 355687428096000.0;     // 1/17!
-  p = p * z - 1.0 / 1307674368000.0;       // 1/15!
-  p = p * z + 1.0 / 6227020800.0;          // 1/13!
-  p = p * z - 1.0 / 39916800.0;            // 1/11!
-  p = p * z + 1.0 / 362880.0;              // 1/9!
-  p = p * z - 1.0 / 5040.0;                // 1/7!
-  p = p * z + 1.0 / 120.0;                 // 1/5!
-  p = p * z - 1.0 / 6.0;                   // 1/3!
+  double p = kSin5;
+  p = p * z + kSin4;
+  p = p * z + kSin3;
+  p = p * z + kSin2;
+  p = p * z + kSin1;
+  p = p * z + kSin0;
   *s = b + (b * z) * p;
-  // cos b = 1 - z/2 + z^2 (1/4! - z/6! + ... + z^8/20!)
-  double q = 1.0 / 2432902008176640000.0;  // 1/20!
-  q = q * z - 1.0 / 6402373705728000.0;    // 1/18!
-  q = q * z + 1.0 / 20922789888000.0;      // 1/16!
-  q = q * z - 1.0 / 87178291200.0;         // 1/14!
-  q = q * z + 1.0 / 479001600.0;           // 1/12!
-  q = q * z - 1.0 / 3628800.0;             // 1/10!
-  q = q * z + 1.0 / 40320.0;               // 1/8!
-  q = q * z - 1.0 / 720.0;                 // 1/6!
-  q = q * z + 1.0 / 24.0;                  // 1/4!
+  double q = kCos5;
+  q = q * z + kCos4;
+  q = q * z + kCos3;
+  q = q * z + kCos2;
+  q = q * z + kCos1;
+  q = q * z + kCos0;
   const double hz = 0.5 * z, w = 1.0 - hz;
-  // 1 - z/2 rounded once, then its rounding error and the series tail (Muller, sec. 11.3)
+  // 1 - z/2 rounded once, then its rounding error and the polynomial tail (Muller, sec. 11.3)
   *c = w + (((1.0 - w) - hz) + (z * z) * q);
 }
 
@@ -209,11 +217,20 @@ BIA_HDF void sincos_pi(double x, double* s, double* c) {
 }
 
 // natural logarithm of a positive normal finite x in FP64 (Muller, 2016: x = 2^k m with m in
-// [sqrt(1/2), sqrt(2)), ln x = k ln 2 + 2 atanh(u), u = (m - 1)/(m + 1), |u| <= 0.1716, the
-// atanh series through u^23 / 23, leaving less than 1e-18 relative); m - 1 is exact (Sterbenz),
-// the quotient is corrected once with its exact residual, ln 2 is split in two parts so that
-// k ln2_hi is exact.  Within about an ulp of the correctly rounded value (tests/test_hostsim.py).
-// Other arguments go to the library log.
+// [sqrt(1/2), sqrt(2)), ln x = k ln 2 + 2 atanh(u), u = (m - 1)/(m + 1), |u| <= 0.1716); m - 1 is
+// exact (Sterbenz), the quotient is corrected once with its exact residual, ln 2 is split in two
+// parts so that k ln2_hi is exact.  2 atanh(u) = 2u + 2u v p_l(v), v = u^2, with p_l the degree-6
+// minimax polynomial of (atanh(sqrt v) - sqrt v) / v^(3/2) on [0, 0.1716^2] (Remez, 60 digits,
+// csrc/minimax_coefficients.py; below 1.8e-18 relative with the coefficients rounded to double).
+// Within about an ulp of the correctly rounded value (tests/test_hostsim.py).  Other arguments
+// go to the library log.
+constexpr double kLog0 = 0.3333333333333367;
+constexpr double kLog1 = 0.1999999999970734;
+constexpr double kLog2 = 0.14285714371193753;
+constexpr double kLog3 = 0.11111099286597308;
+constexpr double kLog4 = 0.0909178216283382;
+constexpr double kLog5 = 0.07657025281120682;
+constexpr double kLog6 = 0.07397974805341635;
 BIA_HDF double log_pos(double x) {
   if (!(x >= 2.2250738585072014e-308 && x <= 1.7976931348623157e308)) return log_lib(x);
   int k;
@@ -229,17 +246,13 @@ BIA_HDF double log_pos(double x) {
   const double r = fast_rcp(d), u0 = f * r;
   const double u = u0 + (fma(-u0, d, f) - u0 * dd) * r;
   const double v = u * u;
-  double p = 1.0 / 23;
-  p = p * v + 1.0 / 21;
-  p = p * v + 1.0 / 19;
-  p = p * v + 1.0 / 17;
-  p = p * v + 1.0 / 15;
-  p = p * v + 1.0 / 13;
-  p = p * v + 1.0 / 11;
-  p = p * v + 1.0 / 9;
-  p = p * v + 1.0 / 7;
-  p = p * v + 1.0 / 5;
-  p = p * v + 1.0 / 3;
+  double p = kLog6;
+  p = p * v + kLog5;
+  p = p * v + kLog4;
+  p = p * v + kLog3;
+  p = p * v + kLog2;
+  p = p * v + kLog1;
+  p = p * v + kLog0;
   // ln 2 = kLn2Hi + kLn2Lo to 2e-31; kLn2Hi has 42 significant bits, so k kLn2Hi is exact
   constexpr double kLn2Hi = 0x1.62e42fefa3800p-1, kLn2Lo = 5.497923018708371e-14;
   const double dk = (double)k;
