@@ -137,6 +137,23 @@ BIA_HDF double fast_rcp(double x) {
 #endif
 }
 
+// a / b given r = fast_rcp(b): q0 = a r, then one correction with the exact remainder a - q0 b
+// (P. Markstein, "IA-64 and Elementary Functions", 2000, ch. 8).  The correction leaves an error
+// below 2^-53 ulp before its rounding, so the result is the correctly rounded quotient unless that
+// lies within 2^-53 ulp of a rounding boundary (never met in practice: the exact quotient of two
+// doubles is not a midpoint).  Several quotients with one divisor share r (three multiply-adds each
+// instead of the full division sequence); b = 0 gives NaN (the division gives +-inf for a != 0).
+// The host build divides.
+BIA_HDF double div_r(double a, double b, double r) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double q0 = a * r;
+  return fma(fma(-q0, b, a), r, q0);
+#else
+  (void)r;
+  return a / b;
+#endif
+}
+
 // sin and cos of |x| <= pi in FP64: a Cody-Waite reduction with a two-part constant (J.-M. Muller,
 // "Elementary Functions: Algorithms and Implementation", 3rd ed., 2016) to b in [-pi/4, pi/4],
 // then minimax polynomials: |x| - pi/2 and pi - |x| are exact where they are taken (Sterbenz) and
@@ -216,6 +233,31 @@ BIA_HDF void sincos_pi(double x, double* s, double* c) {
   *c = ca;
 }
 
+// sincos_pi for |x| <= 3 pi / 4, the half anomalies of the hyperbola's residual (|x| < pi / 2
+// inside the branch): one reduction case fewer, the same results there; beyond, the library
+// routine (iterates far outside the branch, whose residual is NaN in any case)
+BIA_HDF void sincos_half(double x, double* s, double* c) {
+  constexpr double kPio2Hi = 1.5707963267948966, kPio2Lo = 6.123233995736766e-17;
+  constexpr double kPio4 = 0.78539816339744830962, k3Pio4 = 2.35619449019234492885;
+  const double a = fabs(x);
+  if (!(a <= k3Pio4)) {
+    const double2_ r = sincos_lib(x);
+    *s = r.x;
+    *c = r.y;
+    return;
+  }
+  const bool mid = a > kPio4;
+  const double bh = mid ? a - kPio2Hi : a;
+  const double lo = mid ? -kPio2Lo : 0.0;
+  const double b = bh + lo, bl = (bh - b) + lo;
+  double sb, cb;
+  sincos_small(b, &sb, &cb);
+  sb = fma(bl, cb, sb);
+  cb = fma(-bl, sb, cb);
+  *s = copysign(mid ? cb : sb, x);
+  *c = mid ? -sb : cb;
+}
+
 // natural logarithm of a positive normal finite x in FP64 (Muller, 2016: x = 2^k m with m in
 // [sqrt(1/2), sqrt(2)), ln x = k ln 2 + 2 atanh(u), u = (m - 1)/(m + 1), |u| <= 0.1716); m - 1 is
 // exact (Sterbenz), the quotient is corrected once with its exact residual, ln 2 is split in two
@@ -278,16 +320,17 @@ BIA_HDF double fast_rsqrt(double x) {
 //   e_eta = g/|g|, e_zeta = (r x g)/|r x g|, e_xi = e_eta x e_zeta,
 //   yz = (r.e_xi, r.e_eta, 0, |g|)
 // ------------------------------------------------------------------------------------
+// The quotients as the reference's divisions (div_r: correctly rounded, one reciprocal per divisor).
 BIA_HDF void frame(const double R[3], const double G[3], double exi[3], double eeta[3], double yz[4]) {
-  double gm = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
-  eeta[0] = G[0] / gm;
-  eeta[1] = G[1] / gm;
-  eeta[2] = G[2] / gm;
+  const double gm = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]), rgm = fast_rcp(gm);
+  eeta[0] = div_r(G[0], gm, rgm);
+  eeta[1] = div_r(G[1], gm, rgm);
+  eeta[2] = div_r(G[2], gm, rgm);
   double L0 = R[1] * G[2] - R[2] * G[1];
   double L1 = R[2] * G[0] - R[0] * G[2];
   double L2 = R[0] * G[1] - R[1] * G[0];
-  double Lm = sqrt(L0 * L0 + L1 * L1 + L2 * L2);
-  double z0 = L0 / Lm, z1 = L1 / Lm, z2 = L2 / Lm;
+  const double Lm = sqrt(L0 * L0 + L1 * L1 + L2 * L2), rLm = fast_rcp(Lm);
+  double z0 = div_r(L0, Lm, rLm), z1 = div_r(L1, Lm, rLm), z2 = div_r(L2, Lm, rLm);
   exi[0] = eeta[1] * z2 - eeta[2] * z1;
   exi[1] = eeta[2] * z0 - eeta[0] * z2;
   exi[2] = eeta[0] * z1 - eeta[1] * z0;
@@ -306,22 +349,26 @@ struct HypP {
 };
 
 // residual of the dimensionless time equation and one Newton update (find_theta_end_cu :57-66)
-//   f(t) = tn0 sin t / (cos t - cs0) + ln( sin((t0+t)/2) / sin((t0-t)/2) ) - TE
+//   f(t) = tn0 sin t / (cos t - cs0) + ln( sin((t0+t)/2) / sin((t0-t)/2) ) - TE,
+//   f'(t) = tn0 (1 - cs0 cos t) / (cos t - cs0)^2 + sn0 / Q,  Q = 2 sin((t0+t)/2) sin((t0-t)/2).
 // sin/cos of the half angle (|t/2| < pi/2 inside the branch) and the logarithm of the positive
-// ratio from the range-specific routines, the quotients from reciprocals (each within an ulp)
+// ratio from the range-specific routines, the quotients from reciprocals (each within an ulp); the
+// step f / f' is taken as f Q / (f' Q), one reciprocal.  *dfd = f' Q, which has the sign of f'
+// inside the branch (Q > 0 there).
 BIA_HDF double hyp_newton(double th, double TE, const HypP& P, double* err, double* dfd) {
   double sh, ch;
-  sincos_pi(0.5 * th, &sh, &ch);
-  double snp = P.s0h * ch + P.c0h * sh;
-  double snm = P.s0h * ch - P.c0h * sh;
-  double cs = (ch - sh) * (ch + sh);
-  double sn = 2 * sh * ch;
-  double Dinv = fast_rcp(cs - P.cs0);
-  double d = P.tn0 * (1 - P.cs0 * cs) * Dinv * Dinv + P.sn0 * fast_rcp(2 * snp * snm);
-  double e = P.tn0 * sn * Dinv - TE + log_pos(snp * fast_rcp(snm));
+  sincos_half(0.5 * th, &sh, &ch);
+  const double snp = P.s0h * ch + P.c0h * sh;
+  const double snm = P.s0h * ch - P.c0h * sh;
+  const double cs = (ch - sh) * (ch + sh);
+  const double sn = 2 * sh * ch;
+  const double Dinv = fast_rcp(cs - P.cs0);
+  const double Q = 2 * snp * snm;
+  const double dQ = (P.tn0 * (1 - P.cs0 * cs) * Dinv * Dinv) * Q + P.sn0;
+  const double e = P.tn0 * sn * Dinv - TE + log_pos(snp * fast_rcp(snm));
   *err = e;
-  *dfd = d;
-  return th - e * fast_rcp(d);
+  *dfd = dQ;
+  return th - (e * Q) * fast_rcp(dQ);
 }
 
 struct HypState {
@@ -335,26 +382,27 @@ struct HypState {
 // set-up shared by both modes: find_dXYUV_kernel.cu:98-172
 BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, unsigned* fl) {
   double X = yz[0], Y = yz[1], U = yz[2], V = yz[3];
-  double r = sqrt(X * X + Y * Y);
-  double W = U * U + V * V + 2 * eps / r;
+  const double r = sqrt(X * X + Y * Y), rr = fast_rcp(r);
+  double W = U * U + V * V + div_r(2 * eps, r, rr);
   double r_g = X * U + Y * V;
   double L = X * V - Y * U;
-  double g0 = sqrt(W);
-  double b0 = eps / W;
+  const double g0 = sqrt(W), rg0 = fast_rcp(g0);
+  double b0 = div_r(eps, W, fast_rcp(W));
   double aL = fabs(L);
-  S.b = aL / g0;
+  S.b = div_r(aL, g0, rg0);
   double bg = b0 * g0;
   S.P.th0 = atan2(aL, bg);
-  double den = sqrt(bg * bg + L * L);
-  S.P.sn0 = aL / den;
-  S.P.cs0 = bg / den;
-  S.P.tn0 = S.P.sn0 / S.P.cs0;
-  sincos_pi(0.5 * S.P.th0, &S.P.s0h, &S.P.c0h);
-  // the argument of acos with correctly rounded operations: acos near 1 amplifies its rounding
-  // (find_dXYUV_kernel.cu:119-126); the rest, the half-angle sines and cosines of theta_0 and
-  // theta_ini included (|theta_0 / 2|, |theta_ini / 2| <= pi / 2), uses reciprocals and the
-  // range-specific routines, whose ulp-level errors are far below the amplified one
-  double q = S.P.cs0 + S.b * S.P.sn0 / r;
+  const double den = sqrt(bg * bg + L * L), rden = fast_rcp(den);
+  S.P.sn0 = div_r(aL, den, rden);
+  S.P.cs0 = div_r(bg, den, rden);
+  S.P.tn0 = div_r(S.P.sn0, S.P.cs0, fast_rcp(S.P.cs0));
+  sincos_half(0.5 * S.P.th0, &S.P.s0h, &S.P.c0h);
+  // the argument of acos with correctly rounded operations (div_r: the reference's divisions),
+  // since acos near 1 amplifies its rounding (find_dXYUV_kernel.cu:119-126); the rest, the
+  // half-angle sines and cosines of theta_0 and theta_ini included (|theta_0 / 2|,
+  // |theta_ini / 2| <= pi / 2), uses reciprocals and the range-specific routines, whose ulp-level
+  // errors are far below the amplified one
+  double q = S.P.cs0 + div_r(S.b * S.P.sn0, r, rr);
   double th_ini, cs;
   if (fabs(q) > 1) {  // :119-126
     th_ini = 1e-30;
@@ -366,7 +414,7 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   }
   if (L * r_g < 0) th_ini = -th_ini;
   double sh, ch;
-  sincos_pi(0.5 * th_ini, &sh, &ch);
+  sincos_half(0.5 * th_ini, &sh, &ch);
   double sn = 2 * sh * ch;
   double x = r * cs, y = r * sn;
   const double gs = g0 * fast_rcp(S.P.sn0);
@@ -378,7 +426,6 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   S.sn_a = (x * Y - y * X) * rn;
   double snp = S.P.s0h * ch + S.P.c0h * sh;
   double snm = S.P.s0h * ch - S.P.c0h * sh;
-  const double rg0 = fast_rcp(g0);
   double time_ini = (S.P.tn0 * sn * fast_rcp(cs - S.P.cs0) + log_pos(snp * fast_rcp(snm))) * b0 * rg0;  // :153-155
   double time_end = (L > 0) ? time_ini + dt : time_ini - dt;
   S.TE = time_end * g0 * fast_rcp(b0);

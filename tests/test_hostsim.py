@@ -397,7 +397,7 @@ def _ulp_error(y, ref):
 
 def test_range_specific_sincos_and_log():
     """The plain mode's sin/cos of |x| <= pi and log of positive normal x (bia_pair.hpp
-    sincos_pi, log_pos; host build of the same source) against 80-bit extended-precision values:
+    sincos_pi, sincos_half, log_pos; host build of the same source) against 80-bit extended-precision values:
     within 1.1 ulp for sin and cos and 1.5 ulp for log, on random arguments and on the points
     where the range reduction switches (pi/4, 3pi/4) or cancels (near pi/2, pi, and log near 1).
     A wrong reduction constant, a dropped series term or an uncorrected quotient would show as
@@ -420,6 +420,19 @@ def test_range_specific_sincos_and_log():
     es, ec = _ulp_error(s, np.sin(xl)), _ulp_error(c, np.cos(xl))
     print(f"sincos_pi on {x.size} arguments: max {es.max():.3f} ulp (sin), {ec.max():.3f} ulp (cos)")
     assert es.max() < 1.1 and ec.max() < 1.1
+    # the half-angle variant: the same results up to 3 pi / 4, the library sincos's beyond (within
+    # an ulp of the library sin and cos)
+    L.hs_sincos_half.argtypes = [C.c_int64, dp, dp, dp]
+    sh, ch = np.zeros_like(x), np.zeros_like(x)
+    L.hs_sincos_half(x.size, x, sh, ch)
+    inner = np.abs(x) <= 0.75 * pi
+    assert inner.sum() > 200000 and (~inner).sum() > 1000
+    assert np.array_equal(sh[inner], s[inner]) and np.array_equal(ch[inner], c[inner])
+    import math
+    xo = x[~inner].tolist()
+    ms, mc = np.array([math.sin(v) for v in xo]), np.array([math.cos(v) for v in xo])
+    assert (np.abs(sh[~inner] - ms) <= np.spacing(np.abs(ms))).all()
+    assert (np.abs(ch[~inner] - mc) <= np.spacing(np.abs(mc))).all()
     y = np.concatenate([10.0 ** rng.uniform(-307, 307, 200000), 1 + rng.uniform(-0.3, 0.3, 200000),
                         1 + 10.0 ** rng.uniform(-15, -1, 20000), 1 - 10.0 ** rng.uniform(-15, -1, 20000),
                         np.array([1.0, 2.0, 0.5, np.sqrt(2.0), np.sqrt(0.5), 1e-300, 1e300])])
@@ -430,6 +443,40 @@ def test_range_specific_sincos_and_log():
     el = _ulp_error(out[nz], ref[nz])
     print(f"log_pos on {y.size} arguments: max {el.max():.3f} ulp; log(1) = {out[~nz]}")
     assert el.max() < 1.5 and (out[~nz] == 0).all()
+
+
+def test_quotient_from_a_shared_reciprocal():
+    """The device's quotient a / b from a reciprocal r of b (bia_pair.hpp div_r: q0 = a r, then
+    q0 + (a - q0 b) r with both steps fused multiply-adds), emulated here in exact rational
+    arithmetic with one rounding per fused operation: it equals the correctly rounded quotient for
+    r the correctly rounded 1/b and for r one ulp off either way (the device's fast_rcp is within an
+    ulp), over operands spread across 120 binades, and on quotients that are exact or lie next to
+    a rounding boundary.  Dropping the correction (q0 alone) fails on about a quarter of them."""
+    import math
+    from fractions import Fraction as F
+
+    def fma(x, y, z):  # one rounding (float() of a Fraction rounds to nearest even)
+        return float(F(x) * F(y) + F(z))
+
+    rng = np.random.default_rng(8)
+    n = 6000
+    a = rng.uniform(-1, 1, n) * 2.0 ** rng.integers(-60, 60, n)
+    b = rng.uniform(0.5, 1, n) * 2.0 ** rng.integers(-60, 60, n) * rng.choice([-1.0, 1.0], n)
+    # exact quotients and quotients within a few ulps of a rounding boundary: a = b q (1 + k 2^-53)
+    q = rng.uniform(1, 2, 400)
+    a2 = np.array([float(F(bb) * F(qq)) for bb, qq in zip(b[:400], q)])
+    a = np.concatenate([a, a2, b[400:600] * 3.0])
+    b = np.concatenate([b, b[:400], b[400:600]])
+    bad = plain = 0
+    for x, y in zip(a.tolist(), b.tolist()):
+        r0 = 1.0 / y
+        q0 = x * r0
+        plain += q0 != x / y
+        for r in (r0, math.nextafter(r0, math.inf), math.nextafter(r0, -math.inf)):
+            q0 = x * r
+            bad += fma(fma(-q0, y, x), r, q0) != x / y
+    print(f"{3 * a.size} quotients: {bad} not correctly rounded; without the correction {plain} of {a.size}")
+    assert bad == 0 and plain > a.size // 10
 
 
 def _binary_pairs(n=65536, seed=65536):
