@@ -355,20 +355,26 @@ struct HypP {
 // ratio from the range-specific routines, the quotients from reciprocals (each within an ulp); the
 // step f / f' is taken as f Q / (f' Q), one reciprocal.  *dfd = f' Q, which has the sign of f'
 // inside the branch (Q > 0 there).
-BIA_HDF double hyp_newton(double th, double TE, const HypP& P, double* err, double* dfd) {
-  double sh, ch;
-  sincos_half(0.5 * th, &sh, &ch);
-  const double snp = P.s0h * ch + P.c0h * sh;
-  const double snm = P.s0h * ch - P.c0h * sh;
+// the same from sin(t/2), cos(t/2), the two sines snp, snm and lg = ln(snp / snm) formed already
+BIA_HDF double hyp_newton_at(double th, double sh, double ch, double snp, double snm, double lg, double TE,
+                             const HypP& P, double* err, double* dfd) {
   const double cs = (ch - sh) * (ch + sh);
   const double sn = 2 * sh * ch;
   const double Dinv = fast_rcp(cs - P.cs0);
   const double Q = 2 * snp * snm;
   const double dQ = (P.tn0 * (1 - P.cs0 * cs) * Dinv * Dinv) * Q + P.sn0;
-  const double e = P.tn0 * sn * Dinv - TE + log_pos(snp * fast_rcp(snm));
+  const double e = P.tn0 * sn * Dinv - TE + lg;
   *err = e;
   *dfd = dQ;
   return th - (e * Q) * fast_rcp(dQ);
+}
+
+BIA_HDF double hyp_newton(double th, double TE, const HypP& P, double* err, double* dfd) {
+  double sh, ch;
+  sincos_half(0.5 * th, &sh, &ch);
+  const double snp = P.s0h * ch + P.c0h * sh;
+  const double snm = P.s0h * ch - P.c0h * sh;
+  return hyp_newton_at(th, sh, ch, snp, snm, log_pos(snp * fast_rcp(snm)), TE, P, err, dfd);
 }
 
 struct HypState {
@@ -377,6 +383,7 @@ struct HypState {
   double th_ini;    // initial true anomaly
   double b, g0_sn0, u, v, cs_a, sn_a;
   double th_gss;    // the reference's closed-form guess (:172)
+  double e_ini, th_ini_end;  // the residual at th_ini and the Newton update from it (:176)
 };
 
 // set-up shared by both modes: find_dXYUV_kernel.cu:98-172
@@ -426,11 +433,16 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   S.sn_a = (x * Y - y * X) * rn;
   double snp = S.P.s0h * ch + S.P.c0h * sh;
   double snm = S.P.s0h * ch - S.P.c0h * sh;
-  double time_ini = (S.P.tn0 * sn * fast_rcp(cs - S.P.cs0) + log_pos(snp * fast_rcp(snm))) * b0 * rg0;  // :153-155
+  const double lg = log_pos(snp * fast_rcp(snm));
+  double time_ini = (S.P.tn0 * sn * fast_rcp(cs - S.P.cs0) + lg) * b0 * rg0;  // :153-155
   double time_end = (L > 0) ? time_ini + dt : time_ini - dt;
   S.TE = time_end * g0 * fast_rcp(b0);
   S.th_ini = th_ini;
   S.th_gss = 2 / kPi * S.P.th0 * atan(S.TE * fast_rcp(S.P.tn0));
+  // the guess selection's evaluation at theta_ini (:176) from the half-angle sine and cosine, the
+  // sines and the logarithm above: the values hyp_newton(th_ini) forms, without forming them twice
+  double d;
+  S.th_ini_end = hyp_newton_at(th_ini, sh, ch, snp, snm, lg, S.TE, S.P, &S.e_ini, &d);
 }
 
 // state after dt, rotated back to the (xi, eta) frame: :220-237
@@ -515,8 +527,8 @@ BIA_HDF double hyp_loop(const HypState& S, double th_end, double err, int itr, H
 // the reference's guess selection (:169-185) and best-iterate loop
 BIA_HDF double hyp_solve_plain(const HypState& S, int itr, Hist hist, int* iters, unsigned* fl,
                                double* resid = nullptr) {
-  double e_ini, e_gss, d;
-  double th_ini_end = hyp_newton(S.th_ini, S.TE, S.P, &e_ini, &d);
+  double e_ini = S.e_ini, e_gss, d;
+  double th_ini_end = S.th_ini_end;
   double th_gss_end = hyp_newton(S.th_gss, S.TE, S.P, &e_gss, &d);
   double th_end, err;
   if (fabs(e_gss) > 10 * fabs(e_ini)) {
