@@ -154,50 +154,92 @@ BIA_HDF double div_r(double a, double b, double r) {
 #endif
 }
 
-// sin and cos of |x| <= pi in FP64: a Cody-Waite reduction with a two-part constant (J.-M. Muller,
-// "Elementary Functions: Algorithms and Implementation", 3rd ed., 2016) to b in [-pi/4, pi/4],
-// then minimax polynomials: |x| - pi/2 and pi - |x| are exact where they are taken (Sterbenz) and
-// the constants' low parts restore pi/2 and pi to 2^-106.  The polynomials (degree 5 in z = b^2
-// for sin b = b + b^3 p_s(z) and for cos b = 1 - z/2 + z^2 p_c(z)) minimise the error relative to
-// the result over [0, (pi/4)^2]; their coefficients were computed for this library by the Remez
-// exchange algorithm in 60-digit arithmetic (csrc/minimax_coefficients.py), and with the
-// coefficients rounded to double they leave below 5.4e-18 relative (sin) and 9.1e-19 (cos).
-// Within about an ulp of the correctly rounded values (tests/test_hostsim.py).  Other arguments
-// go to the library sincos.
-constexpr double kSin0 = -0.16666666666666632;
-constexpr double kSin1 = 0.008333333333322425;
-constexpr double kSin2 = -0.00019841269829816953;
-constexpr double kSin3 = 2.7557313695226773e-06;
-constexpr double kSin4 = -2.5050758653292795e-08;
-constexpr double kSin5 = 1.5896827929447055e-10;
-constexpr double kCos0 = 0.0416666666666666;
-constexpr double kCos1 = -0.0013888888888874138;
-constexpr double kCos2 = 2.4801587289491837e-05;
-constexpr double kCos3 = -2.7557314355244335e-07;
-constexpr double kCos4 = 2.087572368477449e-09;
-constexpr double kCos5 = -1.1359669885573145e-11;
+// sin and cos of |x| <= pi in FP64, table-driven (P. T. P. Tang, "Table-driven implementation of the
+// exponential function", ACM TOMS 15, 1989, for the scheme; J.-M. Muller, "Elementary Functions", 3rd
+// ed., 2016, ch. 11): a = |x| = k pi/64 + r, k = rint(a 64/pi) in 0..64, with r = rh + rl from a
+// three-part pi/64 (a - k hi is exact; rl the rounding of the second part's step and the third
+// part), |r| <= pi/128;
+// then sin a = (S + C rh) + (S (cos r - 1) + C (sin r - rh)) and cos a = (C - S rh) + (C (cos r - 1)
+// - S (sin r - rh)), the parenthesised leading terms each one fused multiply-add with its rounding
+// error recovered, with S, C = sin,
+// cos of k pi/64 from the table as hi + lo pairs, sin r and cos r - 1 from their
+// Taylor polynomials (through r^7 and r^6: the terms left out are below 4e-19 and 4e-18 of the
+// result).  Within about 0.6 ulp (tests/test_hostsim.py); the table
+// and constants come from csrc/minimax_coefficients.py.  Other arguments go to the library sincos.
+struct SinCosEntry {
+  double s_hi, s_lo, c_hi, c_lo;
+};
+constexpr double kPio64Hi = 0.04908738521234035, kPio64Mid = 1.6844696431744122e-16, kPio64Lo = 2.649508643865281e-33,
+                 k64oPi = 20.371832715762604;
+constexpr SinCosEntry kSinCosTab[65] = {
+    {0.0, 0.0, 1.0, 0.0},
+    {0.049067674327418015, -6.79610372051828e-19, 0.9987954562051724, -1.2291693337075465e-17},
+    {0.0980171403295606, -1.634582362244256e-18, 0.9951847266721969, -4.248691367830441e-17},
+    {0.14673047445536175, 3.726947147046568e-18, 0.989176509964781, -4.098730993704711e-17},
+    {0.19509032201612828, -7.991079068461731e-18, 0.9807852804032304, 1.8546939997825006e-17},
+    {0.2429801799032639, -8.751431529719663e-18, 0.970031253194544, 1.8365300348428844e-17},
+    {0.2902846772544624, -1.892797870777425e-17, 0.9569403357322088, 4.05538698618757e-17},
+    {0.33688985339222005, -4.200094003347509e-19, 0.9415440651830208, -2.789637954769834e-17},
+    {0.3826834323650898, -1.0050772696461588e-17, 0.9238795325112867, 1.7645047084336677e-17},
+    {0.4275550934302821, 9.411189816295473e-18, 0.9039892931234433, -6.609754468748431e-18},
+    {0.47139673682599764, 6.516678136069013e-18, 0.881921264348355, -1.9843248405890562e-17},
+    {0.5141027441932218, -4.5712707523615624e-17, 0.8577286100002721, -4.818344793633662e-17},
+    {0.5555702330196022, 4.709410940561677e-17, 0.8314696123025452, 1.4073856984728024e-18},
+    {0.5956993044924334, -1.3438641936579467e-17, 0.8032075314806449, -3.306060980481491e-17},
+    {0.6343932841636455, 1.0420901929280035e-17, 0.773010453362737, -3.256590703364977e-17},
+    {0.6715589548470184, -4.048903774929669e-17, 0.7409511253549591, -1.4708616952297345e-17},
+    {0.7071067811865476, -4.833646656726457e-17, 0.7071067811865476, -4.833646656726457e-17},
+    {0.7409511253549591, -1.4708616952297345e-17, 0.6715589548470184, -4.048903774929669e-17},
+    {0.773010453362737, -3.256590703364977e-17, 0.6343932841636455, 1.0420901929280035e-17},
+    {0.8032075314806449, -3.306060980481491e-17, 0.5956993044924334, -1.3438641936579467e-17},
+    {0.8314696123025452, 1.4073856984728024e-18, 0.5555702330196022, 4.709410940561677e-17},
+    {0.8577286100002721, -4.818344793633662e-17, 0.5141027441932218, -4.5712707523615624e-17},
+    {0.881921264348355, -1.9843248405890562e-17, 0.47139673682599764, 6.516678136069013e-18},
+    {0.9039892931234433, -6.609754468748431e-18, 0.4275550934302821, 9.411189816295473e-18},
+    {0.9238795325112867, 1.7645047084336677e-17, 0.3826834323650898, -1.0050772696461588e-17},
+    {0.9415440651830208, -2.789637954769834e-17, 0.33688985339222005, -4.200094003347509e-19},
+    {0.9569403357322088, 4.05538698618757e-17, 0.2902846772544624, -1.892797870777425e-17},
+    {0.970031253194544, 1.8365300348428844e-17, 0.2429801799032639, -8.751431529719663e-18},
+    {0.9807852804032304, 1.8546939997825006e-17, 0.19509032201612828, -7.991079068461731e-18},
+    {0.989176509964781, -4.098730993704711e-17, 0.14673047445536175, 3.726947147046568e-18},
+    {0.9951847266721969, -4.248691367830441e-17, 0.0980171403295606, -1.634582362244256e-18},
+    {0.9987954562051724, -1.2291693337075465e-17, 0.049067674327418015, -6.79610372051828e-19},
+    {1.0, 0.0, 0.0, 0.0},
+    {0.9987954562051724, -1.2291693337075465e-17, -0.049067674327418015, 6.79610372051828e-19},
+    {0.9951847266721969, -4.248691367830441e-17, -0.0980171403295606, 1.634582362244256e-18},
+    {0.989176509964781, -4.098730993704711e-17, -0.14673047445536175, -3.726947147046568e-18},
+    {0.9807852804032304, 1.8546939997825006e-17, -0.19509032201612828, 7.991079068461731e-18},
+    {0.970031253194544, 1.8365300348428844e-17, -0.2429801799032639, 8.751431529719663e-18},
+    {0.9569403357322088, 4.05538698618757e-17, -0.2902846772544624, 1.892797870777425e-17},
+    {0.9415440651830208, -2.789637954769834e-17, -0.33688985339222005, 4.200094003347509e-19},
+    {0.9238795325112867, 1.7645047084336677e-17, -0.3826834323650898, 1.0050772696461588e-17},
+    {0.9039892931234433, -6.609754468748431e-18, -0.4275550934302821, -9.411189816295473e-18},
+    {0.881921264348355, -1.9843248405890562e-17, -0.47139673682599764, -6.516678136069013e-18},
+    {0.8577286100002721, -4.818344793633662e-17, -0.5141027441932218, 4.5712707523615624e-17},
+    {0.8314696123025452, 1.4073856984728024e-18, -0.5555702330196022, -4.709410940561677e-17},
+    {0.8032075314806449, -3.306060980481491e-17, -0.5956993044924334, 1.3438641936579467e-17},
+    {0.773010453362737, -3.256590703364977e-17, -0.6343932841636455, -1.0420901929280035e-17},
+    {0.7409511253549591, -1.4708616952297345e-17, -0.6715589548470184, 4.048903774929669e-17},
+    {0.7071067811865476, -4.833646656726457e-17, -0.7071067811865476, 4.833646656726457e-17},
+    {0.6715589548470184, -4.048903774929669e-17, -0.7409511253549591, 1.4708616952297345e-17},
+    {0.6343932841636455, 1.0420901929280035e-17, -0.773010453362737, 3.256590703364977e-17},
+    {0.5956993044924334, -1.3438641936579467e-17, -0.8032075314806449, 3.306060980481491e-17},
+    {0.5555702330196022, 4.709410940561677e-17, -0.8314696123025452, -1.4073856984728024e-18},
+    {0.5141027441932218, -4.5712707523615624e-17, -0.8577286100002721, 4.818344793633662e-17},
+    {0.47139673682599764, 6.516678136069013e-18, -0.881921264348355, 1.9843248405890562e-17},
+    {0.4275550934302821, 9.411189816295473e-18, -0.9039892931234433, 6.609754468748431e-18},
+    {0.3826834323650898, -1.0050772696461588e-17, -0.9238795325112867, -1.7645047084336677e-17},
+    {0.33688985339222005, -4.200094003347509e-19, -0.9415440651830208, 2.789637954769834e-17},
+    {0.2902846772544624, -1.892797870777425e-17, -0.9569403357322088, -4.05538698618757e-17},
+    {0.2429801799032639, -8.751431529719663e-18, -0.970031253194544, -1.8365300348428844e-17},
+    {0.19509032201612828, -7.991079068461731e-18, -0.9807852804032304, -1.8546939997825006e-17},
+    {0.14673047445536175, 3.726947147046568e-18, -0.989176509964781, 4.098730993704711e-17},
+    {0.0980171403295606, -1.634582362244256e-18, -0.9951847266721969, 4.248691367830441e-17},
+    {0.049067674327418015, -6.79610372051828e-19, -0.9987954562051724, 1.2291693337075465e-17},
+    {0.0, 0.0, -1.0, 0.0},
+};
 
-BIA_HDF void sincos_small(double b, double* s, double* c) {
-  const double z = b * b;
-  double p = kSin5;
-  p = p * z + kSin4;
-  p = p * z + kSin3;
-  p = p * z + kSin2;
-  p = p * z + kSin1;
-  p = p * z + kSin0;
-  *s = b + (b * z) * p;
-  double q = kCos5;
-  q = q * z + kCos4;
-  q = q * z + kCos3;
-  q = q * z + kCos2;
-  q = q * z + kCos1;
-  q = q * z + kCos0;
-  const double hz = 0.5 * z, w = 1.0 - hz;
-  // 1 - z/2 rounded once, then its rounding error and the polynomial tail (Muller, sec. 11.3)
-  *c = w + (((1.0 - w) - hz) + (z * z) * q);
-}
-
-// the library routines for the arguments outside the ranges above, kept out of line (rare)
+// the library routines for the arguments outside the ranges here, kept out of line (rare)
 BIA_HDN double2_ sincos_lib(double x) {
   double2_ r;
   ::sincos(x, &r.x, &r.y);
@@ -206,9 +248,7 @@ BIA_HDN double2_ sincos_lib(double x) {
 BIA_HDN double log_lib(double x) { return ::log(x); }
 
 BIA_HDF void sincos_pi(double x, double* s, double* c) {
-  constexpr double kPiHi = 3.141592653589793116, kPiLo = 1.2246467991473532e-16;
-  constexpr double kPio2Hi = 1.5707963267948966, kPio2Lo = 6.123233995736766e-17;
-  constexpr double kPio4 = 0.78539816339744830962, k3Pio4 = 2.35619449019234492885;
+  constexpr double kPiHi = 3.141592653589793116;
   const double a = fabs(x);
   if (!(a <= kPiHi)) {  // beyond pi, NaN: the library routine
     const double2_ r = sincos_lib(x);
@@ -216,46 +256,41 @@ BIA_HDF void sincos_pi(double x, double* s, double* c) {
     *c = r.y;
     return;
   }
-  // one reduction to |b| <= pi/4: a itself, a - pi/2 or pi - a; the difference with the
-  // constant's high part is exact, and adding its low part rounds once: that rounding, bl, is
-  // recovered exactly (Fast2Sum) and applied to first order, sin(b + bl) = sin b + bl cos b
-  const bool mid = a > kPio4 && a <= k3Pio4, top = a > k3Pio4;
-  const double bh = mid ? a - kPio2Hi : top ? kPiHi - a : a;
-  const double lo = mid ? -kPio2Lo : top ? kPiLo : 0.0;
-  const double b = bh + lo, bl = (bh - b) + lo;
-  double sb, cb;
-  sincos_small(b, &sb, &cb);
-  sb = fma(bl, cb, sb);
-  cb = fma(-bl, sb, cb);
-  // sin a = sin b, cos b, sin b; cos a = cos b, -sin b, -cos b
-  const double sa = mid ? cb : sb, ca = mid ? -sb : top ? -cb : cb;
+  const double kd = rint(a * k64oPi);
+  const double t = fma(-kd, kPio64Hi, a);  // exact
+  const double rh = fma(-kd, kPio64Mid, t);
+  const double rl = fma(-kd, kPio64Lo, fma(-kd, kPio64Mid, t - rh));
+#if defined(__HIP_DEVICE_COMPILE__)
+  // the entry by two raw buffer loads at a 32-bit offset (fewer address instructions than a 64-bit
+  // pointer; the descriptor's size bounds them: an offset past the table reads zeros)
+  typedef int int4_v __attribute__((ext_vector_type(4)));
+  typedef double double2_v __attribute__((ext_vector_type(2)));
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void*)kSinCosTab, 0, sizeof(kSinCosTab), 0x00020000);
+  const unsigned off = (unsigned)kd * (unsigned)sizeof(SinCosEntry);
+  const double2_v e0 = __builtin_bit_cast(double2_v, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+  const double2_v e1 = __builtin_bit_cast(double2_v, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0));
+  const SinCosEntry E{e0.x, e0.y, e1.x, e1.y};
+#else
+  const SinCosEntry E = kSinCosTab[(unsigned)kd];
+#endif
+  const double z = rh * rh;
+  // the innermost steps as a multiply and an add, each with one constant operand (no constant
+  // moved into a vector register)
+  double p = z * (-1.0 / 5040), q = z * (-1.0 / 720);
+  p = p + 1.0 / 120;
+  q = q + 1.0 / 24;
+  const double pz = fma(rh * z, fma(z, p, -1.0 / 6), rl);  // sin r - rh
+  const double cm = z * fma(z, q, -0.5);                   // cos r - 1
+  // the leading terms S + C rh (C - S rh) in one fused multiply-add, whose rounding error is
+  // recovered by a second one (S - head is exact: |C rh| <= S / 2 and |S rh| <= |C| / 2 for
+  // k >= 1, and head = rh, 1 for k = 0), then added with the rest
+  const double hs = fma(E.c_hi, rh, E.s_hi), hc = fma(-E.s_hi, rh, E.c_hi);
+  const double es = fma(E.c_hi, rh, E.s_hi - hs), ec = fma(-E.s_hi, rh, E.c_hi - hc);
+  const double sa = hs + (fma(E.c_hi, pz, fma(E.s_hi, cm, E.s_lo)) + es);
+  const double ca = hc + (fma(-E.s_hi, pz, fma(E.c_hi, cm, E.c_lo)) + ec);
   *s = copysign(sa, x);
   *c = ca;
-}
-
-// sincos_pi for |x| <= 3 pi / 4, the half anomalies of the hyperbola's residual (|x| < pi / 2
-// inside the branch): one reduction case fewer, the same results there; beyond, the library
-// routine (iterates far outside the branch, whose residual is NaN in any case)
-BIA_HDF void sincos_half(double x, double* s, double* c) {
-  constexpr double kPio2Hi = 1.5707963267948966, kPio2Lo = 6.123233995736766e-17;
-  constexpr double kPio4 = 0.78539816339744830962, k3Pio4 = 2.35619449019234492885;
-  const double a = fabs(x);
-  if (!(a <= k3Pio4)) {
-    const double2_ r = sincos_lib(x);
-    *s = r.x;
-    *c = r.y;
-    return;
-  }
-  const bool mid = a > kPio4;
-  const double bh = mid ? a - kPio2Hi : a;
-  const double lo = mid ? -kPio2Lo : 0.0;
-  const double b = bh + lo, bl = (bh - b) + lo;
-  double sb, cb;
-  sincos_small(b, &sb, &cb);
-  sb = fma(bl, cb, sb);
-  cb = fma(-bl, sb, cb);
-  *s = copysign(mid ? cb : sb, x);
-  *c = mid ? -sb : cb;
 }
 
 // natural logarithm of a positive normal finite x in FP64 (Muller, 2016: x = 2^k m with m in
@@ -371,7 +406,7 @@ BIA_HDF double hyp_newton_at(double th, double sh, double ch, double snp, double
 
 BIA_HDF double hyp_newton(double th, double TE, const HypP& P, double* err, double* dfd) {
   double sh, ch;
-  sincos_half(0.5 * th, &sh, &ch);
+  sincos_pi(0.5 * th, &sh, &ch);
   const double snp = P.s0h * ch + P.c0h * sh;
   const double snm = P.s0h * ch - P.c0h * sh;
   return hyp_newton_at(th, sh, ch, snp, snm, log_pos(snp * fast_rcp(snm)), TE, P, err, dfd);
@@ -403,7 +438,7 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   S.P.sn0 = div_r(aL, den, rden);
   S.P.cs0 = div_r(bg, den, rden);
   S.P.tn0 = div_r(S.P.sn0, S.P.cs0, fast_rcp(S.P.cs0));
-  sincos_half(0.5 * S.P.th0, &S.P.s0h, &S.P.c0h);
+  sincos_pi(0.5 * S.P.th0, &S.P.s0h, &S.P.c0h);
   // the argument of acos with correctly rounded operations (div_r: the reference's divisions),
   // since acos near 1 amplifies its rounding (find_dXYUV_kernel.cu:119-126); the rest, the
   // half-angle sines and cosines of theta_0 and theta_ini included (|theta_0 / 2|,
@@ -421,7 +456,7 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   }
   if (L * r_g < 0) th_ini = -th_ini;
   double sh, ch;
-  sincos_half(0.5 * th_ini, &sh, &ch);
+  sincos_pi(0.5 * th_ini, &sh, &ch);
   double sn = 2 * sh * ch;
   double x = r * cs, y = r * sn;
   const double gs = g0 * fast_rcp(S.P.sn0);

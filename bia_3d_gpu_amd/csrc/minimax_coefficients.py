@@ -1,17 +1,20 @@
-"""Coefficients of the plain-mode kernel's range-specific sin, cos and log (bia_pair.hpp,
-sincos_small and log_pos), computed for this library by the Remez exchange algorithm in 60-digit
-arithmetic (mpmath).  Run it to print the C constants the header holds:
+"""Constants of the plain-mode kernel's range-specific sin, cos and log (bia_pair.hpp): the
+coefficients of log_pos's polynomial, computed for this library by the Remez exchange algorithm in
+60-digit arithmetic (mpmath), and sincos_pi's table of sin and cos at k pi / 64 with its reduction
+constants.  Run it to print the C constants the header holds:
 
     python3 bia_3d_gpu_amd/csrc/minimax_coefficients.py
 
-Each polynomial p of degree n in z minimises max |w(z) (p(z) - f(z))| on [0, zmax]:
+sincos_pi: a = |x| <= pi is reduced to r = a - k pi/64, |r| <= pi/128, k = rint(a 64/pi), with
+pi/64 = P1 + P2 + P3 (P1 to 46 bits, so k P1 is exact for k <= 64); sin a and cos a follow from the
+table's sin and cos of k pi/64, each as a pair of doubles (hi + lo to 2^-106), and the Taylor
+polynomials of sin r and cos r - 1 (through r^7 and r^6: the first terms left out are below 4e-19
+and 4e-18 of the result).
 
-* sin b = b + b^3 p_s(b^2), |b| <= pi/4:  f_s(z) = (sin(sqrt z) - sqrt z) / z^(3/2), w = z
-  (the error of sin b relative to b);
-* cos b = 1 - b^2/2 + b^4 p_c(b^2), |b| <= pi/4:  f_c(z) = (cos(sqrt z) - 1 + z/2) / z^2,
-  w = z^2;
+The log polynomial p of degree n in z minimises max |w(z) (p(z) - f(z))| on [0, zmax]:
+
 * ln m = 2 atanh(u) = 2u + 2u v p_l(v), v = u^2, |u| <= (sqrt 2 - 1)/(sqrt 2 + 1):
-  f_l(v) = (atanh(sqrt v) - sqrt v) / v^(3/2), w = v.
+  f_l(v) = (atanh(sqrt v) - sqrt v) / v^(3/2), w = v (the error relative to 2u).
 
 The exchange: start from Chebyshev points, solve p(x_i) - f(x_i) = (-1)^i E / w(x_i) at n + 2
 points, move the points to the alternating extrema of the weighted error (dense sampling, then
@@ -95,19 +98,6 @@ def golden_max(g, lo, hi, steps=80):
     return (lo + hi) / 2
 
 
-def f_sin(z):
-    if z == 0:
-        return mp.mpf(-1) / 6
-    r = mp.sqrt(z)
-    return (mp.sin(r) - r) / (z * r)
-
-
-def f_cos(z):
-    if z == 0:
-        return mp.mpf(1) / 24
-    return (mp.cos(mp.sqrt(z)) - 1 + z / 2) / (z * z)
-
-
 def f_log(v):
     if v == 0:
         return mp.mpf(1) / 3
@@ -126,11 +116,30 @@ def check(f, w, a, b, coef):
     return worst
 
 
+def sincos_table():
+    """sin and cos of k pi / 64, k = 0..64, each as hi + lo (hi the double nearest, lo the double
+    nearest the remainder), and the reduction constants"""
+    pi64 = mp.pi / 64
+    e = int(mp.floor(mp.log(pi64, 2)))
+    scale = mp.mpf(2) ** (45 - e)
+    p1 = mp.floor(pi64 * scale) / scale  # 46 significant bits
+    assert mp.mpf(float(p1)) == p1
+    p2 = float(pi64 - p1)
+    p3 = float(pi64 - p1 - p2)
+    print(f"constexpr double kPio64Hi = {float(p1)!r}, kPio64Mid = {p2!r}, kPio64Lo = {p3!r},")
+    print(f"                 k64oPi = {float(64 / mp.pi)!r};")
+    print("constexpr SinCosEntry kSinCosTab[65] = {")
+    for k in range(65):
+        s, c = mp.sinpi(mp.mpf(k) / 64), mp.cospi(mp.mpf(k) / 64)
+        sh, ch = float(s), float(c)
+        print(f"    {{{sh!r}, {float(s - sh)!r}, {ch!r}, {float(c - ch)!r}}},")
+    print("};")
+
+
 def main():
-    zs = (mp.pi / 4) ** 2
+    sincos_table()
     us = (mp.sqrt(2) - 1) / (mp.sqrt(2) + 1)
-    for name, f, w, b, n in (("sin", f_sin, lambda z: z, zs, 5), ("cos", f_cos, lambda z: z * z, zs, 5),
-                             ("log", f_log, lambda v: v, us * us, 6)):
+    for name, f, w, b, n in (("log", f_log, lambda v: v, us * us, 6),):
         coef, E = remez(f, w, mp.mpf(0), b, n)
         worst = check(f, w, mp.mpf(0), b, coef)
         print(f"// {name}: degree {n}, weighted minimax error {mp.nstr(abs(E), 3)}, "

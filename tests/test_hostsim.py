@@ -397,11 +397,11 @@ def _ulp_error(y, ref):
 
 def test_range_specific_sincos_and_log():
     """The plain mode's sin/cos of |x| <= pi and log of positive normal x (bia_pair.hpp
-    sincos_pi, sincos_half, log_pos; host build of the same source) against 80-bit extended-precision values:
-    within 1.1 ulp for sin and cos and 1.5 ulp for log, on random arguments and on the points
-    where the range reduction switches (pi/4, 3pi/4) or cancels (near pi/2, pi, and log near 1).
-    A wrong reduction constant, a dropped series term or an uncorrected quotient would show as
-    many ulps."""
+    sincos_pi, log_pos; host build of the same source) against 80-bit extended-precision values:
+    within 0.7 ulp for sin and cos and 1.5 ulp for log, on random arguments, on the points where
+    the table index switches ((k + 1/2) pi/64) or the reduction cancels (near k pi/64, pi/2, pi),
+    and log near 1.  A wrong table entry or reduction constant, a dropped series term, the leading
+    term's rounding left uncorrected (1.5 ulp) or an uncorrected quotient would show."""
     L = C.CDLL(LIB)
     dp = np.ctypeslib.ndpointer(np.float64, flags="C")
     L.hs_sincos_pi.argtypes = [C.c_int64, dp, dp, dp]
@@ -410,7 +410,9 @@ def test_range_specific_sincos_and_log():
         pytest.skip("no 80-bit long double on this host")
     rng = np.random.default_rng(2016)
     pi = np.pi
-    edges = (np.array([0.25, 0.5, 0.75, 1.0]) * pi * (1 + rng.uniform(-1e-6, 1e-6, (2000, 1)))).ravel()
+    grid = np.concatenate([np.arange(1, 65), np.arange(0, 64) + 0.5]) * (pi / 64)
+    edges = np.concatenate([(np.array([0.25, 0.5, 0.75, 1.0]) * pi * (1 + rng.uniform(-1e-6, 1e-6, (2000, 1)))).ravel(),
+                            (grid * (1 + rng.uniform(-1e-9, 1e-9, (300, 1)))).ravel()])
     x = np.concatenate([rng.uniform(-pi, pi, 400000), edges, -edges, 10.0 ** rng.uniform(-300, 0, 20000),
                         np.array([0.0, pi, -pi, pi / 2, pi / 4, 3 * pi / 4])])
     x = np.ascontiguousarray(x[np.abs(x) <= pi])
@@ -419,20 +421,7 @@ def test_range_specific_sincos_and_log():
     xl = x.astype(np.longdouble)
     es, ec = _ulp_error(s, np.sin(xl)), _ulp_error(c, np.cos(xl))
     print(f"sincos_pi on {x.size} arguments: max {es.max():.3f} ulp (sin), {ec.max():.3f} ulp (cos)")
-    assert es.max() < 1.1 and ec.max() < 1.1
-    # the half-angle variant: the same results up to 3 pi / 4, the library sincos's beyond (within
-    # an ulp of the library sin and cos)
-    L.hs_sincos_half.argtypes = [C.c_int64, dp, dp, dp]
-    sh, ch = np.zeros_like(x), np.zeros_like(x)
-    L.hs_sincos_half(x.size, x, sh, ch)
-    inner = np.abs(x) <= 0.75 * pi
-    assert inner.sum() > 200000 and (~inner).sum() > 1000
-    assert np.array_equal(sh[inner], s[inner]) and np.array_equal(ch[inner], c[inner])
-    import math
-    xo = x[~inner].tolist()
-    ms, mc = np.array([math.sin(v) for v in xo]), np.array([math.cos(v) for v in xo])
-    assert (np.abs(sh[~inner] - ms) <= np.spacing(np.abs(ms))).all()
-    assert (np.abs(ch[~inner] - mc) <= np.spacing(np.abs(mc))).all()
+    assert es.max() < 0.7 and ec.max() < 0.7
     y = np.concatenate([10.0 ** rng.uniform(-307, 307, 200000), 1 + rng.uniform(-0.3, 0.3, 200000),
                         1 + 10.0 ** rng.uniform(-15, -1, 20000), 1 - 10.0 ** rng.uniform(-15, -1, 20000),
                         np.array([1.0, 2.0, 0.5, np.sqrt(2.0), np.sqrt(0.5), 1e-300, 1e300])])
@@ -528,10 +517,11 @@ def test_reference_solvers_with_learned_starts_mechanics(hs, fixture1332):
     assert wrong.sum() == 5
     assert (_quad_error(rel[wrong], eps, dt, b[wrong]) < 1e-10).all()
     # elsewhere the two agree to the reference formulas' own rounding: weak hyperbolic pairs carry
-    # ~1e-8 of it in their position departure (both modes alike against the quad solution)
+    # 1e-8 to 1e-7 of it in their position departure (both modes alike against the quad solution:
+    # the largest difference, 1.1e-7, is a pair both of whose results are 4e-8 and 6e-8 off it)
     ok = ~wrong
     d = np.abs(a[ok] - b[ok]).max(axis=1) / np.abs(a[ok]).max(axis=1)
-    assert (d <= 1e-7).all()
+    assert (d <= 3e-7).all()
     far = d > 1e-9
     if far.any():
         qa, qb = _quad_error(rel[ok][far], eps, dt, a[ok][far]), _quad_error(rel[ok][far], eps, dt, b[ok][far])
