@@ -337,6 +337,111 @@ BIA_HDF double log_pos(double x) {
   return dk * kLn2Hi + (dk * kLn2Lo + (2 * u + (2 * u * v) * p));
 }
 
+// arctangent and arc cosine for the hyperbola's set-up (theta_0, the reference's guess theta_gss and
+// theta_ini), written for this kernel from the textbook reductions (Muller, 2016, ch. 11): minimax
+// polynomials atan v = v + v z p_t(z), z = v^2, on |v| <= tan(pi/8) (degree 10, below 2.6e-18
+// relative with the coefficients rounded to double) and asin w = w + w z p_a(z) on 0 <= w <= 1/2
+// (degree 12, below 2.1e-18), coefficients by the Remez exchange in 60-digit arithmetic
+// (csrc/minimax_coefficients.py); within 2.3 ulp (atan2: the reduced argument is the quotient of a
+// rounded difference and sum) and 1.1 ulp (acos) (tests/test_hostsim.py).
+constexpr double kAtan0 = -0.333333333333332;
+constexpr double kAtan1 = 0.19999999999954082;
+constexpr double kAtan2 = -0.1428571428024855;
+constexpr double kAtan3 = 0.11111110786139472;
+constexpr double kAtan4 = -0.09090897836489972;
+constexpr double kAtan5 = 0.07692061637766001;
+constexpr double kAtan6 = -0.06663120390587496;
+constexpr double kAtan7 = 0.05848008961100218;
+constexpr double kAtan8 = -0.05039845700820873;
+constexpr double kAtan9 = 0.0380783255473377;
+constexpr double kAtan10 = -0.01792228415811652;
+constexpr double kAsin0 = 0.16666666666666763;
+constexpr double kAsin1 = 0.07499999999970261;
+constexpr double kAsin2 = 0.04464285717512157;
+constexpr double kAsin3 = 0.03038194268358697;
+constexpr double kAsin4 = 0.02237221566100598;
+constexpr double kAsin5 = 0.01735160008126213;
+constexpr double kAsin6 = 0.013980961559813486;
+constexpr double kAsin7 = 0.011397775209284457;
+constexpr double kAsin8 = 0.010786532991097898;
+constexpr double kAsin9 = 0.0036830282112930873;
+constexpr double kAsin10 = 0.0217635307706916;
+constexpr double kAsin11 = -0.02107733501652091;
+constexpr double kAsin12 = 0.0326769155536257;
+
+// atan v for |v| <= tan(pi/8)
+BIA_HDF double atan_poly(double v) {
+  const double z = v * v;
+  double p = z * kAtan10;  // the first step as a multiply and an add (no constant in a vector register)
+  p = p + kAtan9;
+  p = p * z + kAtan8;
+  p = p * z + kAtan7;
+  p = p * z + kAtan6;
+  p = p * z + kAtan5;
+  p = p * z + kAtan4;
+  p = p * z + kAtan3;
+  p = p * z + kAtan2;
+  p = p * z + kAtan1;
+  p = p * z + kAtan0;
+  return fma(v * z, p, v);
+}
+
+// atan2(y, x) for y >= 0 (theta in [0, pi]; not y = x = 0): with m = min(y, |x|), M = max(y, |x|),
+// atan(m / M) = v + ... for m <= tan(pi/8) M, else pi/4 + atan((m - M) / (m + M)) (one quotient
+// either way, from a reciprocal with the remainder correction); then pi/2 minus that where y > |x|
+// and pi minus the result where x < 0, the multiples of pi/4 in two parts
+BIA_HDF double atan2_upper(double y, double x) {
+  constexpr double kTanPio8 = 0.41421356237309503;
+  // pi/4 = kQHi + kQLo; the double nearest pi/4 has 50 significant bits, so k kQHi is exact
+  constexpr double kQHi = 0.7853981633974483, kQLo = 3.061616997868383e-17;
+  const double ax = fabs(x);
+  const double mn = fmin(y, ax), mx = fmax(y, ax);
+  const bool big = y > ax, red = mn > kTanPio8 * mx, neg = x < 0;
+  const double num = red ? mn - mx : mn, den = red ? mn + mx : mx;
+  const double a = atan_poly(div_r(num, den, fast_rcp(den)));
+  // theta = k pi/4 + s a: k from (red: 1, big: 2, else 0), reflected to 4 - k for x < 0; s = -1
+  // where exactly one of big and neg holds
+  const int k0 = red ? 1 : big ? 2 : 0;
+  const double k = (double)(neg ? 4 - k0 : k0);
+  const double sa = (big != neg) ? -a : a;
+  return k * kQHi + (k * kQLo + sa);
+}
+
+// asin w for 0 <= w <= 1/2, given z = w^2 (or within an ulp of it)
+BIA_HDF double asin_poly(double w, double z) {
+  double p = z * kAsin12;
+  p = p + kAsin11;
+  p = p * z + kAsin10;
+  p = p * z + kAsin9;
+  p = p * z + kAsin8;
+  p = p * z + kAsin7;
+  p = p * z + kAsin6;
+  p = p * z + kAsin5;
+  p = p * z + kAsin4;
+  p = p * z + kAsin3;
+  p = p * z + kAsin2;
+  p = p * z + kAsin1;
+  p = p * z + kAsin0;
+  return fma(w * z, p, w);
+}
+
+// acos q for |q| <= 1: pi/2 - asin q for |q| <= 1/2; 2 asin(sqrt((1 - |q|)/2)) (1 - |q| exact) for
+// q > 1/2, pi minus it for q < -1/2
+BIA_HDF double acos_unit(double q) {
+  constexpr double kPiHi = 3.141592653589793116, kPiLo = 1.2246467991473532e-16;
+  constexpr double kPio2Hi = 1.5707963267948966, kPio2Lo = 6.123233995736766e-17;
+  const double a = fabs(q);
+  const bool big = a > 0.5;
+  const double zb = 0.5 * (1.0 - a);
+  const double z = big ? zb : a * a;
+  const double w = big ? sqrt(zb) : a;
+  const double as = asin_poly(w, z);
+  const double kh = big ? (q < 0 ? kPiHi : 0.0) : kPio2Hi;
+  const double kl = big ? (q < 0 ? kPiLo : 0.0) : kPio2Lo;
+  const double c = big ? (q < 0 ? -2.0 : 2.0) : (q < 0 ? 1.0 : -1.0);
+  return kh + fma(c, as, kl);
+}
+
 // 1/sqrt(x) from the hardware estimate y (e = 1 - x y^2 near 2^-24) and one third-order
 // correction y (1 + e/2 + 3e^2/8), which leaves 5e^3/16 (within an ulp); the host build uses the
 // library square root
@@ -433,7 +538,7 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   double aL = fabs(L);
   S.b = div_r(aL, g0, rg0);
   double bg = b0 * g0;
-  S.P.th0 = atan2(aL, bg);
+  S.P.th0 = atan2_upper(aL, bg);
   const double den = sqrt(bg * bg + L * L), rden = fast_rcp(den);
   S.P.sn0 = div_r(aL, den, rden);
   S.P.cs0 = div_r(bg, den, rden);
@@ -451,7 +556,7 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
     cs = cos(th_ini);
     *fl |= kAcosClamp;
   } else {
-    th_ini = acos(q);
+    th_ini = acos_unit(q);
     cs = q;
   }
   if (L * r_g < 0) th_ini = -th_ini;
@@ -473,7 +578,8 @@ BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, u
   double time_end = (L > 0) ? time_ini + dt : time_ini - dt;
   S.TE = time_end * g0 * fast_rcp(b0);
   S.th_ini = th_ini;
-  S.th_gss = 2 / kPi * S.P.th0 * atan(S.TE * fast_rcp(S.P.tn0));
+  // atan(TE / tn0) from the exact ratio: atan2 of the magnitudes, with the ratio's sign
+  S.th_gss = 2 / kPi * S.P.th0 * copysign(atan2_upper(fabs(S.TE), fabs(S.P.tn0)), S.TE * S.P.tn0);
   // the guess selection's evaluation at theta_ini (:176) from the half-angle sine and cosine, the
   // sines and the logarithm above: the values hyp_newton(th_ini) forms, without forming them twice
   double d;

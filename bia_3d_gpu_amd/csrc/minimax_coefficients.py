@@ -1,7 +1,7 @@
 """Constants of the plain-mode kernel's range-specific sin, cos and log (bia_pair.hpp): the
 coefficients of log_pos's polynomial, computed for this library by the Remez exchange algorithm in
-60-digit arithmetic (mpmath), and sincos_pi's table of sin and cos at k pi / 64 with its reduction
-constants.  Run it to print the C constants the header holds:
+60-digit arithmetic (mpmath), the same for the arctangent and arcsine polynomials of atan_ratio and
+acos_ref, and sincos_pi's table of sin and cos at k pi / 64 with its reduction constants.  Run it to print the C constants the header holds:
 
     python3 bia_3d_gpu_amd/csrc/minimax_coefficients.py
 
@@ -11,10 +11,14 @@ table's sin and cos of k pi/64, each as a pair of doubles (hi + lo to 2^-106), a
 polynomials of sin r and cos r - 1 (through r^7 and r^6: the first terms left out are below 4e-19
 and 4e-18 of the result).
 
-The log polynomial p of degree n in z minimises max |w(z) (p(z) - f(z))| on [0, zmax]:
+Each polynomial p of degree n in z minimises max |w(z) (p(z) - f(z))| on [0, zmax]:
 
 * ln m = 2 atanh(u) = 2u + 2u v p_l(v), v = u^2, |u| <= (sqrt 2 - 1)/(sqrt 2 + 1):
-  f_l(v) = (atanh(sqrt v) - sqrt v) / v^(3/2), w = v (the error relative to 2u).
+  f_l(v) = (atanh(sqrt v) - sqrt v) / v^(3/2), w = v (the error relative to 2u);
+* atan v = v + v z p_t(z), z = v^2, |v| <= tan(pi/8): f_t(z) = (atan(sqrt z) - sqrt z) / z^(3/2),
+  w = z (relative to v);
+* asin w = w + w z p_a(z), z = w^2, 0 <= w <= 1/2: f_a(z) = (asin(sqrt z) - sqrt z) / z^(3/2),
+  w = z (relative to w).
 
 The exchange: start from Chebyshev points, solve p(x_i) - f(x_i) = (-1)^i E / w(x_i) at n + 2
 points, move the points to the alternating extrema of the weighted error (dense sampling, then
@@ -98,6 +102,20 @@ def golden_max(g, lo, hi, steps=80):
     return (lo + hi) / 2
 
 
+def f_atan(z):
+    if z == 0:
+        return mp.mpf(-1) / 3
+    r = mp.sqrt(z)
+    return (mp.atan(r) - r) / (z * r)
+
+
+def f_asin(z):
+    if z == 0:
+        return mp.mpf(1) / 6
+    r = mp.sqrt(z)
+    return (mp.asin(r) - r) / (z * r)
+
+
 def f_log(v):
     if v == 0:
         return mp.mpf(1) / 3
@@ -139,7 +157,9 @@ def sincos_table():
 def main():
     sincos_table()
     us = (mp.sqrt(2) - 1) / (mp.sqrt(2) + 1)
-    for name, f, w, b, n in (("log", f_log, lambda v: v, us * us, 6),):
+    t8 = mp.tan(mp.pi / 8)
+    for name, f, w, b, n in (("log", f_log, lambda v: v, us * us, 6), ("atan", f_atan, lambda z: z, t8 * t8, 10),
+                             ("asin", f_asin, lambda z: z, mp.mpf(1) / 4, 12)):
         coef, E = remez(f, w, mp.mpf(0), b, n)
         worst = check(f, w, mp.mpf(0), b, coef)
         print(f"// {name}: degree {n}, weighted minimax error {mp.nstr(abs(E), 3)}, "

@@ -31,6 +31,12 @@ extern "C" void hs_pairs(int method, int64_t n, double eps, double dt, int itr, 
 extern "C" void hs_sincos_pi(int64_t n, const double* x, double* s, double* c) {
   for (int64_t k = 0; k < n; k++) bia::sincos_pi(x[k], s + k, c + k);
 }
+extern "C" void hs_atan2_upper(int64_t n, const double* y, const double* x, double* out) {
+  for (int64_t k = 0; k < n; k++) out[k] = bia::atan2_upper(y[k], x[k]);
+}
+extern "C" void hs_acos_unit(int64_t n, const double* q, double* out) {
+  for (int64_t k = 0; k < n; k++) out[k] = bia::acos_unit(q[k]);
+}
 extern "C" void hs_log_pos(int64_t n, const double* x, double* y) {
   for (int64_t k = 0; k < n; k++) y[k] = bia::log_pos(x[k]);
 }

@@ -57,7 +57,8 @@ def test_failure_log_records_the_capped_pairs(ctx):
     pairs the statistics count with the cap, non-finite or RKF failure bits, writes one record per
     pair (each i < j, no pair twice) up to its capacity, and the oracle (the same algorithm on the
     CPU) reports the cap on most recorded pairs; the others are pairs whose best residual sits
-    within a few rounding units of the tolerance, where one libm rounding decides.  The Kepler
+    within a few rounding units of the tolerance, where one libm rounding decides, and a few
+    ill-conditioned ones (at most 2 % of the sample, residual below 1e-12).  The Kepler
     drift logs none on the same box, and a log smaller than the count keeps counting."""
     import torch
     from bia_3d_gpu_amd import COUPLING_COULOMB, MODE_GUESS, MODE_PLAIN, STATS_LEN, FailureLog, ic
@@ -89,12 +90,17 @@ def test_failure_log_records_the_capped_pairs(ctx):
     # 2.22e-16 for the hyperbola, dimensionless)
     capped_rec = rec[(rec["flags"] & 4) != 0]
     assert (np.abs(capped_rec["error"]) >= 2e-16).all()
-    # the oracle on a sample of the recorded pairs: the same branch; it stops at the cap too, or
-    # the pair is one whose best residual sits within a few rounding units of the tolerance
-    # (most of this box's capped pairs: whether the cap is reached there is decided by one libm
-    # rounding of the residual)
-    capped = near = 0
-    sample = rec[:: max(1, len(rec) // 200)]
+    # the oracle on a sample of the recorded pairs (in (i, j) order: the log's own order follows
+    # the order the workgroups finish in): the same branch; it stops at the cap too, or the pair is
+    # one whose best residual sits within a few rounding units of the tolerance (most of this box's
+    # capped pairs: whether the cap is reached there is decided by one libm rounding of the
+    # residual), or, for a few in a thousand, an ill-conditioned pair (near-parabolic and
+    # repulsive, e = 1.006 for the pair 158-1697) whose loop ends by a cycle in the oracle and at the
+    # cap here, both with a residual near the rounding of the time equation's terms (5e-14 here,
+    # 2e-15 for the reference's CPU routine)
+    capped = near = hard = 0
+    rs = np.sort(rec, order=["i", "j"])
+    sample = rs[:: max(1, len(rs) // 200)]
     for r in sample:
         i, j = int(r["i"]), int(r["j"])
         rel = st[:, i] - st[:, j]
@@ -103,13 +109,15 @@ def test_failure_log_records_the_capped_pairs(ctx):
         assert r["branch"] == br
         if fl & O.F_ITR_CAP:
             capped += 1
-        else:
-            assert abs(r["error"]) < 8 * 2.22e-16, r
+        elif abs(r["error"]) < 8 * 2.22e-16:
             near += 1
-    print(f"oracle reports the cap on {capped} of {len(sample)} recorded pairs; the other {near} have best "
-          f"residuals within 8 rounding units of the tolerance; recorded residuals "
+        else:
+            assert abs(r["error"]) < 1e-12, r
+            hard += 1
+    print(f"oracle reports the cap on {capped} of {len(sample)} recorded pairs; {near} others have best "
+          f"residuals within 8 rounding units of the tolerance, {hard} are ill-conditioned; recorded residuals "
           f"{np.abs(rec['error']).min():.2e} .. {np.abs(rec['error']).max():.2e}")
-    assert capped >= len(sample) // 2
+    assert capped >= len(sample) // 2 and hard <= len(sample) // 50
     t_small, rec_small, _ = res["plain_small"]
     assert t_small == total and len(rec_small) == 100
     t_k, rec_k, stats_k = res["kepler"]

@@ -434,6 +434,47 @@ def test_range_specific_sincos_and_log():
     assert el.max() < 1.5 and (out[~nz] == 0).all()
 
 
+def test_range_specific_atan2_and_acos():
+    """The plain mode's set-up inverses (bia_pair.hpp atan2_upper for y >= 0, acos_unit for
+    |q| <= 1; host build of the same source) against 80-bit extended precision: within 2.5 ulp
+    (atan2: the reduced argument is the quotient of a rounded difference and sum; 2.3 measured)
+    and 1.1 ulp (acos),
+    on random arguments over many binades and both signs of x, at the reduction's switch points
+    (y/|x| = tan(pi/8), y = |x|; |q| = 1/2) and near the ends (q = +-1, y << |x|, y >> |x|).  A
+    wrong polynomial coefficient, quadrant constant or reflection would show as many ulps."""
+    L = C.CDLL(LIB)
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    L.hs_atan2_upper.argtypes = [C.c_int64, dp, dp, dp]
+    L.hs_acos_unit.argtypes = [C.c_int64, dp, dp]
+    if np.finfo(np.longdouble).nmant < 63:
+        pytest.skip("no 80-bit long double on this host")
+    rng = np.random.default_rng(1974)
+    n = 200000
+    y = 10.0 ** rng.uniform(-8, 8, n)
+    x = 10.0 ** rng.uniform(-8, 8, n) * rng.choice([-1.0, 1.0], n)
+    t8 = np.tan(np.pi / 8)
+    ye = np.abs(x[:40000]) * np.concatenate([t8 * (1 + rng.uniform(-1e-9, 1e-9, 20000)),
+                                             1 + rng.uniform(-1e-9, 1e-9, 20000)])
+    y = np.ascontiguousarray(np.concatenate([y, ye, np.abs(x[:2000]) * 1e-12, np.abs(x[:2000]) * 1e12]))
+    x = np.ascontiguousarray(np.concatenate([x, x[:40000], x[:2000], x[:2000]]))
+    out = np.zeros_like(y)
+    L.hs_atan2_upper(y.size, y, x, out)
+    ref = np.arctan2(y.astype(np.longdouble), x.astype(np.longdouble))
+    ea = _ulp_error(out, ref)
+    q = np.concatenate([rng.uniform(-1, 1, 200000), 0.5 * (1 + rng.uniform(-1e-9, 1e-9, 10000)),
+                        -0.5 * (1 + rng.uniform(-1e-9, 1e-9, 10000)), 1 - 10.0 ** rng.uniform(-16, -1, 10000),
+                        -1 + 10.0 ** rng.uniform(-16, -1, 10000), np.array([1.0, -1.0, 0.0, 0.5, -0.5])])
+    q = np.ascontiguousarray(q)
+    oq = np.zeros_like(q)
+    L.hs_acos_unit(q.size, q, oq)
+    refq = np.arccos(q.astype(np.longdouble))
+    nz = refq != 0
+    eq = _ulp_error(oq[nz], refq[nz])
+    print(f"atan2_upper on {y.size} arguments: max {ea.max():.3f} ulp; acos_unit on {q.size}: max {eq.max():.3f} "
+          f"ulp, acos(1) = {oq[~nz]}")
+    assert ea.max() < 2.5 and eq.max() < 1.1 and (oq[~nz] == 0).all()
+
+
 def test_quotient_from_a_shared_reciprocal():
     """The device's quotient a / b from a reciprocal r of b (bia_pair.hpp div_r: q0 = a r, then
     q0 + (a - q0 b) r with both steps fused multiply-adds), emulated here in exact rational
