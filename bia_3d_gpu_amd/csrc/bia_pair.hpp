@@ -455,21 +455,36 @@ BIA_HDF double fast_rsqrt(double x) {
 #endif
 }
 
+// sqrt(x) given y = fast_rsqrt(x): s0 = x y, then one correction with the exact remainder x - s0^2
+// (Markstein, 2000), the correctly rounded square root unless it lies within about 2^-53 ulp of a
+// rounding boundary; y serves as the reciprocal of the root for div_r.  x = 0 gives NaN (the
+// square root gives 0).  The host build uses the library square root.
+BIA_HDF double sqrt_r(double x, double y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double s0 = x * y;
+  return fma(fma(-s0, s0, x), 0.5 * y, s0);
+#else
+  (void)y;
+  return sqrt(x);
+#endif
+}
+
 // ------------------------------------------------------------------------------------
 // orbital-plane frame, find_dXYUV_kernel.cu:467-488:
 //   e_eta = g/|g|, e_zeta = (r x g)/|r x g|, e_xi = e_eta x e_zeta,
 //   yz = (r.e_xi, r.e_eta, 0, |g|)
 // ------------------------------------------------------------------------------------
-// The quotients as the reference's divisions (div_r: correctly rounded, one reciprocal per divisor).
+// The quotients and square roots as the reference's (div_r, sqrt_r: correctly rounded, one
+// reciprocal square root per norm, which is also the reciprocal its quotients take).
 BIA_HDF void frame(const double R[3], const double G[3], double exi[3], double eeta[3], double yz[4]) {
-  const double gm = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]), rgm = fast_rcp(gm);
+  const double g2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2], rgm = fast_rsqrt(g2), gm = sqrt_r(g2, rgm);
   eeta[0] = div_r(G[0], gm, rgm);
   eeta[1] = div_r(G[1], gm, rgm);
   eeta[2] = div_r(G[2], gm, rgm);
   double L0 = R[1] * G[2] - R[2] * G[1];
   double L1 = R[2] * G[0] - R[0] * G[2];
   double L2 = R[0] * G[1] - R[1] * G[0];
-  const double Lm = sqrt(L0 * L0 + L1 * L1 + L2 * L2), rLm = fast_rcp(Lm);
+  const double L2s = L0 * L0 + L1 * L1 + L2 * L2, rLm = fast_rsqrt(L2s), Lm = sqrt_r(L2s, rLm);
   double z0 = div_r(L0, Lm, rLm), z1 = div_r(L1, Lm, rLm), z2 = div_r(L2, Lm, rLm);
   exi[0] = eeta[1] * z2 - eeta[2] * z1;
   exi[1] = eeta[2] * z0 - eeta[0] * z2;
@@ -529,17 +544,17 @@ struct HypState {
 // set-up shared by both modes: find_dXYUV_kernel.cu:98-172
 BIA_HDF void hyp_setup(double eps, double dt, const double yz[4], HypState& S, unsigned* fl) {
   double X = yz[0], Y = yz[1], U = yz[2], V = yz[3];
-  const double r = sqrt(X * X + Y * Y), rr = fast_rcp(r);
+  const double r2 = X * X + Y * Y, rr = fast_rsqrt(r2), r = sqrt_r(r2, rr);
   double W = U * U + V * V + div_r(2 * eps, r, rr);
   double r_g = X * U + Y * V;
   double L = X * V - Y * U;
-  const double g0 = sqrt(W), rg0 = fast_rcp(g0);
+  const double rg0 = fast_rsqrt(W), g0 = sqrt_r(W, rg0);
   double b0 = div_r(eps, W, fast_rcp(W));
   double aL = fabs(L);
   S.b = div_r(aL, g0, rg0);
   double bg = b0 * g0;
   S.P.th0 = atan2_upper(aL, bg);
-  const double den = sqrt(bg * bg + L * L), rden = fast_rcp(den);
+  const double den2 = bg * bg + L * L, rden = fast_rsqrt(den2), den = sqrt_r(den2, rden);
   S.P.sn0 = div_r(aL, den, rden);
   S.P.cs0 = div_r(bg, den, rden);
   S.P.tn0 = div_r(S.P.sn0, S.P.cs0, fast_rcp(S.P.cs0));
@@ -1608,7 +1623,8 @@ BIA_HDF void pair_eval(double eps, double dt, int itr, const double rel[6], doub
       *branch = kRkf;
       rkf_pair(eps, dt, yz, yn, &it, fl, tol.rel, tol.abs, resid);
     } else {
-      const double W = sqrt(yz[0] * yz[0] + yz[1] * yz[1]) * (yz[2] * yz[2] + yz[3] * yz[3]) + 2 * eps;
+      const double r2 = yz[0] * yz[0] + yz[1] * yz[1];
+      const double W = sqrt_r(r2, fast_rsqrt(r2)) * (yz[2] * yz[2] + yz[3] * yz[3]) + 2 * eps;
       if (W > 0) {
         *branch = kHyperbola;
         HypState S;

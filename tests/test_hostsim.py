@@ -481,7 +481,8 @@ def test_quotient_from_a_shared_reciprocal():
     arithmetic with one rounding per fused operation: it equals the correctly rounded quotient for
     r the correctly rounded 1/b and for r one ulp off either way (the device's fast_rcp is within an
     ulp), over operands spread across 120 binades, and on quotients that are exact or lie next to
-    a rounding boundary.  Dropping the correction (q0 alone) fails on about a quarter of them."""
+    a rounding boundary.  Dropping the correction (q0 alone) fails on about a quarter of them.
+    The same for the square root from a reciprocal square root (sqrt_r)."""
     import math
     from fractions import Fraction as F
 
@@ -507,6 +508,18 @@ def test_quotient_from_a_shared_reciprocal():
             bad += fma(fma(-q0, y, x), r, q0) != x / y
     print(f"{3 * a.size} quotients: {bad} not correctly rounded; without the correction {plain} of {a.size}")
     assert bad == 0 and plain > a.size // 10
+    # the square root from a reciprocal square root y (sqrt_r: s0 = x y, then s0 + (x - s0^2) y / 2):
+    # correctly rounded for y the rounded 1/sqrt(x) and one ulp off either way
+    x = rng.uniform(0.5, 2, 6000) * 2.0 ** rng.integers(-80, 80, 6000)
+    bad = plain = 0
+    for v in x.tolist():
+        y0 = 1.0 / math.sqrt(v)
+        plain += v * y0 != math.sqrt(v)
+        for y in (y0, math.nextafter(y0, math.inf), math.nextafter(y0, -math.inf)):
+            s0 = v * y
+            bad += fma(fma(-s0, s0, v), 0.5 * y, s0) != math.sqrt(v)
+    print(f"{3 * x.size} square roots: {bad} not correctly rounded; without the correction {plain} of {x.size}")
+    assert bad == 0 and plain > x.size // 10
 
 
 def _binary_pairs(n=65536, seed=65536):
