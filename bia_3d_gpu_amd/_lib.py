@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbia_hip.so")
 
-BIA_OK, BIA_ERR_ARG, BIA_ERR_HIP, BIA_ERR_OOM, BIA_ERR_STATE, BIA_ERR_RKF = 0, 1, 2, 3, 4, 5
+BIA_OK, BIA_ERR_ARG, BIA_ERR_HIP, BIA_ERR_OOM, BIA_ERR_STATE, BIA_ERR_RKF, BIA_ERR_IO = 0, 1, 2, 3, 4, 5, 6
 METHOD_RKF, METHOD_ALG = 0, 1
 MODE_PLAIN, MODE_LEARN, MODE_GUESS, MODE_PLAIN_LEARN = 0, 1, 2, 3
 STATS_LEN = 16
@@ -82,6 +82,10 @@ SIGNATURES = {
                                      C.c_double, _dptr, _vp, _ip, _ip, _ip, _ip]),
     "bia_dim_rkf45_coupled_host": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, _vp, _vp, _dptr, C.c_double, _dp,
                                              _ip, C.c_double, C.c_double, _dptr, _vp, _ip, _ip, _ip, _ip]),
+    "bia_records_scan": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "bia_record_read": (C.c_int, [C.c_char_p, C.c_int, C.c_longlong, _vp, _vp, C.c_longlong,
+                                  C.POINTER(C.c_longlong)]),
+    "bia_record_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _vp, _vp, C.c_longlong, C.c_longlong]),
 }
 
 _lib = None

@@ -263,7 +263,6 @@ BIA_HDF void sincos_pi(double x, double* s, double* c) {
 #if defined(__HIP_DEVICE_COMPILE__)
   // the entry by two raw buffer loads at a 32-bit offset (fewer address instructions than a 64-bit
   // pointer; the descriptor's size bounds them: an offset past the table reads zeros)
-  typedef int int4_v __attribute__((ext_vector_type(4)));
   typedef double double2_v __attribute__((ext_vector_type(2)));
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc((void*)kSinCosTab, 0, sizeof(kSinCosTab), 0x00020000);

@@ -69,6 +69,7 @@ const char* bia_error_string(int status) {
     case BIA_ERR_OOM: return "device out of memory";
     case BIA_ERR_STATE: return "invalid call sequence";
     case BIA_ERR_RKF: return "RKF45 integration failed";
+    case BIA_ERR_IO: return "record file unreadable, unwritable or malformed";
     default: return "unknown status";
   }
 }

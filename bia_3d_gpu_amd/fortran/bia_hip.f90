@@ -4,7 +4,7 @@
 module bia_hip
   use iso_c_binding
   implicit none
-  integer(c_int), parameter :: BIA_OK = 0, BIA_METHOD_RKF = 0, BIA_METHOD_ALG = 1
+  integer(c_int), parameter :: BIA_OK = 0, BIA_ERR_IO = 6, BIA_METHOD_RKF = 0, BIA_METHOD_ALG = 1
   integer(c_int), parameter :: BIA_MODE_PLAIN = 0, BIA_MODE_LEARN = 1, BIA_MODE_GUESS = 2, BIA_MODE_PLAIN_LEARN = 3
   integer(c_int), parameter :: BIA_STATS_LEN = 16
   integer(c_int), parameter :: BIA_COUPLING_GRAVITY = 1, BIA_COUPLING_COULOMB = 2
@@ -92,6 +92,33 @@ module bia_hip
       real(c_double), value :: coup, xe, relerr, abserr
       real(c_double) :: x0, y(*), h, ese(*)
       integer(c_int) :: init, nde, naccept, nreject, ier
+    end function
+
+    ! the reference's record files (.bbb: nhead 0; .DIM, .BIA: nhead 1), for callers that do not
+    ! read them themselves: count the records, read one into head(nhead) and soa(ld, 6), write one
+    ! (append 0 truncates, 1 appends).  path: a NUL-terminated string, trim(name)//c_null_char.
+    integer(c_int) function bia_records_scan(path, nhead, nrec, nmax) bind(C, name='bia_records_scan')
+      import :: c_int, c_char, c_long_long
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int), value :: nhead
+      integer(c_long_long) :: nrec, nmax
+    end function
+
+    integer(c_int) function bia_record_read(path, nhead, rec, head, soa, ld, n) bind(C, name='bia_record_read')
+      import :: c_int, c_char, c_long_long, c_double
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int), value :: nhead
+      integer(c_long_long), value :: rec, ld
+      real(c_double) :: head(*), soa(*)
+      integer(c_long_long) :: n
+    end function
+
+    integer(c_int) function bia_record_write(path, append, nhead, head, soa, ld, n) bind(C, name='bia_record_write')
+      import :: c_int, c_char, c_long_long, c_double
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int), value :: append, nhead
+      real(c_double), intent(in) :: head(*), soa(*)
+      integer(c_long_long), value :: ld, n
     end function
   end interface
 end module bia_hip

@@ -37,6 +37,7 @@ extern "C" {
 #define BIA_ERR_OOM 3     /* device allocation failed */
 #define BIA_ERR_STATE 4   /* call sequence error (e.g. learned mode without trained weights) */
 #define BIA_ERR_RKF 5     /* DIM RKF45 failure; the reference's IER code is returned separately */
+#define BIA_ERR_IO 6      /* record file cannot be opened, read or written, or is malformed */
 
 /* method codes (reference METHOD values, bia/bia.f90:135-139) */
 #define BIA_METHOD_RKF 0  /* per-pair adaptive RKF45 */
@@ -325,6 +326,28 @@ int bia_mlp_build_table(bia_ctx *ctx, int n);
  * exit without the cap (hyperbola) or whose root of Kepler's equation bisection finds (ellipse). */
 int bia_record_samples_dev(bia_ctx *ctx, int n, double eps, double dt, const double *d_state, int ld, int nsamples,
                            unsigned long long seed, float *d_out);
+
+/* ---- the reference's record files (data plane; host only, no context) ----
+ * Fortran unformatted sequential files with 4-byte little-endian record markers, as the
+ * reference's programs read and write them: .bbb, one record of 6N doubles per realisation,
+ * particle-major x, y, z, u, v, w (IC/bbb_maker_REV.f90:71; the N_calc loop of bia/bia.f90:62-64
+ * reads one per pass; nhead = 0); .DIM, records [t, 6N] (dim/dim_3D.f90:32; nhead = 1); .BIA,
+ * records [dt, 6N deltas] (bia/bia.f90:206; nhead = 1).  Arrays on the caller's side are host
+ * memory in the step entry points' layout (six rows of stride ld).  No counterpart in the
+ * reference's C ABI: its Fortran reads and writes these files itself; these are for C and C++
+ * callers.  BIA_ERR_IO: the file cannot be opened, read or written, or a record is malformed (a
+ * marker that is negative, does not fit the file or differs from its closing marker, or a payload
+ * that is not nhead doubles and whole particles).
+ * bia_records_scan: number of records and the largest particle count among them.
+ * bia_record_read: record `rec` (0-based; BIA_ERR_ARG past the last) into head[nhead] and
+ *   soa[6][ld] (ld >= its particle count); with soa NULL only *n is returned.
+ * bia_record_write: one record of n particles, truncating the file (append 0) or appending (1);
+ *   at most (2^31 - 1 - 8 nhead) / 48 particles (the marker's limit). */
+int bia_records_scan(const char *path, int nhead, long long *nrec, long long *nmax);
+int bia_record_read(const char *path, int nhead, long long rec, double *head, double *soa, long long ld,
+                    long long *n);
+int bia_record_write(const char *path, int append, int nhead, const double *head, const double *soa, long long ld,
+                     long long n);
 
 #ifdef __cplusplus
 }
