@@ -134,8 +134,8 @@ def test_benchmark_system_65536_against_rkf45_and_quad(ctx):
        rows: both members of the three hardest binaries, the particles where the Kepler drift and
        the reference solvers disagree most with RKF45, the first and the last particle and two
        more.  The Kepler drift, with and without the learned start, is within 1e-12 normwise in
-       positions and velocities (max |x - x_quad| / max |x_quad| over the system, SURVEY's
-       definition) and within 1e-11 of each row's Kepler correction (the sum of its pair deltas,
+       positions and velocities (max |x - x_quad| / max |x_quad| over the system, the
+       normwise error of the accuracy bar) and within 1e-11 of each row's Kepler correction (the sum of its pair deltas,
        the part of the delta that is not drift).  The reference solvers (plain mode) are
        reported: they stop at their iteration cap on some binary pairs and are off there."""
     import torch
